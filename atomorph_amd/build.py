@@ -1,0 +1,104 @@
+"""Builds the native libraries in-tree (no JIT cache, nothing installed).
+
+  lib/libatomorph_hip.so  hand-written HIP kernels + C ABI, gfx950 only
+  lib/libatomorph.so      C++ am::morph host engine over the C ABI
+  lib/atomorph_cli        CLI driver of am::morph (PNG-free, raw RGBA)
+
+Run `python -m atomorph_amd.build` or __graft_entry__.build().
+"""
+import glob
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HOST = os.path.join(HERE, "host")
+LIB = os.path.join(HERE, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+
+ARCH = "gfx950"
+# -ffp-contract=off keeps the FP64 colour / Perlin / spline arithmetic in the
+# reference's rounding order; packed FP32 ops are disabled for every kernel.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-pthread"]
+
+
+def _hipcc():
+    for c in ("/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build_hip(verbose=False):
+    os.makedirs(LIB, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    out = os.path.join(LIB, "libatomorph_hip.so")
+    tmp = out + ".tmp"
+    # compile objects in parallel, then link
+    objdir = os.path.join(LIB, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    procs = []
+    objs = []
+    for s in srcs:
+        o = os.path.join(objdir, os.path.basename(s) + ".o")
+        objs.append(o)
+        cmd = [_hipcc()] + HIP_FLAGS + ["-I", INCLUDE, "-c", s, "-o", o]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    for cmd, p in procs:
+        text, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"build failed: {' '.join(cmd)}\n{text}")
+        if verbose and text.strip():
+            print(text)
+    _run([_hipcc()] + HIP_FLAGS + ["-shared", "-o", tmp] + objs)
+    os.replace(tmp, out)
+    return out
+
+
+def build_host(verbose=False):
+    srcs = sorted(glob.glob(os.path.join(HOST, "*.cpp")))
+    if not srcs:
+        return None
+    os.makedirs(LIB, exist_ok=True)
+    lib_srcs = [s for s in srcs if not os.path.basename(s).startswith("cli")]
+    out = os.path.join(LIB, "libatomorph.so")
+    _run(["g++"] + CXX_FLAGS + ["-shared", "-I", INCLUDE, "-I", HOST, "-o", out + ".tmp"] + lib_srcs +
+         ["-L", LIB, "-latomorph_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(out + ".tmp", out)
+    cli = [s for s in srcs if os.path.basename(s).startswith("cli")]
+    if cli:
+        exe = os.path.join(LIB, "atomorph_cli")
+        _run(["g++"] + CXX_FLAGS + ["-I", INCLUDE, "-I", HOST, "-o", exe] + cli +
+             ["-L", LIB, "-latomorph", "-latomorph_hip", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
+def build_oracle(verbose=False):
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir])
+    if os.path.isdir("/root/reference"):
+        _run(["make", "-s", "-C", odir, "ref"])
+
+
+def build_all(verbose=False):
+    build_hip(verbose)
+    build_host(verbose)
+    build_oracle(verbose)
+
+
+if __name__ == "__main__":
+    build_all(verbose="-v" in sys.argv)
+    print("built:", ", ".join(sorted(os.listdir(LIB))))

@@ -1,0 +1,330 @@
+// Optimal-transport refinement: the reference's random pair-swap descent
+// (thread::morph / morph_asynch, thread.cpp:989-1068) on gfx950.
+//
+// Acceptance.  The reference swaps p[x1][y] and p[x2][y] when c1 >= c2
+// (ties accepted), with c1/c2 the four squared distances to rows y-1 and
+// y+1 before/after (thread.cpp:1014-1022; exact integers summed in double).
+// Writing C for a slot's point in row y and S = P + N for the sum of its
+// neighbours in rows y-1 and y+1 (fixed while row y is refined), every
+// squared norm cancels and
+//        c1 - c2 = -2 (C1 - C2) . (S1 - S2),
+// so a swap is taken iff (C1 - C2).(S1 - S2) <= 0.  With 16.8 fixed point
+// (|C| < 2^24, |S| < 2^25) the dot product fits int64 exactly, so every
+// decision equals the reference's.
+//
+// Schedule.  A round is a random (near-)perfect matching of the slots of
+// row y: the fold {i, c - i mod w} for a random c (round_params in
+// common.h).  Pairs of a round are disjoint and only read the neighbour
+// rows, so all of them run in parallel with no conflicts and the round
+// equals the same attempts made one after another.
+#include "common.h"
+#include "../../include/atomorph_hip.h"
+
+#include <algorithm>
+#include <vector>
+
+using namespace amh;
+
+namespace {
+
+constexpr int kCostThreads = 256;
+constexpr int kRoundThreads = 256;
+
+// ------------------------------------------------------------ chain cost
+__global__ __launch_bounds__(kCostThreads) void k_chain_cost(const uint64_t *__restrict__ pts, int h, int64_t wtot,
+                                                             const int64_t *__restrict__ off,
+                                                             const int32_t *__restrict__ cw, int nchains,
+                                                             unsigned long long *cost) {
+    const int64_t a = int64_t(blockIdx.x) * kCostThreads + threadIdx.x;
+    const bool valid = a < wtot;
+    int c = 0;
+    uint64_t v = 0;
+    if (valid) {
+        int lo = 0, hi = nchains - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= a) lo = mid; else hi = mid - 1;
+        }
+        c = lo;
+        if (cw[c] > 1) {
+            uint64_t prev = pts[int64_t(h - 1) * wtot + a];
+            for (int j = 0; j < h; ++j) {
+                uint64_t cur = pts[int64_t(j) * wtot + a];
+                v += pt_dist(prev, cur);
+                prev = cur;
+            }
+        }
+    }
+    const int c0 = __shfl(c, 0, 64);
+    const bool uniform = __all(!valid || c == c0);
+    if (uniform) {
+        uint64_t s = wave_sum_u64(v);
+        if ((threadIdx.x & 63) == 0 && s)
+            atomic_add_u128(&cost[2 * c0], &cost[2 * c0 + 1], s);
+    } else if (valid && v) {
+        atomic_add_u128(&cost[2 * c], &cost[2 * c + 1], v);
+    }
+}
+
+__device__ __forceinline__ void flush_stats(amh_refine_stats *st, uint64_t att, uint64_t acc, int64_t gain) {
+    att = wave_sum_u64(att);
+    acc = wave_sum_u64(acc);
+    gain = wave_sum_i64(gain);
+    if ((threadIdx.x & 63) == 0 && att) {
+        atomicAdd((unsigned long long *)&st->attempts, (unsigned long long)att);
+        atomicAdd((unsigned long long *)&st->accepted, (unsigned long long)acc);
+        atomicAdd((unsigned long long *)&st->gain, (unsigned long long)gain);
+    }
+}
+
+// One pair of one round.  Returns the signed dot product d; the caller swaps
+// when d <= 0 (c1 >= c2) and records a gain of -2d.
+__device__ __forceinline__ int64_t swap_test(uint64_t ci, uint64_t cj, int32_t sxi, int32_t syi, int32_t sxj,
+                                             int32_t syj) {
+    return int64_t(pt_fx(ci) - pt_fx(cj)) * int64_t(sxi - sxj) +
+           int64_t(pt_fy(ci) - pt_fy(cj)) * int64_t(syi - syj);
+}
+
+// -------------------------------------------- plain: one round, one chain
+__global__ __launch_bounds__(kRoundThreads) void k_refine_round(uint64_t *__restrict__ pts, int h, int64_t wtot,
+                                                                int64_t off, int32_t w, int32_t y, RoundPar rp,
+                                                                amh_refine_stats *st) {
+    const uint32_t t = blockIdx.x * kRoundThreads + threadIdx.x;
+    uint64_t att = 0, acc = 0;
+    int64_t gain = 0;
+    if (t < rp.npairs) {
+        uint32_t i, j;
+        round_pair(rp, uint32_t(w), t, i, j);
+        {
+            const int yp = y > 0 ? y - 1 : h - 1, yn = (y + 1) % h;
+            uint64_t *row = pts + int64_t(y) * wtot + off;
+            const uint64_t *rprev = pts + int64_t(yp) * wtot + off;
+            const uint64_t *rnext = pts + int64_t(yn) * wtot + off;
+            const uint64_t ci = row[i], cj = row[j];
+            const uint64_t pi = rprev[i], pj = rprev[j], ni = rnext[i], nj = rnext[j];
+            const int64_t d = swap_test(ci, cj, pt_fx(pi) + pt_fx(ni), pt_fy(pi) + pt_fy(ni), pt_fx(pj) + pt_fx(nj),
+                                        pt_fy(pj) + pt_fy(nj));
+            att = 1;
+            if (d <= 0) {
+                row[i] = cj;
+                row[j] = ci;
+                acc = 1;
+                gain = -2 * d;
+            }
+        }
+    }
+    flush_stats(st, att, acc, gain);
+}
+
+// ------------------------------------- batched: persistent, LDS resident
+// queue[0] = next item, queue[1] = workgroups finished; the last workgroup
+// to finish resets both, so the kernel can be replayed (hipGraph) without a
+// memset node.
+template <int CAP, int NT>
+__global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, int h, int64_t wtot,
+                                                    const int64_t *__restrict__ off, const int32_t *__restrict__ cw,
+                                                    const int32_t *__restrict__ items, int nitems, uint64_t pass,
+                                                    int rounds, uint64_t seed, uint32_t *queue,
+                                                    amh_refine_stats *st) {
+    __shared__ uint64_t C[CAP];
+    __shared__ int32_t SX[CAP], SY[CAP];
+    __shared__ int item_s;
+    const int tid = threadIdx.x;
+    uint64_t att = 0, acc = 0;
+    int64_t gain = 0;
+    for (;;) {
+        if (tid == 0) item_s = int(atomicAdd(&queue[0], 1u));
+        __syncthreads();
+        const int it = item_s;
+        if (it >= nitems) break;
+        const int c = items[2 * it], g = items[2 * it + 1];
+        const int32_t w = cw[c];
+        const int64_t o = off[c];
+        const int y = int(pass_row(seed, uint64_t(c), pass, uint32_t(h)));
+        const int yp = y > 0 ? y - 1 : h - 1, yn = (y + 1) % h;
+        const int ng = (w + CAP - 1) / CAP;
+        const int gs = (w + ng - 1) / ng;
+        const int g0 = g * gs;
+        const int n = min(gs, w - g0);
+        const bool permuted = ng > 1;
+        const uint64_t pkey = pass_stream(uint64_t(c), pass, 0xFFFFFFFFull);
+        const uint64_t *rprev = pts + int64_t(yp) * wtot + o;
+        const uint64_t *rnext = pts + int64_t(yn) * wtot + o;
+        uint64_t *row = pts + int64_t(y) * wtot + o;
+        for (int l = tid; l < n; l += NT) {
+            const uint32_t s = permuted ? feistel_perm(uint32_t(g0 + l), uint32_t(w), pkey) : uint32_t(l);
+            const uint64_t p = rprev[s], q = rnext[s];
+            C[l] = row[s];
+            SX[l] = pt_fx(p) + pt_fx(q);
+            SY[l] = pt_fy(p) + pt_fy(q);
+        }
+        __syncthreads();
+        if (n >= 2) {
+            const uint64_t sid = pass_stream(uint64_t(c), pass, uint64_t(g));
+            for (int r = 0; r < rounds; ++r) {
+                const RoundPar rp = round_params(seed, sid, uint64_t(r), uint32_t(n));
+                for (uint32_t t = tid; t < rp.npairs; t += NT) {
+                    uint32_t i, j;
+                    round_pair(rp, uint32_t(n), t, i, j);
+                    const uint64_t ci = C[i], cj = C[j];
+                    const int64_t d = swap_test(ci, cj, SX[i], SY[i], SX[j], SY[j]);
+                    ++att;
+                    if (d <= 0) {
+                        C[i] = cj;
+                        C[j] = ci;
+                        ++acc;
+                        gain += -2 * d;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int l = tid; l < n; l += NT) {
+            const uint32_t s = permuted ? feistel_perm(uint32_t(g0 + l), uint32_t(w), pkey) : uint32_t(l);
+            row[s] = C[l];
+        }
+        __syncthreads();
+    }
+    flush_stats(st, att, acc, gain);
+    if (tid == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&queue[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+        }
+    }
+}
+
+struct TileVariant {
+    int cap, threads;
+    void (*fn)(uint64_t *, int, int64_t, const int64_t *, const int32_t *, const int32_t *, int, uint64_t, int,
+               uint64_t, uint32_t *, amh_refine_stats *);
+};
+const TileVariant kTiles[] = {
+    {1024, 256, k_refine_pass<1024, 256>},  {2048, 256, k_refine_pass<2048, 256>},
+    {4096, 256, k_refine_pass<4096, 256>},  {4096, 512, k_refine_pass<4096, 512>},
+    {8192, 512, k_refine_pass<8192, 512>},  {8192, 1024, k_refine_pass<8192, 1024>},
+};
+constexpr int kNumTiles = int(sizeof(kTiles) / sizeof(kTiles[0]));
+int g_tile_grid[kNumTiles] = {0};
+
+int tile_grid(int tile) {
+    if (g_tile_grid[tile] == 0) {
+        int dev = 0, ncu = 0, occ = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(kTiles[tile].fn),
+                                                         kTiles[tile].threads, 0) != hipSuccess)
+            return 0;
+        g_tile_grid[tile] = std::max(1, ncu * std::max(occ, 1));
+    }
+    return g_tile_grid[tile];
+}
+
+}  // namespace
+
+extern "C" int amh_chain_cost(const uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
+                              const int32_t *chain_w_dev, int32_t nchains, uint64_t *cost_dev, void *stream) {
+    if (nkeys <= 0 || wtot < 0 || nchains <= 0 || !pts_dev || !chain_off_dev || !chain_w_dev || !cost_dev) {
+        set_error("amh_chain_cost: invalid argument");
+        return AMH_E_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipError_t e = hipMemsetAsync(cost_dev, 0, sizeof(uint64_t) * 2 * size_t(nchains), s);
+    if (e != hipSuccess) return check_hip(e, "amh_chain_cost memset");
+    if (wtot == 0) return AMH_OK;
+    const int64_t blocks = (wtot + kCostThreads - 1) / kCostThreads;
+    hipLaunchKernelGGL(k_chain_cost, dim3(uint32_t(blocks)), dim3(kCostThreads), 0, s, pts_dev, nkeys, wtot,
+                       chain_off_dev, chain_w_dev, nchains, reinterpret_cast<unsigned long long *>(cost_dev));
+    AMH_LAUNCH_CHECK("amh_chain_cost");
+    return AMH_OK;
+}
+
+extern "C" int amh_refine_chain(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, int64_t chain_off, int32_t chain_w,
+                                int32_t y, uint64_t rounds, uint64_t seed, uint64_t stream_id,
+                                amh_refine_stats *stats_dev, void *stream) {
+    if (!pts_dev || !stats_dev || nkeys <= 0 || chain_w < 0 || chain_off < 0 || chain_off + chain_w > wtot ||
+        y < 0 || y >= nkeys) {
+        set_error("amh_refine_chain: invalid argument (keys %d, off %lld, w %d, wtot %lld, y %d)", nkeys,
+                  (long long)chain_off, chain_w, (long long)wtot, y);
+        return AMH_E_ARG;
+    }
+    // One key frame: prev == next == the row itself, swaps never change the
+    // cost; nothing to refine.  Chains of width < 2 have no pairs.
+    if (nkeys < 2 || chain_w < 2) return AMH_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (uint64_t r = 0; r < rounds; ++r) {
+        const RoundPar rp = round_params(seed, stream_id, r, uint32_t(chain_w));
+        if (rp.npairs == 0) continue;
+        const uint32_t blocks = (rp.npairs + kRoundThreads - 1) / kRoundThreads;
+        hipLaunchKernelGGL(k_refine_round, dim3(blocks), dim3(kRoundThreads), 0, s, pts_dev, nkeys, wtot, chain_off,
+                           chain_w, y, rp, stats_dev);
+    }
+    AMH_LAUNCH_CHECK("amh_refine_chain");
+    return AMH_OK;
+}
+
+extern "C" int amh_refine_tiles(int32_t *count, int32_t *capacity, int32_t *threads, int32_t max_count) {
+    if (count) *count = kNumTiles;
+    for (int i = 0; i < kNumTiles && i < max_count; ++i) {
+        if (capacity) capacity[i] = kTiles[i].cap;
+        if (threads) threads[i] = kTiles[i].threads;
+    }
+    return AMH_OK;
+}
+
+extern "C" int amh_refine_items(const int32_t *chain_w, int32_t nchains, int32_t tile, int32_t *items,
+                                int32_t max_items, int32_t *nitems) {
+    if (!chain_w || !nitems || tile < 0 || tile >= kNumTiles || nchains < 0) {
+        set_error("amh_refine_items: invalid argument");
+        return AMH_E_ARG;
+    }
+    const int cap = kTiles[tile].cap;
+    struct It { int c, g, n; };
+    std::vector<It> v;
+    for (int c = 0; c < nchains; ++c) {
+        const int w = chain_w[c];
+        if (w < 2) continue;
+        const int ng = (w + cap - 1) / cap, gs = (w + ng - 1) / ng;
+        for (int g = 0; g < ng; ++g) v.push_back({c, g, std::min(gs, w - g * gs)});
+    }
+    // Longest work first, so the persistent grid finishes together.
+    std::stable_sort(v.begin(), v.end(), [](const It &a, const It &b) { return a.n > b.n; });
+    *nitems = int32_t(v.size());
+    if (items) {
+        if (int32_t(v.size()) > max_items) {
+            set_error("amh_refine_items: %d items exceed the buffer (%d)", int(v.size()), max_items);
+            return AMH_E_ARG;
+        }
+        for (size_t i = 0; i < v.size(); ++i) {
+            items[2 * i] = v[i].c;
+            items[2 * i + 1] = v[i].g;
+        }
+    }
+    return AMH_OK;
+}
+
+extern "C" int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
+                                  const int32_t *chain_w_dev, const int32_t *items_dev, int32_t nitems, int32_t tile,
+                                  uint64_t pass, int32_t rounds, uint64_t seed, uint32_t *queue_dev,
+                                  amh_refine_stats *stats_dev, void *stream) {
+    if (!pts_dev || !chain_off_dev || !chain_w_dev || !queue_dev || !stats_dev || tile < 0 || tile >= kNumTiles ||
+        nkeys <= 0 || nitems < 0 || rounds < 0) {
+        set_error("amh_refine_batched: invalid argument");
+        return AMH_E_ARG;
+    }
+    if (nkeys < 2 || nitems == 0 || rounds == 0) return AMH_OK;
+    if (!items_dev) {
+        set_error("amh_refine_batched: items_dev is NULL");
+        return AMH_E_ARG;
+    }
+    const int grid = std::min(tile_grid(tile), int(nitems));
+    if (grid <= 0) return check_hip(hipGetLastError(), "amh_refine_batched occupancy");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(kTiles[tile].fn, dim3(grid), dim3(kTiles[tile].threads), 0, s, pts_dev, nkeys, wtot,
+                       chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed, queue_dev,
+                       stats_dev);
+    AMH_LAUNCH_CHECK("amh_refine_batched");
+    return AMH_OK;
+}

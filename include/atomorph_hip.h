@@ -1,0 +1,176 @@
+/*
+ * atomorph_hip.h -- C ABI of the MI355X (gfx950) morph engine.
+ *
+ * The drop-in boundary of the atomorph morph-engine hot path.  The C++ class
+ * am::morph (atomorph_amd/host/morph.h, same interface as the reference's
+ * morph.h:9-76) calls these entry points; any other host language binds them
+ * directly (INTEGRATION.md shows ctypes / cgo / JNI stubs).
+ *
+ * Conventions
+ *  - Every entry point returns AMH_OK (0) or a negative AMH_E_* code and never
+ *    throws.  amh_last_error() returns the calling thread's last message.
+ *  - Buffers named *_dev are device (HBM) pointers; all others are host.
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream).  Work
+ *    is enqueued asynchronously unless the comment says "synchronous".
+ *  - A point is the reference's 8-byte am::point word (atomorph.h:275-284):
+ *      bits 0-15 x, 16-31 y, 32-39 x_fract, 40-47 y_fract, 48-55 flags.
+ *  - Chain storage on the device is key-frame-major: the point of atom a in
+ *    key frame y is pts_dev[y * wtot + a], where atom a = off[c] + x is slot x
+ *    of chain c and wtot = sum of all chain widths.  (The reference keeps
+ *    points[x][y] per chain: atomorph.h:286-298, atomorph.cpp:1085-1157.)
+ *  - Colours are packed RGBA8 words: r | g<<8 | b<<16 | a<<24.
+ */
+#ifndef ATOMORPH_HIP_H
+#define ATOMORPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMH_OK 0
+#define AMH_E_ARG (-1)   /* invalid argument or size                       */
+#define AMH_E_HIP (-2)   /* HIP runtime error (message in amh_last_error)  */
+#define AMH_E_NOMEM (-3) /* device allocation failed                       */
+
+/* ------------------------------------------------------------ plumbing -- */
+/* Library version and the calling thread's last error message. */
+const char *amh_version(void);
+const char *amh_last_error(void);
+/* Device 0 properties (synchronous). */
+int amh_device_info(int32_t *compute_units, uint64_t *hbm_bytes, char *name, size_t name_len);
+int amh_malloc(void **ptr_dev, size_t bytes);
+int amh_free(void *ptr_dev);
+int amh_memcpy_h2d(void *dst_dev, const void *src, size_t bytes, void *stream);
+int amh_memcpy_d2h(void *dst, const void *src_dev, size_t bytes, void *stream);
+int amh_memcpy_d2d(void *dst_dev, const void *src_dev, size_t bytes, void *stream);
+int amh_memset(void *dst_dev, int value, size_t bytes, void *stream);
+int amh_stream_create(void **stream);
+int amh_stream_destroy(void *stream);
+int amh_stream_sync(void *stream);
+/* Event timer on `stream`: amh_timer_begin records a start event into
+ * *timer (allocating it on first use), amh_timer_end records the stop event,
+ * waits for it and returns the elapsed milliseconds. */
+int amh_timer_begin(void **timer, void *stream);
+int amh_timer_end(void *timer, void *stream, float *ms);
+int amh_timer_free(void *timer);
+
+/* -------------------------------------------------------------- ingest -- */
+/* One key frame: per pixel, `stored` = what morph::add_pixel keeps
+ * (rgb_to_hsp when hsp != 0; morph.cpp:298-301, color.cpp:31-44) and
+ * `shown` = what morph::get_pixel returns (hsp_to_rgb of it; morph.cpp:
+ * 378-392, color.cpp:46-59).  Pixels with alpha 0 are absent (the CLI's
+ * fill_morph skips them, demo/main.cpp:96-127) and become 0 in both. */
+int amh_ingest_frame(const uint32_t *rgba_dev, int32_t width, int32_t height, int32_t hsp,
+                     uint32_t *stored_dev, uint32_t *shown_dev, void *stream);
+
+/* -------------------------------------------------------- segmentation -- */
+/* Blob detection at the library default blob_threshold 1.0
+ * (thread::blobify_frame, thread.cpp:225-412): every 4-neighbour pair of
+ * present pixels is mergeable, so blobs are the 4-connected components of
+ * alpha > 0.  labels_dev[i] = smallest row-major index of i's component,
+ * -1 for background.  Needs no workspace. */
+int amh_segment(const uint32_t *rgba_dev, int32_t width, int32_t height, int32_t *labels_dev,
+                void *stream);
+
+/* Per-blob statistics, blobs numbered 0..nblobs-1 in increasing order of
+ * their smallest pixel index.  Replaces the running means the reference
+ * keeps per blob (thread.cpp:235-243, 346-354): sums are exact integers. */
+typedef struct amh_blob_stat {
+    int64_t count;                 /* surface size                         */
+    int64_t sum_x, sum_y;          /* pixel coordinates                    */
+    int64_t sum_c[4];              /* stored colour channels (HSP or RGB)  */
+    int32_t x1, y1, x2, y2;        /* bounding box                         */
+    int64_t first;                 /* smallest row-major pixel index       */
+} amh_blob_stat;
+/* Workspace for amh_blob_count / amh_blob_stats, in bytes. */
+size_t amh_blob_workspace_bytes(int32_t width, int32_t height);
+/* Number of blobs in a labelled frame (synchronous: returns it in *nblobs). */
+int amh_blob_count(const int32_t *labels_dev, int32_t width, int32_t height, void *ws_dev,
+                   int32_t *nblobs, void *stream);
+/* blob_of_pixel_dev[i] = blob number of pixel i or -1; stats_dev[nblobs].
+ * Must follow amh_blob_count on the same ws_dev. */
+int amh_blob_stats(const int32_t *labels_dev, const uint32_t *stored_dev, int32_t width, int32_t height,
+                   void *ws_dev, int32_t nblobs, int32_t *blob_of_pixel_dev, amh_blob_stat *stats_dev,
+                   void *stream);
+/* Stable grouping of pixel indices by blob: pixels of blob b, in row-major
+ * order (the std::set order of blob::surface), land at
+ * pixel_list_dev[blob_first_dev[b] ...], blob_first_dev = exclusive scan of
+ * the counts (nblobs + 1 entries, written here). */
+size_t amh_group_workspace_bytes(int32_t npixels, int32_t nblobs);
+int amh_group_pixels(const int32_t *blob_of_pixel_dev, int32_t npixels, int32_t nblobs, void *ws_dev,
+                     size_t ws_bytes, int64_t *blob_first_dev, int32_t *pixel_list_dev, void *stream);
+
+/* ---------------------------------------------------------- chain build -- */
+/* thread::init_morph (thread.cpp:740-891): row y of chain c holds the pixels
+ * of the chain's blob in key frame y in row-major order (flags HAS_PIXEL |
+ * HAS_FLUID, zero fractions), then duplicates of a shuffled copy with random
+ * fractions and HAS_FLUID cleared; an empty blob contributes one HAS_FLUID
+ * point at its rounded centroid first. */
+typedef struct amh_chain_build {
+    int32_t nkeys;                  /* h                                     */
+    int32_t nchains;
+    int64_t wtot;                   /* sum of widths                         */
+    int32_t frame_width;            /* width of the key-frame images         */
+    int32_t pad_;
+    const int64_t *chain_off_dev;   /* [nchains] first atom of each chain    */
+    const int32_t *chain_w_dev;     /* [nchains] width                       */
+    /* [nchains * nkeys]: blob of chain c in key frame y (-1 = empty blob),
+     * and for empty blobs the centroid pixel (x | y<<16) to use. */
+    const int32_t *chain_blob_dev;
+    const uint32_t *empty_xy_dev;
+    /* per key frame y: pixel lists grouped by blob (amh_group_pixels) */
+    const int32_t *const *pixel_list_dev;   /* host array of nkeys device ptrs */
+    const int64_t *const *blob_first_dev;   /* host array of nkeys device ptrs */
+    uint64_t seed;
+} amh_chain_build;
+int amh_build_chains(const amh_chain_build *args, uint64_t *pts_dev, void *stream);
+
+/* ------------------------------------------------ transport cost + refine */
+/* Exact cost per chain, sum_x sum_y d(p[x][y], p[x][y+1 mod h]) with
+ * point_distance (atomorph.h:334-339; thread::get_energy(chain*),
+ * thread.cpp:1109-1125).  cost_dev[2c] / [2c+1] = low / high 64 bits. */
+int amh_chain_cost(const uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
+                   const int32_t *chain_w_dev, int32_t nchains, uint64_t *cost_dev, void *stream);
+
+/* Counters kept on the device by the refine entry points (accumulated). */
+typedef struct amh_refine_stats {
+    uint64_t attempts;   /* pair swaps evaluated                        */
+    uint64_t accepted;   /* swaps made (c1 >= c2, ties accepted)        */
+    int64_t gain;        /* exact cost decrease                         */
+    uint64_t reserved;
+} amh_refine_stats;
+
+/* Plain descent: one visit of thread::morph / morph_asynch
+ * (thread.cpp:989-1068) to chain c, row y, as `rounds` conflict-free rounds.
+ * Each round is one kernel over a random perfect matching of the chain's
+ * slots; every pair whose swap does not raise the cost is swapped.  Pairs of
+ * one round are disjoint and read rows y-1 / y+1 only, so the round is
+ * exactly a sequence of independent reference attempts.  stream_id selects
+ * the random matchings (see pass_stream in the kernel source). */
+int amh_refine_chain(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, int64_t chain_off, int32_t chain_w,
+                     int32_t y, uint64_t rounds, uint64_t seed, uint64_t stream_id,
+                     amh_refine_stats *stats_dev, void *stream);
+
+/* Batched descent: one pass over all chains in a single persistent launch.
+ * Work items (chain, slot group) are dequeued from an atomic queue in the
+ * order given by items_dev (largest first); each loads its chain's row
+ * y = pass_row(seed, c, pass) into LDS, runs `rounds` rounds there, and
+ * writes the row back.  Chains wider than the tile capacity are split into
+ * groups by a random slot permutation that changes every pass.
+ * items_dev[2*i], [2*i+1] = (chain, group); build it with
+ * amh_refine_items.  `tile` selects the kernel variant (amh_refine_tiles). */
+int amh_refine_tiles(int32_t *count, int32_t *capacity, int32_t *threads, int32_t max_count);
+int amh_refine_items(const int32_t *chain_w, int32_t nchains, int32_t tile, int32_t *items,
+                     int32_t max_items, int32_t *nitems);
+int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
+                       const int32_t *chain_w_dev, const int32_t *items_dev, int32_t nitems,
+                       int32_t tile, uint64_t pass, int32_t rounds, uint64_t seed,
+                       uint32_t *queue_dev, amh_refine_stats *stats_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,31 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+@pytest.fixture(scope="session")
+def golden_functions():
+    return dict(np.load(os.path.join(GOLDEN, "functions.npz")))
+
+
+@pytest.fixture(scope="session")
+def golden_cfg1():
+    return dict(np.load(os.path.join(GOLDEN, "cfg1.npz")))
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle_py
+    return oracle_py

@@ -1,0 +1,250 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled atomorph
+reference (oracle/_ref/libamref.so, built by `make -C oracle ref` from the
+reference sources).  Runs only where the reference source tree is present;
+the fixtures it writes are plain data (inputs and expected outputs).
+
+    python tests/golden/make_golden.py
+
+Fixtures
+  functions.npz  HSP round trips, Perlin tables/values, Catmull-Rom values,
+                 morph_asynch before/after, FluidModel::step before/after.
+  cfg1.npz       the 2 x 64^2 six-ellipse workload (seed 1) through the
+                 reference pipeline driven by iteration counts with
+                 threads = 0: blob membership, chains before and after one
+                 2^16-attempt visit per chain, and 16 rendered frames for
+                 several option sets.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from atomorph_amd import synth  # noqa: E402
+from oracle import oracle_py as op  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+STATE = {0: "BLOB_DETECTION", 1: "BLOB_UNIFICATION", 2: "BLOB_MATCHING", 3: "ATOM_MORPHING", 4: "DONE"}
+
+
+def functions_fixture():
+    r = op.ref()
+    rng = np.random.default_rng(1234)
+    d = {}
+    cols = rng.integers(0, 1 << 32, 4096, dtype=np.uint64).astype(np.uint32)
+    greys = np.array([(v | (v << 8) | (v << 16) | (255 << 24)) for v in range(256)], np.uint32)
+    prim = np.array([0xFF0000FF, 0xFF00FF00, 0xFFFF0000, 0xFFFFFFFF, 0xFF000000, 0x80FFFF00,
+                     0xFF00FFFF, 0xFFFF00FF], np.uint32)
+    cols = np.concatenate([cols, greys, prim])
+    d["hsp_in"] = cols
+    d["rgb_to_hsp"] = np.array([r.ref_rgb_to_hsp(int(c)) for c in cols], np.uint32)
+    d["hsp_to_rgb"] = np.array([r.ref_hsp_to_rgb(int(c)) for c in cols], np.uint32)
+
+    perms = []
+    for s in (0, 1, 2, 3):
+        p = np.zeros(512, np.int32)
+        r.ref_perlin_perm(s, op.ptr(p))
+        perms.append(p)
+    d["perlin_seeds"] = np.array([0, 1, 2, 3], np.int32)
+    d["perlin_perm"] = np.stack(perms)
+    xy = rng.uniform(-2.0, 20.0, (512, 2))
+    d["perlin_xy"] = xy
+    d["perlin_octave8_s1"] = np.array([r.ref_perlin_octave2(1, x, y, 8) for x, y in xy])
+    d["perlin_octave8_s2"] = np.array([r.ref_perlin_octave2(2, x, y, 8) for x, y in xy])
+
+    sp_n, sp_v, sp_t, sp_out = [], [], [], []
+    for i in range(256):
+        n = int(rng.integers(2, 7))
+        v = np.zeros((2, 8))
+        v[:, :n] = rng.uniform(0, 300, (2, n))
+        t = float(rng.uniform(0, 1))
+        ox, oy = C.c_double(), C.c_double()
+        vx = np.ascontiguousarray(v[0]); vy = np.ascontiguousarray(v[1])
+        r.ref_spline_eval(op.ptr(vx), op.ptr(vy), n, t, C.byref(ox), C.byref(oy))
+        sp_n.append(n); sp_v.append(v); sp_t.append(t); sp_out.append((ox.value, oy.value))
+    d["spline_n"] = np.array(sp_n, np.int32)
+    d["spline_pts"] = np.stack(sp_v)
+    d["spline_t"] = np.array(sp_t)
+    d["spline_out"] = np.array(sp_out)
+
+    # morph_asynch on random chains (atom-major points[x][y])
+    cases = []
+    for (w, h, rep, seed) in ((2, 2, 100, 0), (9, 2, 1000, 1), (64, 2, 5000, 2), (50, 3, 5000, 3),
+                              (200, 4, 20000, 4), (333, 2, 65536, 5)):
+        x = rng.integers(0, 128, (w, h)); y = rng.integers(0, 128, (w, h))
+        xf = rng.integers(0, 256, (w, h)); yf = rng.integers(0, 256, (w, h)); fl = rng.integers(0, 4, (w, h))
+        pin = op.pack_points(x, y, xf, yf, fl).ravel()
+        pout = pin.copy()
+        gain = r.ref_morph_asynch(op.ptr(pout), w, h, rep, seed)
+        cases.append((w, h, rep, seed, pin, pout, gain))
+    d["ma_meta"] = np.array([c[:4] for c in cases], np.int64)
+    d["ma_gain"] = np.array([c[6] for c in cases])
+    d["ma_in"] = np.concatenate([c[4] for c in cases])
+    d["ma_out"] = np.concatenate([c[5] for c in cases])
+
+    # FluidModel::step on a 64x64 grid, 300 particles, 3 steps
+    n = 300
+    P = np.zeros((n, 20))
+    P[:, 0] = rng.uniform(3, 60, n); P[:, 1] = rng.uniform(3, 60, n)
+    P[:, 2:4] = rng.normal(0, 0.3, (n, 2))
+    P[:, 4] = rng.uniform(5, 55, n); P[:, 5] = rng.uniform(5, 55, n); P[:, 6] = 1.0
+    P[:, 7:15] = rng.uniform(0, 1, (n, 8)); P[:, 15] = rng.choice([0.1, 1.0], n)
+    P[:, 16] = 1.0; P[:, 17] = rng.integers(0, 2, n)
+    d["fluid_in"] = P.copy()
+    outs = []
+    Q = P.copy()
+    for k in range(3):
+        r.ref_fluid_step(op.ptr(Q), n, 64, 64, 2 - k, 6.0, 0)
+        outs.append(Q.copy())
+    d["fluid_out"] = np.stack(outs)
+    d["fluid_meta"] = np.array([64, 64, 6], np.float64)  # grid x, grid y, freedom radius
+    return d
+
+
+def run_cfg1():
+    r = op.ref()
+    frames = synth.workload("ellipses64")
+    m = r.ref_morph_new()
+    opts = {"seed": 1, "threads": 0, "cycle_length": 1 << 16}
+    for k, v in opts.items():
+        r.ref_morph_set_option(m, k.encode(), float(v))
+    for f, img in enumerate(frames):
+        a = np.ascontiguousarray(img)
+        r.ref_morph_add_rgba(m, f, op.ptr(a), 64, 64)
+    r.ref_morph_set_resolution(m, 64, 64)
+    d = {"frames": np.stack(frames)}
+
+    # blob detection: one worker step at a time until the state moves on
+    steps = 0
+    state = r.ref_morph_run(m, 0)
+    while state == 0:
+        state = r.ref_morph_run(m, 1)
+        steps += 1
+    d["blobify_steps"] = np.int64(steps)
+    # membership after detection (labels = smallest index, like the oracle)
+    labs = []
+    for f in range(2):
+        lab = -np.ones(64 * 64, np.int32)
+        nb = r.ref_morph_blob_count(m, f)
+        for b in range(nb):
+            g = C.c_size_t()
+            cnt = r.ref_morph_blob(m, f, b, C.byref(g), None, None)
+            surf = np.zeros(cnt, np.uint64)
+            r.ref_morph_blob(m, f, b, C.byref(g), None, op.ptr(surf))
+            idx = (surf // 65536) * 64 + (surf % 65536)
+            lab[idx.astype(np.int64)] = idx.min()
+        labs.append(lab)
+    d["labels"] = np.stack(labs)
+
+    # matching: a fixed number of steps, then skip to init_morph
+    while state in (1,):
+        state = r.ref_morph_run(m, 1)
+    match_steps = 2000
+    state = r.ref_morph_run(m, match_steps)
+    r.ref_morph_next_state(m)
+    state = r.ref_morph_run(m, 1)
+    assert state == 3, STATE.get(state)
+    d["match_steps"] = np.int64(match_steps)
+
+    def dump_chains():
+        n = r.ref_morph_chain_count(m)
+        widths, pts = [], []
+        for key in range(n):
+            hh, ms = C.c_size_t(), C.c_size_t()
+            w = r.ref_morph_chain(m, key, C.byref(hh), C.byref(ms), None)
+            p = np.zeros(w * hh.value, np.uint64)
+            r.ref_morph_chain(m, key, C.byref(hh), C.byref(ms), op.ptr(p))
+            widths.append(w)
+            pts.append(p)
+        return np.array(widths, np.int64), np.concatenate(pts)
+
+    w0, p0 = dump_chains()
+    d["chain_w"] = w0
+    d["chain_before"] = p0
+    # blob order and means per frame
+    nb = r.ref_morph_blob_count(m, 0)
+    order = np.zeros((2, nb), np.int32)
+    means = np.zeros((2, nb, 6))
+    sizes = np.zeros((2, nb), np.int64)
+    for f in range(2):
+        for b in range(nb):
+            g = C.c_size_t()
+            mm = np.zeros(6)
+            sizes[f, b] = r.ref_morph_blob(m, f, b, C.byref(g), op.ptr(mm), None)
+            order[f, b] = g.value
+            means[f, b] = mm
+    d["order"] = order
+    d["blob_means"] = means
+    d["blob_sizes"] = sizes
+    bbox = np.zeros(4, np.int32)
+    r.ref_morph_bbox(m, op.ptr(bbox))
+    d["bbox"] = bbox
+
+    # refinement: one visit per chain, cycle_length = 2^16 attempts
+    nchains = len(w0)
+    state = r.ref_morph_run(m, nchains)
+    w1, p1 = dump_chains()
+    assert (w1 == w0).all()
+    d["chain_after"] = p1
+    d["visits"] = np.int64(nchains)
+    r.ref_morph_next_state(m)
+    state = r.ref_morph_run(m, 1)
+    assert state == 4, STATE.get(state)
+
+    # rendered frames from the refined chains, several option sets
+    variants = [
+        ("default", {}),
+        ("linear_cosine", {"motion": 3, "fading": 5}),
+        ("none_linear_bg", {"motion": 2, "fading": 3, "keep_background": 1}),
+        ("spline_perlin_bg_feather2", {"keep_background": 1, "feather": 2}),
+        ("average_blobs", {"show_blobs": 1}),
+        ("distinct_blobs", {"show_blobs": 2, "finite": 1}),
+    ]
+    F = 16
+    renders = {}
+    base = {"motion": 4, "fading": 6, "keep_background": 0, "feather": 0, "show_blobs": 0, "finite": 0}
+    for name, ch in variants:
+        cfg = dict(base)
+        cfg.update(ch)
+        for k, v in cfg.items():
+            r.ref_morph_set_option(m, k.encode(), float(v))
+        imgs = np.zeros((F, 64, 64, 4), np.uint8)
+        ts = np.zeros(F)
+        for i in range(F):
+            t = r.ref_morph_get_time(m, i, F)
+            ts[i] = t
+            r.ref_morph_render(m, t, op.ptr(imgs[i]))
+        renders[name] = (imgs, ts, cfg)
+        d[f"render_{name}"] = imgs
+        d[f"render_{name}_t"] = ts
+        d[f"render_{name}_opts"] = np.array([cfg["motion"], cfg["fading"], cfg["keep_background"],
+                                             cfg["feather"], cfg["show_blobs"], cfg["finite"]], np.int32)
+    d["render_variants"] = np.array([v[0] for v in variants])
+    # what get_pixel returns per key frame (HSP round trip)
+    shown = np.zeros((2, 64 * 64), np.uint32)
+    for f in range(2):
+        for i in range(64 * 64):
+            shown[f, i] = r.ref_morph_get_pixel(m, f, i % 64, i // 64)
+    d["shown"] = shown
+    r.ref_morph_free(m)
+    return d
+
+
+def main():
+    if not op.have_ref():
+        sys.exit("oracle/_ref/libamref.so is missing: run `make -C oracle ref` where the reference exists")
+    f = functions_fixture()
+    np.savez_compressed(os.path.join(OUT, "functions.npz"), **f)
+    c = run_cfg1()
+    np.savez_compressed(os.path.join(OUT, "cfg1.npz"), **c)
+    for name in ("functions.npz", "cfg1.npz"):
+        print(name, os.path.getsize(os.path.join(OUT, name)), "bytes")
+    print("cfg1: blobify steps", int(c["blobify_steps"]), "chains", len(c["chain_w"]),
+          "widths", c["chain_w"].tolist())
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,371 @@
+"""The CPU oracle, pinned before it is trusted as the checker.
+
+Each oracle function is checked against (a) the golden fixtures written from
+the compiled reference (tests/golden/make_golden.py), (b) the live reference
+harness when it has been built in this container, and (c) what the
+mathematics fixes: closed forms, invariants, brute force on tiny inputs.
+"""
+import ctypes as C
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import oracle_py as op
+
+o = op.oracle()
+
+
+def _pts(rng, w, h, lim=128):
+    return op.pack_points(rng.integers(0, lim, (w, h)), rng.integers(0, lim, (w, h)),
+                          rng.integers(0, 256, (w, h)), rng.integers(0, 256, (w, h)),
+                          rng.integers(0, 4, (w, h))).ravel()
+
+
+# ------------------------------------------------------------------ points
+def test_point_distance_closed_form():
+    rng = np.random.default_rng(0)
+    for _ in range(2000):
+        a = [int(v) for v in rng.integers(0, 65536, 2)] + [int(v) for v in rng.integers(0, 256, 2)]
+        b = [int(v) for v in rng.integers(0, 65536, 2)] + [int(v) for v in rng.integers(0, 256, 2)]
+        pa = int(op.pack_points(a[0], a[1], a[2], a[3], 3))
+        pb = int(op.pack_points(b[0], b[1], b[2], b[3], 1))
+        dx = (256 * a[0] + a[2]) - (256 * b[0] + b[2])
+        dy = (256 * a[1] + a[3]) - (256 * b[1] + b[3])
+        assert o.or_point_distance(pa, pb) == dx * dx + dy * dy
+
+
+def test_chain_cost_closed_form():
+    rng = np.random.default_rng(1)
+    for w, h in ((1, 3), (2, 2), (7, 3), (40, 4)):
+        p = _pts(rng, w, h)
+        x = (p & 0xFFFF).astype(np.int64) * 256 + ((p >> 32) & 0xFF).astype(np.int64)
+        y = ((p >> 16) & 0xFFFF).astype(np.int64) * 256 + ((p >> 40) & 0xFF).astype(np.int64)
+        x, y = x.reshape(w, h), y.reshape(w, h)
+        want = 0 if w <= 1 else int(((x - np.roll(x, -1, 1)) ** 2 + (y - np.roll(y, -1, 1)) ** 2).sum())
+        assert op.chain_cost(p, h) == want
+
+
+# ----------------------------------------------------- reference swap descent
+def test_morph_asynch_matches_reference_fixture(golden_functions):
+    g = golden_functions
+    off = 0
+    for (w, h, rep, seed), gain in zip(g["ma_meta"], g["ma_gain"]):
+        n = int(w * h)
+        p = g["ma_in"][off:off + n].copy()
+        gg = C.c_double(0.0)
+        o.or_morph_asynch(op.ptr(p), int(w), int(h), int(rep), int(seed), C.byref(gg))
+        assert (p == g["ma_out"][off:off + n]).all(), (w, h)
+        assert gg.value == gain
+        off += n
+
+
+def test_morph_asynch_gain_is_exact_cost_drop():
+    rng = np.random.default_rng(2)
+    for w, h in ((30, 2), (64, 3), (100, 5)):
+        p = _pts(rng, w, h)
+        before = op.chain_cost(p, h)
+        gg = C.c_double(0.0)
+        o.or_morph_asynch(op.ptr(p), w, h, 20000, 9, C.byref(gg))
+        assert before - op.chain_cost(p, h) == int(gg.value)
+
+
+# ------------------------------------------------------ the GPU's schedule
+def _rows_are_permutations(a, b, w, h, y):
+    a, b = a.reshape(w, h), b.reshape(w, h)
+    for j in range(h):
+        if j == y:
+            assert sorted(a[:, j].tolist()) == sorted(b[:, j].tolist())
+        else:
+            assert (a[:, j] == b[:, j]).all()
+
+
+def test_refine_rounds_invariants():
+    rng = np.random.default_rng(3)
+    for w, h in ((2, 2), (3, 2), (17, 3), (100, 2), (257, 4)):
+        p = _pts(rng, w, h)
+        for y in range(h):
+            before = op.chain_cost(p, h)
+            q, acc, gain, att = op.refine_rounds(p, h, y, 50, seed=11, stream=y)
+            _rows_are_permutations(p, q, w, h, y)
+            after = op.chain_cost(q, h)
+            assert after <= before
+            assert before - after == gain
+            # a fold round pairs floor(w/2) or floor((w-1)/2) slots
+            assert 50 * ((w - 1) // 2) <= att <= 50 * (w // 2)
+            p = q
+
+
+def test_fold_matching_structure():
+    """A fold round {i, c - i mod n} is a matching (disjoint pairs) of
+    floor(n/2) or floor((n-1)/2) pairs, and the n centres together cover
+    every unordered pair exactly once."""
+    for n in (2, 5, 8, 13):
+        allpairs = []
+        for c in range(n):
+            pairs = [(i, (c - i) % n) for i in range(n) if i < (c - i) % n]
+            flat = [v for p in pairs for v in p]
+            assert len(flat) == len(set(flat))
+            assert len(pairs) == (n // 2 if c % 2 else (n - 1) // 2)
+            allpairs += pairs
+        assert sorted(allpairs) == list(itertools.combinations(range(n), 2))
+
+
+def test_refine_rounds_brute_force_local_optimum():
+    """On w <= 7 enough rounds reach a 2-swap local optimum: no single pair
+    swap lowers the cost, and the cost is never below the true optimum."""
+    rng = np.random.default_rng(4)
+    for w in (2, 3, 5, 7):
+        h = 2
+        p = _pts(rng, w, h, lim=32)
+        q, _, _, _ = op.refine_rounds(p, h, 1, 400, seed=5, stream=0)
+        qq = q.reshape(w, h)
+        c = op.chain_cost(q, h)
+        for i, j in itertools.combinations(range(w), 2):
+            s = qq.copy()
+            s[[i, j], 1] = s[[j, i], 1]
+            assert op.chain_cost(s.ravel(), h) >= c
+        best = min(op.chain_cost(np.stack([qq[:, 0], qq[list(perm), 1]], 1).ravel(), h)
+                   for perm in itertools.permutations(range(w)))
+        assert c >= best
+
+
+def test_refine_pass_matches_rounds_for_single_group():
+    """A chain that fits one LDS tile: the batched pass is plain rounds with
+    the pass's row and stream."""
+    rng = np.random.default_rng(5)
+    w, h = 300, 3
+    p = _pts(rng, w, h)
+    q, acc, gain, att = op.refine_pass(p, [0], [w], h, cap=4096, pas=7, rounds=9, seed=21)
+    # find the row that changed
+    changed = [j for j in range(h) if not (q.reshape(w, h)[:, j] == p.reshape(w, h)[:, j]).all()]
+    assert len(changed) <= 1
+    assert op.chain_cost(p, h) - op.chain_cost(q, h) == gain
+
+
+def test_refine_pass_split_groups_invariants():
+    rng = np.random.default_rng(6)
+    widths = [1, 2, 50, 1000, 2500]
+    h = 2
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]])
+    p = np.concatenate([_pts(rng, w, h) for w in widths])
+    before = sum(op.chain_cost(p[o_ * h:(o_ + w) * h], h) for o_, w in zip(off, widths))
+    q, acc, gain, att = op.refine_pass(p, off, widths, h, cap=512, pas=3, rounds=8, seed=1)
+    after = sum(op.chain_cost(q[o_ * h:(o_ + w) * h], h) for o_, w in zip(off, widths))
+    assert before - after == gain and gain >= 0 and att > 0
+    for o_, w in zip(off, widths):
+        a, b = p[o_ * h:(o_ + w) * h].reshape(w, h), q[o_ * h:(o_ + w) * h].reshape(w, h)
+        for j in range(h):
+            assert sorted(a[:, j].tolist()) == sorted(b[:, j].tolist())
+
+
+def test_round_schedule_statistical_parity_with_reference(golden_cfg1):
+    """After the same number of attempts, the round schedule's cost is within
+    0.5% of the reference descent's on the cfg1 chains (2^16 attempts each,
+    starting from the reference's initial chains)."""
+    g = golden_cfg1
+    h = 2
+    widths = g["chain_w"]
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]])
+    for c, (o_, w) in enumerate(zip(off, widths)):
+        p0 = g["chain_before"][o_ * h:(o_ + w) * h]
+        pr = g["chain_after"][o_ * h:(o_ + w) * h]
+        ref_cost = op.chain_cost(pr, h)
+        # round schedule from the same start, attempts counted exactly
+        q = p0.copy()
+        budget, att = 1 << 16, 0
+        rnd = 0
+        while att < budget:
+            q, acc, gain, a = op.refine_rounds(q, h, rnd % h, 1, seed=99, stream=rnd)
+            att += a
+            rnd += 1
+        mine = op.chain_cost(q, h)
+        assert abs(mine - ref_cost) <= 0.005 * ref_cost, (c, mine, ref_cost)
+
+
+# --------------------------------------------------------------- colour
+def test_hsp_matches_reference_fixture(golden_functions):
+    g = golden_functions
+    got1 = np.array([o.or_rgb_to_hsp(int(c)) for c in g["hsp_in"]], np.uint32)
+    got2 = np.array([o.or_hsp_to_rgb(int(c)) for c in g["hsp_in"]], np.uint32)
+    assert (got1 == g["rgb_to_hsp"]).all()
+    assert (got2 == g["hsp_to_rgb"]).all()
+
+
+def test_hsp_closed_forms():
+    # greys have no hue or saturation; P = grey level
+    for v in range(256):
+        c = v | (v << 8) | (v << 16) | (255 << 24)
+        assert o.or_rgb_to_hsp(c) == (0 | (0 << 8) | (v << 16) | (255 << 24))
+    # pure red: B >= G selects the hue branch 6/6 - ..., so H = 1 (255),
+    # S = 1, P = sqrt(.299)
+    assert o.or_rgb_to_hsp(0xFF0000FF) == (255 | (255 << 8) | (int(np.floor(np.sqrt(0.299) * 255 + 0.5)) << 16)
+                                          | (255 << 24))
+    # pure green: G largest, R >= B -> H = 2/6 - 0 = 1/3 -> 85
+    assert (o.or_rgb_to_hsp(0xFF00FF00) & 255) == 85
+    # alpha is carried through both ways
+    assert (o.or_hsp_to_rgb(0x12345678) >> 24) == 0x12
+
+
+@pytest.mark.skipif(not op.have_ref(), reason="reference harness not built here")
+def test_hsp_matches_live_reference_sample():
+    r = op.ref()
+    rng = np.random.default_rng(7)
+    for c in rng.integers(0, 1 << 32, 20000, dtype=np.uint64):
+        c = int(c)
+        assert o.or_rgb_to_hsp(c) == r.ref_rgb_to_hsp(c)
+        assert o.or_hsp_to_rgb(c) == r.ref_hsp_to_rgb(c)
+
+
+# --------------------------------------------------------------- perlin
+def test_perlin_matches_reference_fixture(golden_functions):
+    g = golden_functions
+    for s, p in zip(g["perlin_seeds"], g["perlin_perm"]):
+        assert (op.perlin_perm(int(s)) == p).all()
+    for s, key in ((1, "perlin_octave8_s1"), (2, "perlin_octave8_s2")):
+        p = op.perlin_perm(s)
+        got = [o.or_perlin_octave2(op.ptr(p), x, y, 8) for x, y in g["perlin_xy"]]
+        assert (np.array(got) == g[key]).all()
+
+
+def test_perlin_lattice_zero_and_range():
+    p = op.perlin_perm(3)
+    for x in range(-3, 10):
+        for y in range(-3, 10):
+            assert o.or_perlin_noise3(op.ptr(p), float(x), float(y), 0.0) == 0.0
+    rng = np.random.default_rng(8)
+    v = [o.or_perlin_octave2(op.ptr(p), *rng.uniform(0, 30, 2), 8) for _ in range(2000)]
+    assert max(abs(np.array(v))) < 2.0
+    # the permutation is one of [0, 256) repeated
+    assert sorted(p[:256].tolist()) == list(range(256)) and (p[:256] == p[256:]).all()
+
+
+# --------------------------------------------------------------- spline
+def test_spline_matches_reference_fixture(golden_functions):
+    g = golden_functions
+    for n, pts, t, out in zip(g["spline_n"], g["spline_pts"], g["spline_t"], g["spline_out"]):
+        vx, vy = np.ascontiguousarray(pts[0]), np.ascontiguousarray(pts[1])
+        ox, oy = C.c_double(), C.c_double()
+        o.or_spline_eval(op.ptr(vx), op.ptr(vy), int(n), float(t), C.byref(ox), C.byref(oy))
+        assert (ox.value, oy.value) == tuple(out)
+
+
+def test_spline_interpolates_control_points():
+    rng = np.random.default_rng(9)
+    for n in (2, 3, 4, 6):
+        vx, vy = rng.uniform(0, 100, n), rng.uniform(0, 100, n)
+        for i in range(n):
+            ox, oy = C.c_double(), C.c_double()
+            o.or_spline_eval(op.ptr(vx), op.ptr(vy), n, i / n, C.byref(ox), C.byref(oy))
+            assert abs(ox.value - vx[i]) < 1e-9 and abs(oy.value - vy[i]) < 1e-9
+
+
+# --------------------------------------------------------------- fluid
+def test_fluid_step_matches_reference_fixture(golden_functions):
+    g = golden_functions
+    P = g["fluid_in"].copy()
+    gx, gy, fr = g["fluid_meta"]
+    for k in range(3):
+        o.or_fluid_step(op.ptr(P), P.shape[0], int(gx), int(gy), 2 - k, float(fr), 0)
+        assert (P[:, [0, 1, 2, 3, 11, 12, 13, 14]] == g["fluid_out"][k][:, [0, 1, 2, 3, 11, 12, 13, 14]]).all()
+
+
+def test_fluid_partition_of_unity():
+    """Quadratic B-spline weights sum to one: a lone resting particle at its
+    attractor with no pressure gradient does not move (beyond the attractor)."""
+    P = np.zeros((1, 20))
+    P[0, :2] = [20.3, 17.8]
+    P[0, 4:6] = [20.3, 17.8]
+    P[0, 6] = 1.0
+    P[0, 15] = 1.0
+    P[0, 16] = 1.0
+    P[0, 17] = 1.0
+    o.or_fluid_step(op.ptr(P), 1, 40, 40, 0, 100.0, 0)
+    # pressure from a single particle's own density is symmetric about it
+    assert abs(P[0, 0] - 20.3) < 0.05 and abs(P[0, 1] - 17.8) < 0.05
+
+
+# --------------------------------------------------------------- segment
+def test_segment_matches_reference_blobs(golden_cfg1):
+    g = golden_cfg1
+    for f in range(2):
+        lab, n = op.segment(g["frames"][f])
+        assert (lab == g["labels"][f]).all()
+
+
+def test_segment_brute_force():
+    from scipy import ndimage
+    rng = np.random.default_rng(10)
+    for size in (1, 7, 33, 64):
+        a = (rng.uniform(0, 1, (size, size)) < 0.55).astype(np.uint8) * 255
+        img = np.zeros((size, size, 4), np.uint8)
+        img[..., 3] = a
+        lab, n = op.segment(img)
+        sl, sn = ndimage.label(a > 0, structure=[[0, 1, 0], [1, 1, 1], [0, 1, 0]])
+        assert n == sn
+        # same partition, canonical label = smallest index
+        flat = sl.ravel()
+        for k in range(1, sn + 1):
+            idx = np.nonzero(flat == k)[0]
+            assert (lab[idx] == idx.min()).all()
+        assert (lab[flat == 0] == -1).all()
+
+
+# --------------------------------------------------------------- LAP
+def test_lap_matches_scipy_and_brute_force():
+    from scipy.optimize import linear_sum_assignment
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 3, 5, 8, 40, 120):
+        c = rng.integers(0, 10 ** 9, (n, n)).astype(np.int64)
+        v, a = op.lap_min_cost(c)
+        r, cc = linear_sum_assignment(c)
+        assert v == int(c[r, cc].sum())
+        assert sorted(a.tolist()) == list(range(n)) and int(c[np.arange(n), a].sum()) == v
+        if n <= 8:
+            assert v == min(int(c[np.arange(n), list(p)].sum()) for p in itertools.permutations(range(n)))
+
+
+# --------------------------------------------------------------- render
+def render_inputs(g, variant):
+    """Oracle render inputs from the cfg1 fixture."""
+    h = 2
+    opts = g[f"render_{variant}_opts"]
+    motion, fading, keep_bg, feather, show, finite = [int(v) for v in opts]
+    widths = g["chain_w"].astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]]).astype(np.int64)
+    keys = np.ascontiguousarray(g["shown"]).view(np.uint8).reshape(h, 64, 64, 4)
+    order = np.ascontiguousarray(g["order"], np.int32)
+    nb = order.shape[1]
+    lag, slope = op.perlin_perm(1), op.perlin_perm(2)
+    bbox = g["bbox"]
+    cfg = op.RenderCfg(64, 64, h, int(bbox[0]), int(bbox[1]), int(bbox[2]), int(bbox[3]), motion, fading,
+                       show, 1, feather, keep_bg, 0, 0, op.ptr(lag), op.ptr(slope))
+    blob_rgba = np.zeros((h, nb), np.uint32)
+    for f in range(h):
+        for b in range(nb):
+            if show == 2:
+                blob_rgba[f, b] = o.or_distinct_color(int(order[f, b]))
+            else:
+                m = g["blob_means"][f, b]
+                blob_rgba[f, b] = o.or_hsp_to_rgb(o.or_create_color(*[float(v) for v in m[2:6]]))
+    return cfg, keys, off, widths, order, nb, blob_rgba, (lag, slope)
+
+
+@pytest.mark.parametrize("variant", ["default", "linear_cosine", "none_linear_bg", "spline_perlin_bg_feather2",
+                                     "average_blobs", "distinct_blobs"])
+def test_render_matches_reference_frames(golden_cfg1, variant):
+    g = golden_cfg1
+    cfg, keys, off, widths, order, nb, blob_rgba, keep = render_inputs(g, variant)
+    pts = np.ascontiguousarray(g["chain_after"])
+    out = np.zeros((64, 64, 4), np.uint8)
+    for i, t in enumerate(g[f"render_{variant}_t"]):
+        o.or_render_atoms(C.byref(cfg), op.ptr(keys), op.ptr(pts), op.ptr(off), op.ptr(widths), len(widths),
+                          op.ptr(order), nb, op.ptr(blob_rgba), float(t), op.ptr(out))
+        want = g[f"render_{variant}"][i]
+        assert (out == want).all(), (variant, i, int((out != want).sum()))
+
+
+def test_get_time_closed_form():
+    assert o.or_get_time(3, 16, 0, 2) == 3 / 16
+    assert o.or_get_time(0, 1, 1, 4) == (1 - 1 / 4) / 2
+    assert o.or_get_time(15, 16, 1, 2) == 1.0 * (1 - 1 / 2)
