@@ -388,6 +388,19 @@ uint32_t or_hsp_to_rgb(uint32_t c) {
                  narrow8_(std::min(std::round(b * 255.0), 255.0)), v.a);
 }
 
+/* morph.cpp:298-301 (add_pixel) and 378-392 (get_pixel). */
+void or_ingest(const uint32_t *rgba, size_t n, int hsp, uint32_t *stored, uint32_t *shown) {
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t c = rgba[i], st = 0, sh = 0;
+        if ((c >> 24) != 0) {
+            st = hsp ? or_rgb_to_hsp(c) : c;
+            sh = hsp ? or_hsp_to_rgb(st) : st;
+        }
+        if (stored) stored[i] = st;
+        if (shown) shown[i] = sh;
+    }
+}
+
 /* perlin.cpp:11-17 */
 void or_perlin_perm(unsigned seed, int32_t *p) {
     if (seed == 0) seed = std::mt19937::default_seed;

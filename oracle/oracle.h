@@ -85,6 +85,10 @@ uint64_t or_hash64(uint64_t x);
 /* ---- colour (color.cpp:31-59, 71-178) ----------------------------------- */
 uint32_t or_rgb_to_hsp(uint32_t rgba);   /* packed r | g<<8 | b<<16 | a<<24 */
 uint32_t or_hsp_to_rgb(uint32_t hspa);
+/* morph::add_pixel + morph::get_pixel for n pixels: stored = rgb_to_hsp(c)
+ * (or c when hsp == 0), shown = hsp_to_rgb(stored) (or stored); alpha 0 ->
+ * both 0. */
+void or_ingest(const uint32_t *rgba, size_t n, int hsp, uint32_t *stored, uint32_t *shown);
 
 /* ---- Perlin (perlin.cpp:11-89) ------------------------------------------ */
 /* p[512] permutation exactly as PerlinNoise(seed) builds it. */

@@ -36,6 +36,7 @@ _ORACLE = {
     "or_hash64": (u64, [u64]),
     "or_rgb_to_hsp": (u32, [u32]),
     "or_hsp_to_rgb": (u32, [u32]),
+    "or_ingest": (None, [vp, sz, C.c_int, vp, vp]),
     "or_perlin_perm": (None, [C.c_uint, vp]),
     "or_perlin_noise3": (f64, [vp, f64, f64, f64]),
     "or_perlin_octave2": (f64, [vp, f64, f64, C.c_int]),
@@ -164,6 +165,14 @@ def refine_pass(pts_atom_major, offsets, widths, h, cap, pas, rounds, seed):
     acc = oracle().or_refine_pass(ptr(pts), ptr(off), ptr(wid), len(wid), h, cap, pas, rounds, seed,
                                   C.byref(g), C.byref(att))
     return pts, acc, g.value, att.value
+
+
+def ingest(words, hsp=1):
+    w = np.ascontiguousarray(words, np.uint32)
+    st = np.zeros_like(w)
+    sh = np.zeros_like(w)
+    oracle().or_ingest(ptr(w), w.size, hsp, ptr(st), ptr(sh))
+    return st, sh
 
 
 def lap_min_cost(cost):
