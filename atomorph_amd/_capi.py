@@ -35,6 +35,15 @@ class ChainBuild(C.Structure):
                 ("blob_first_dev", C.POINTER(vp)), ("seed", u64)]
 
 
+class RenderArgs(C.Structure):
+    _fields_ = [("width", i32), ("height", i32), ("nkeys", i32), ("nblobs", i32),
+                ("bbox_x1", i32), ("bbox_y1", i32), ("bbox_x2", i32), ("bbox_y2", i32),
+                ("motion", i32), ("fading", i32), ("show_blobs", i32), ("density", i32),
+                ("feather", i32), ("keep_background", i32), ("blend_blobs", i32), ("nchains", i32),
+                ("wtot", i64), ("keys_dev", vp), ("pts_dev", vp), ("chain_off_dev", vp), ("chain_w_dev", vp),
+                ("rank_dev", vp), ("blob_rgba_dev", vp), ("lag_perm_dev", vp), ("slope_perm_dev", vp)]
+
+
 # name -> (restype, argtypes); every entry point declared in include/*.h
 PROTOTYPES = {
     "amh_version": (C.c_char_p, []),
@@ -65,6 +74,9 @@ PROTOTYPES = {
     "amh_refine_tiles": (C.c_int, [C.POINTER(i32), vp, vp, i32]),
     "amh_refine_items": (C.c_int, [vp, i32, i32, vp, i32, C.POINTER(i32)]),
     "amh_refine_batched": (C.c_int, [vp, i32, i64, vp, vp, vp, i32, i32, u64, i32, u64, vp, vp, vp]),
+    "amh_render_workspace_bytes": (sz, [i32, i32, i64]),
+    "amh_render_workspace_init": (C.c_int, [vp, sz, i32, i32, i64, vp]),
+    "amh_render_frame": (C.c_int, [C.POINTER(RenderArgs), f64, vp, sz, vp, vp]),
 }
 
 

@@ -1,0 +1,676 @@
+// Frame rendering on the atom path: morph::get_pixels(t) -> draw_atoms
+// (morph.cpp:1302-1421) over get_pixels(blob, t) (morph.cpp:452-678).
+//
+// Per frame:
+//   prepare  per atom: trajectory point and colour at t (FP64, the
+//            reference's operation order), bilinear taps; per tap pixel the
+//            smallest blob index that touches it (atomicMin).
+//   splat    taps of a pixel's smallest blob go to dense per-pixel integer
+//            accumulators (count, sum n, sum c*n with n = the integer
+//            bilinear weight numerator); taps of any other blob are counted
+//            per pixel.
+//   scan     exclusive scan of those counts -> per-pixel overflow segments.
+//   overflow the other taps scattered into their pixel's segment (the count
+//            is decremented back to zero, so the workspace cleans itself).
+//   composite per pixel: resolve the smallest blob, then the overflow blobs
+//            in increasing blob order, "over" in that order, feather, the
+//            background blend, RGBA8 out.
+// Composite order equals the reference's (blob order); sums are exact, so
+// nothing depends on the order the atoms land in.
+#include "colour.h"
+#include "common.h"
+#include "../../include/atomorph_hip.h"
+
+using namespace amh;
+
+namespace {
+
+constexpr uint32_t kNoPos = 0xFFFFFFFFu;
+constexpr int kScan = 1024;  // elements per scan block
+constexpr int kNone = 2, kLinear = 3, kSpline = 4, kCosine = 5, kPerlin = 6;
+
+struct Frame {  // per-frame scalars, computed on the host
+    double tn;   // normalised global time
+    double tl;   // time within the key segment
+    int fk;      // key frame (row y)
+    int fnext;   // next key frame (row y + 1)
+};
+
+// ------------------------------------------------------------- Perlin
+// perlin.cpp:19-69 on the z = 0 plane.  With z = 0 the reference's outer
+// lerp has weight fade(0) = 0 and returns its first argument unchanged, so
+// the z = 1 half never contributes and is not evaluated.
+__device__ __forceinline__ double pfade(double t) { return t * t * t * (t * (t * 6 - 15) + 10); }
+__device__ __forceinline__ double plerp(double t, double a, double b) { return a + t * (b - a); }
+__device__ __forceinline__ double pgrad(int hash, double x, double y) {
+    const int h = hash & 15;
+    const double u = h < 8 ? x : y, v = h < 4 ? y : (h == 12 || h == 14) ? x : 0.0;
+    return ((h & 1) == 0 ? u : -u) + ((h & 2) == 0 ? v : -v);
+}
+__device__ double pnoise(const int *p, double x, double y) {
+    const int X = static_cast<int>(floor(x)) & 255;
+    const int Y = static_cast<int>(floor(y)) & 255;
+    x -= floor(x);
+    y -= floor(y);
+    const double u = pfade(x), v = pfade(y);
+    const int A = p[X] + Y, AA = p[A], AB = p[A + 1];
+    const int B = p[X + 1] + Y, BA = p[B], BB = p[B + 1];
+    return plerp(v, plerp(u, pgrad(p[AA], x, y), pgrad(p[BA], x - 1, y)),
+                 plerp(u, pgrad(p[AB], x, y - 1), pgrad(p[BB], x - 1, y - 1)));
+}
+__device__ double poctave(const int *p, double x, double y) {
+    double result = 0.0, amp = 1.0;
+    for (int i = 0; i < 8; ++i) {
+        result += pnoise(p, x, y) * amp;
+        x *= 2.0;
+        y *= 2.0;
+        amp *= 0.5;
+    }
+    return result;
+}
+
+// ------------------------------------------------------------- colours
+__device__ __forceinline__ uint32_t ch(uint32_t c, int k) { return (c >> (8 * k)) & 255u; }
+
+// morph.cpp:1481-1488
+__device__ uint32_t mix(uint32_t c1, uint32_t c2, double w1) {
+    uint32_t o = 0;
+    for (int k = 0; k < 4; ++k)
+        o |= narrow8(round(w1 * double(ch(c1, k)) + (1.0 - w1) * double(ch(c2, k)))) << (8 * k);
+    return o;
+}
+// morph.cpp:1467-1479 (cosine ramp when fading is COSINE or PERLIN)
+__device__ uint32_t fade_mix(uint32_t c1, uint32_t c2, double lag, double slope, double str, int fading) {
+    if (fading == kCosine || fading == kPerlin) {
+        const double pi = 3.14159265358;
+        const double s = (slope + 0.1) / 1.1;
+        const double l = (1.0 - s) * lag;
+        if (str <= l) str = 0.0;
+        else if (str >= (l + s)) str = 1.0;
+        else str = ((-cos((str - l) * (pi / s)) + 1.0) / 2.0);
+    }
+    return mix(c1, c2, str);
+}
+// The colour fade of get_pixels / get_background for perlin coordinates
+// (px, py) (only used for PERLIN).
+__device__ uint32_t fade_colour(uint32_t c1, uint32_t c2, double px, double py, double str, int fading,
+                                const int *lagp, const int *slopep) {
+    if (fading == kPerlin) {
+        const double lag = poctave(lagp, px, py) * 0.5 + 0.5;
+        const double slope = poctave(slopep, px, py) * 0.5 + 0.5;
+        return fade_mix(c1, c2, lag, slope, str, fading);
+    }
+    if (fading == kCosine) return fade_mix(c1, c2, 0.5, 0.5, str, fading);
+    return mix(c1, c2, str);
+}
+
+__device__ __forceinline__ uint32_t key_px(const uint32_t *keys, int f, int W, int H, uint32_t x, uint32_t y) {
+    if (x >= uint32_t(W) || y >= uint32_t(H)) return 0u;
+    const uint32_t c = keys[int64_t(f) * W * H + int64_t(y) * W + x];
+    return (c >> 24) ? c : 0u;
+}
+
+// --------------------------------------------------------------- prepare
+struct Rec {       // one atom's splat, 16 bytes
+    uint32_t pos;  // y * W + x of the base tap, kNoPos if not drawn
+    uint32_t fm;   // x_fract | y_fract << 8 | tap mask << 16
+    uint32_t col;  // RGBA8
+    int32_t blob;  // composite index
+};
+
+__device__ __forceinline__ int find_chain(const int64_t *off, int nchains, int64_t a) {
+    int lo = 0, hi = nchains - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= a) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_prepare(amh_render R, Frame F, Rec *__restrict__ recs, int *__restrict__ bmin) {
+    __shared__ int lagp[512], slopep[512];
+    if (R.fading == kPerlin) {
+        for (int i = threadIdx.x; i < 512; i += 256) {
+            lagp[i] = R.lag_perm_dev[i];
+            slopep[i] = R.slope_perm_dev[i];
+        }
+    }
+    __syncthreads();
+    const int64_t a = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (a >= R.wtot) return;
+    Rec rec = {kNoPos, 0u, 0u, -1};
+    const int W = R.width, H = R.height, h = R.nkeys;
+    const int c = find_chain(R.chain_off_dev, R.nchains, a);
+    const int b = R.rank_dev[int64_t(F.fk) * R.nchains + c];
+    const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a];
+    const uint64_t p2 = R.pts_dev[int64_t(F.fnext) * R.wtot + a];
+    const bool h1 = pt_flags(p1) & HAS_PIXEL, h2 = pt_flags(p2) & HAS_PIXEL;
+    if (b < 0 || (!h1 && !h2)) {
+        recs[a] = rec;
+        return;
+    }
+    // colours at both ends (morph.cpp:495-522); a missing end fades from /
+    // to the other end's colour with alpha 0
+    uint32_t c1, c2;
+    if (h1 && !h2) {
+        c1 = key_px(R.keys_dev, F.fk, W, H, pt_x(p1), pt_y(p1));
+        c2 = c1 & 0x00FFFFFFu;
+    } else if (h2 && !h1) {
+        c2 = key_px(R.keys_dev, F.fnext, W, H, pt_x(p2), pt_y(p2));
+        c1 = c2 & 0x00FFFFFFu;
+    } else {
+        c1 = key_px(R.keys_dev, F.fk, W, H, pt_x(p1), pt_y(p1));
+        c2 = key_px(R.keys_dev, F.fnext, W, H, pt_x(p2), pt_y(p2));
+    }
+    // trajectory point (morph.cpp:525-531, 1501-1515)
+    uint32_t qx = pt_x(p1), qy = pt_y(p1), qxf = pt_xf(p1), qyf = pt_yf(p1);
+    if (R.motion == kLinear) {
+        const double w1 = 1.0 - F.tl;
+        const double x1 = 256.0 * pt_x(p1) + pt_xf(p1), y1 = 256.0 * pt_y(p1) + pt_yf(p1);
+        const double x2 = 256.0 * pt_x(p2) + pt_xf(p2), y2 = 256.0 * pt_y(p2) + pt_yf(p2);
+        const double xx = (w1 * x1 + (1.0 - w1) * x2), yy = (w1 * y1 + (1.0 - w1) * y2);
+        qx = narrow16(xx / 256.0);
+        qxf = narrow8(xx - double(int(qx) * 256));
+        qy = narrow16(yy / 256.0);
+        qyf = narrow8(yy - double(int(qy) * 256));
+    } else if (R.motion == kSpline) {
+        // closed Catmull-Rom through the key positions (spline.cpp:29-57)
+        const double dt = 1.0 / h;
+        const int pk = (int)(F.tn / dt);
+        auto wrap = [h](int i) { return i < 0 ? h - 1 : (i >= h ? i - h : i); };
+        const int i0 = wrap(pk - 1), i1 = wrap(pk), i2 = wrap(pk + 1), i3 = wrap(pk + 2);
+        const double lt = (F.tn - dt * (double)pk) / dt;
+        const double t2 = lt * lt, t3 = t2 * lt;
+        const double b1 = 0.5 * (-t3 + 2.0 * t2 - lt);
+        const double b2 = 0.5 * (3.0 * t3 - 5.0 * t2 + 2.0);
+        const double b3 = 0.5 * (-3.0 * t3 + 4.0 * t2 + lt);
+        const double b4 = 0.5 * (t3 - t2);
+        const uint64_t q0 = R.pts_dev[int64_t(i0) * R.wtot + a], q1 = R.pts_dev[int64_t(i1) * R.wtot + a];
+        const uint64_t q2 = R.pts_dev[int64_t(i2) * R.wtot + a], q3 = R.pts_dev[int64_t(i3) * R.wtot + a];
+        auto cx = [](uint64_t q) { return pt_x(q) + pt_xf(q) / double(256); };
+        auto cy = [](uint64_t q) { return pt_y(q) + pt_yf(q) / double(256); };
+        const double sx = cx(q0) * b1 + cx(q1) * b2 + cx(q2) * b3 + cx(q3) * b4;
+        const double sy = cy(q0) * b1 + cy(q1) * b2 + cy(q2) * b3 + cy(q3) * b4;
+        double ig;
+        double fr = modf(sx, &ig);
+        qx = narrow16(ig);
+        qxf = narrow8(round(fr * 255));
+        fr = modf(sy, &ig);
+        qy = narrow16(ig);
+        qyf = narrow8(round(fr * 255));
+    }
+    // colour (morph.cpp:537-550); Perlin coordinates follow the source point
+    double perx = 0.0, pery = 0.0;
+    if (R.fading == kPerlin) {
+        const double bw = R.bbox_x2 - R.bbox_x1 + 1.0, bh = R.bbox_y2 - R.bbox_y1 + 1.0;
+        perx = (((int(pt_x(p1)) - R.bbox_x1) * 256 + int(pt_xf(p1))) / double(bw * 256)) * 8.0;
+        pery = (((int(pt_y(p1)) - R.bbox_y1) * 256 + int(pt_yf(p1))) / double(bh * 256)) * 8.0;
+    }
+    const uint32_t col = fade_colour(c1, c2, perx, pery, 1.0 - F.tl, R.fading, lagp, slopep);
+    // guards and taps (morph.cpp:552-589)
+    const int X = int(qx), Y = int(qy);
+    if (X >= W || Y >= H) {
+        if (X > R.bbox_x2 || X < R.bbox_x1 || Y > R.bbox_y2 || Y < R.bbox_y1) {
+            recs[a] = rec;
+            return;
+        }
+    }
+    const int xf = int(qxf), yf = int(qyf);
+    uint32_t mask = 0;
+    if ((255 - xf) * (255 - yf) > 0) mask |= 1u;
+    if ((X < R.bbox_x2 || X + 1 < W) && xf * (255 - yf) > 0) mask |= 2u;
+    if ((Y < R.bbox_y2 || Y + 1 < H) && (255 - xf) * yf > 0) mask |= 4u;
+    if (xf * yf > 0 && ((Y < R.bbox_y2 && X < R.bbox_x2) || (Y + 1 < H && X + 1 < W))) mask |= 8u;
+    // taps outside the image are dropped (draw_atoms skips pos >= W*H)
+    if (X >= W || Y >= H) mask &= 0u;
+    if (X + 1 >= W) mask &= ~(2u | 8u);
+    if (Y + 1 >= H) mask &= ~(4u | 8u);
+    if (!mask) {
+        recs[a] = rec;
+        return;
+    }
+    rec.pos = uint32_t(Y * W + X);
+    rec.fm = qxf | (qyf << 8) | (mask << 16);
+    rec.col = col;
+    rec.blob = b;
+    recs[a] = rec;
+    if (mask & 1u) atomicMin(&bmin[rec.pos], b);
+    if (mask & 2u) atomicMin(&bmin[rec.pos + 1], b);
+    if (mask & 4u) atomicMin(&bmin[rec.pos + W], b);
+    if (mask & 8u) atomicMin(&bmin[rec.pos + W + 1], b);
+}
+
+// Integer bilinear weight numerators of tap k (weights are these / 255^2).
+__device__ __forceinline__ void tap(const Rec &r, int W, int k, uint32_t &p, uint32_t &n) {
+    const uint32_t xf = r.fm & 255u, yf = (r.fm >> 8) & 255u;
+    switch (k) {
+        case 0: p = r.pos; n = (255u - xf) * (255u - yf); break;
+        case 1: p = r.pos + 1; n = xf * (255u - yf); break;
+        case 2: p = r.pos + W; n = (255u - xf) * yf; break;
+        default: p = r.pos + W + 1; n = xf * yf; break;
+    }
+}
+
+struct Acc {  // dense per-pixel accumulators (structure of arrays)
+    unsigned long long *cn;       // count << 40 | sum n
+    unsigned long long *s[4];     // sum c * n per channel
+};
+
+__global__ __launch_bounds__(256) void k_splat(const Rec *__restrict__ recs, int64_t n_atoms, int W,
+                                               const int *__restrict__ bmin, Acc acc, int *__restrict__ ovcount) {
+    const int64_t a = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (a >= n_atoms) return;
+    const Rec r = recs[a];
+    if (r.pos == kNoPos) return;
+    const uint32_t mask = r.fm >> 16;
+    for (int k = 0; k < 4; ++k) {
+        if (!(mask & (1u << k))) continue;
+        uint32_t p, n;
+        tap(r, W, k, p, n);
+        if (bmin[p] == r.blob) {
+            atomicAdd(&acc.cn[p], (1ull << 40) | n);
+            for (int c = 0; c < 4; ++c) atomicAdd(&acc.s[c][p], (unsigned long long)(ch(r.col, c) * n));
+        } else {
+            atomicAdd(&ovcount[p], 1);
+        }
+    }
+}
+
+struct Ov {  // an overflow tap, 16 bytes
+    int32_t blob;
+    uint32_t n;
+    uint32_t col;
+    uint32_t pad;
+};
+
+__global__ __launch_bounds__(256) void k_overflow(const Rec *__restrict__ recs, int64_t n_atoms, int W,
+                                                  const int *__restrict__ bmin, int *__restrict__ ovcount,
+                                                  const int *__restrict__ ovstart, Ov *__restrict__ ov) {
+    const int64_t a = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (a >= n_atoms) return;
+    const Rec r = recs[a];
+    if (r.pos == kNoPos) return;
+    const uint32_t mask = r.fm >> 16;
+    for (int k = 0; k < 4; ++k) {
+        if (!(mask & (1u << k))) continue;
+        uint32_t p, n;
+        tap(r, W, k, p, n);
+        if (bmin[p] != r.blob) {
+            const int slot = atomicSub(&ovcount[p], 1) - 1;
+            Ov o = {r.blob, n, r.col, 0u};
+            ov[ovstart[p] + slot] = o;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- scan
+__global__ __launch_bounds__(256) void k_blk_sum(const int *__restrict__ in, int64_t n, int *__restrict__ sums) {
+    __shared__ int part[4];
+    const int64_t base = int64_t(blockIdx.x) * kScan;
+    int s = 0;
+    for (int l = threadIdx.x; l < kScan; l += 256) {
+        const int64_t i = base + l;
+        if (i < n) s += in[i];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) sums[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+__global__ __launch_bounds__(1024) void k_scan_sums(const int *__restrict__ in, int nb, int *__restrict__ out) {
+    __shared__ int buf[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? in[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) out[i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[nb] = carry;
+}
+// exclusive scan inside each block of kScan, plus the block's offset;
+// out[n] = total.
+__global__ __launch_bounds__(256) void k_blk_apply(const int *__restrict__ in, int64_t n,
+                                                   const int *__restrict__ offs, int nb, int *__restrict__ out) {
+    __shared__ int wsum[4];
+    const int64_t base = int64_t(blockIdx.x) * kScan;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int running = offs[blockIdx.x];
+    for (int chunk = 0; chunk < kScan / 256; ++chunk) {
+        const int64_t i = base + chunk * 256 + threadIdx.x;
+        const int v = i < n ? in[i] : 0;
+        int incl = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int before = running;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        if (i < n) out[i] = before + incl - v;
+        running += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = offs[nb];
+}
+
+// ------------------------------------------------------------ composite
+struct Px {  // one resolved (pixel, blob) colour
+    uint32_t r, g, b, a;
+};
+
+// morph.cpp:591-623 with exact integer sums: round(sum c*n / sum n), alpha
+// times min(1, count / density).
+__device__ Px resolve(uint64_t cnt, uint64_t sn, const uint64_t s[4], int density) {
+    Px q;
+    const uint64_t d2 = 2 * sn;
+    q.r = uint32_t((2 * s[0] + sn) / d2);
+    q.g = uint32_t((2 * s[1] + sn) / d2);
+    q.b = uint32_t((2 * s[2] + sn) / d2);
+    double w = density > 0 ? double(cnt) / double(density) : 0.0;
+    if (w >= 1.0) {
+        q.a = uint32_t((2 * s[3] + sn) / d2);
+    } else {
+        q.a = narrow8(round(w * (double(s[3]) / double(sn))));
+    }
+    return q;
+}
+
+struct Seg {  // overflow segment of one pixel
+    const Ov *ov;
+    int begin, end;
+};
+
+// Is pixel q (inside the image) a member of blob b's resolved pixel set?
+__device__ bool member(const int *bmin, const int *ovstart, const Ov *ov, int64_t q, int b) {
+    if (bmin[q] == b) return true;
+    for (int i = ovstart[q]; i < ovstart[q + 1]; ++i)
+        if (ov[i].blob == b) return true;
+    return false;
+}
+
+// Inward feather (morph.cpp:625-674): the peel layer of pixel (x, y) in
+// blob b is d - 1, d = L1 distance to the nearest pixel outside the blob
+// (coordinates 0 and 65535 count as blob borders).  Returns d (> feather:
+// untouched).
+__device__ int feather_depth(int x, int y, int W, int H, int F, int b, const int *bmin, const int *ovstart,
+                             const Ov *ov) {
+    int best = F + 1;
+    best = min(best, x + 1);
+    best = min(best, y + 1);
+    best = min(best, 65536 - x);
+    best = min(best, 65536 - y);
+    for (int r = 1; r < best; ++r) {
+        // pixels at L1 distance exactly r
+        for (int dx = -r; dx <= r; ++dx) {
+            const int ady = r - (dx < 0 ? -dx : dx);
+            for (int sgn = -1; sgn <= 1; sgn += 2) {
+                if (ady == 0 && sgn > 0) continue;
+                const int qx = x + dx, qy = y + sgn * ady;
+                bool in;
+                if (qx < 0 || qy < 0 || qx >= W || qy >= H) in = false;
+                else in = member(bmin, ovstart, ov, int64_t(qy) * W + qx, b);
+                if (!in) return r;
+            }
+        }
+    }
+    return best;
+}
+
+struct Fold {
+    double r = 0, g = 0, b = 0, a = 0, rs = 0, gs = 0, bs = 0, as = 0;
+    int n = 0;
+    __device__ void add(uint32_t c) {
+        // morph.cpp:1357-1380
+        const double sr = (c & 255u) / 255.0, sg = ((c >> 8) & 255u) / 255.0;
+        const double sb = ((c >> 16) & 255u) / 255.0, sa = (c >> 24) / 255.0;
+        as += sa;
+        rs += sr * sa;
+        gs += sg * sa;
+        bs += sb * sa;
+        if (n == 0) {
+            r = sr; g = sg; b = sb; a = sa;
+        } else {
+            r = sa * sr + (1.0 - sa) * r;
+            g = sa * sg + (1.0 - sa) * g;
+            b = sa * sb + (1.0 - sa) * b;
+            a = a + (1.0 - a) * ((c >> 24) / 255.0);
+        }
+        ++n;
+    }
+};
+
+__device__ uint32_t background(const amh_render &R, const Frame &F, int x, int y, const int *lagp,
+                               const int *slopep) {
+    const uint32_t c1 = key_px(R.keys_dev, F.fk, R.width, R.height, x, y);
+    const uint32_t c2 = key_px(R.keys_dev, F.fnext, R.width, R.height, x, y);
+    double px = 0.0, py = 0.0;
+    if (R.fading == kPerlin) {
+        const double bw = R.bbox_x2 - R.bbox_x1 + 1.0, bh = R.bbox_y2 - R.bbox_y1 + 1.0;
+        px = ((x - R.bbox_x1) / double(bw)) * 8.0;
+        py = ((y - R.bbox_y1) / double(bh)) * 8.0;
+    }
+    return fade_colour(c1, c2, px, py, 1.0 - F.tl, R.fading, lagp, slopep);
+}
+
+__global__ __launch_bounds__(256) void k_composite(amh_render R, Frame F, int *__restrict__ bmin, Acc acc,
+                                                   const int *__restrict__ ovstart, const Ov *__restrict__ ov,
+                                                   uint32_t *__restrict__ out, int clear_bmin) {
+    __shared__ int lagp[512], slopep[512];
+    if (R.keep_background && R.fading == kPerlin) {
+        for (int i = threadIdx.x; i < 512; i += 256) {
+            lagp[i] = R.lag_perm_dev[i];
+            slopep[i] = R.slope_perm_dev[i];
+        }
+    }
+    __syncthreads();
+    const int W = R.width, H = R.height;
+    const int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (p >= int64_t(W) * H) return;
+    const int x = int(p % W), y = int(p / W);
+    const uint32_t bg = R.keep_background ? background(R, F, x, y, lagp, slopep) : 0u;
+    const int b0 = bmin[p];
+    if (b0 == 0x7FFFFFFF) {
+        out[p] = bg;
+        return;
+    }
+    const int nb = R.nblobs;
+    const bool texture = R.show_blobs != 1 && R.show_blobs != 2;
+    Fold fold;
+    auto emit = [&](int b, Px q) {
+        if (texture && R.feather > 0) {
+            const int d = feather_depth(x, y, W, H, R.feather, b, bmin, ovstart, ov);
+            if (d <= R.feather) q.a = narrow8(round(double(q.a) * (double(d) / double(R.feather + 1))));
+        }
+        uint32_t c = q.r | (q.g << 8) | (q.b << 16) | (q.a << 24);
+        if (!texture && R.blob_rgba_dev) c = R.blob_rgba_dev[int64_t(F.fk) * nb + b];
+        if ((c >> 24) == 0) return;
+        fold.add(c);
+    };
+    {
+        const uint64_t cn = acc.cn[p];
+        uint64_t s[4];
+        for (int k = 0; k < 4; ++k) {
+            s[k] = acc.s[k][p];
+            acc.s[k][p] = 0;
+        }
+        acc.cn[p] = 0;
+        emit(b0, resolve(cn >> 40, cn & ((1ull << 40) - 1), s, R.density));
+    }
+    const int o0 = ovstart[p], o1 = ovstart[p + 1];
+    int last = b0;
+    while (o1 > o0) {
+        // next blob above `last`, with its sums (one pass per blob)
+        int nbk = 0x7FFFFFFF;
+        uint64_t cnt = 0, sn = 0, s[4] = {0, 0, 0, 0};
+        for (int i = o0; i < o1; ++i) {
+            const Ov e = ov[i];
+            if (e.blob <= last || e.blob > nbk) continue;
+            if (e.blob < nbk) {
+                nbk = e.blob;
+                cnt = sn = 0;
+                s[0] = s[1] = s[2] = s[3] = 0;
+            }
+            ++cnt;
+            sn += e.n;
+            for (int k = 0; k < 4; ++k) s[k] += uint64_t(ch(e.col, k)) * e.n;
+        }
+        if (nbk == 0x7FFFFFFF) break;
+        emit(nbk, resolve(cnt, sn, s, R.density));
+        last = nbk;
+    }
+    if (clear_bmin) bmin[p] = 0x7FFFFFFF;
+    if (fold.n == 0) {
+        out[p] = bg;
+        return;
+    }
+    double r = fold.r, g = fold.g, b = fold.b, a = fold.a;
+    if (R.blend_blobs) {
+        r = fold.rs / fold.as;
+        g = fold.gs / fold.as;
+        b = fold.bs / fold.as;
+    }
+    if (R.keep_background) {
+        const double bgr = double(bg & 255u) / 255.0, bgg = double((bg >> 8) & 255u) / 255.0;
+        const double bgb = double((bg >> 16) & 255u) / 255.0, bga = double(bg >> 24) / 255.0;
+        r = a * r + (1.0 - a) * bgr;
+        g = a * g + (1.0 - a) * bgg;
+        b = a * b + (1.0 - a) * bgb;
+        a = bga + (1.0 - bga) * a;
+    }
+    out[p] = rgba_pack(narrow8(round(r * 255.0)), narrow8(round(g * 255.0)), narrow8(round(b * 255.0)),
+                       narrow8(round(a * 255.0)));
+}
+
+__global__ void k_fill_i32(int *p, int64_t n, int v) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------ workspace
+struct Layout {
+    int64_t npx, nblk;
+    size_t bmin, ovcount, ovstart, sums, offs, acc, recs, ov, total;
+};
+Layout layout(int32_t W, int32_t H, int64_t wtot) {
+    Layout L;
+    L.npx = int64_t(W) * H;
+    L.nblk = (L.npx + kScan - 1) / kScan;
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t o = 0;
+    L.bmin = o; o = al(o + sizeof(int) * L.npx);
+    L.ovcount = o; o = al(o + sizeof(int) * L.npx);
+    L.ovstart = o; o = al(o + sizeof(int) * (L.npx + 1));
+    L.sums = o; o = al(o + sizeof(int) * (L.nblk + 1));
+    L.offs = o; o = al(o + sizeof(int) * (L.nblk + 1));
+    L.acc = o; o = al(o + 5 * sizeof(uint64_t) * L.npx);
+    L.recs = o; o = al(o + sizeof(Rec) * size_t(wtot > 0 ? wtot : 1));
+    L.ov = o; o = al(o + sizeof(Ov) * 4 * size_t(wtot > 0 ? wtot : 1));
+    L.total = o;
+    return L;
+}
+
+Acc acc_of(char *ws, const Layout &L) {
+    Acc a;
+    unsigned long long *base = reinterpret_cast<unsigned long long *>(ws + L.acc);
+    a.cn = base;
+    for (int k = 0; k < 4; ++k) a.s[k] = base + (k + 1) * L.npx;
+    return a;
+}
+
+}  // namespace
+
+extern "C" size_t amh_render_workspace_bytes(int32_t W, int32_t H, int64_t wtot) {
+    return layout(W, H, wtot).total;
+}
+
+extern "C" int amh_render_workspace_init(void *ws, size_t ws_bytes, int32_t W, int32_t H, int64_t wtot,
+                                         void *stream) {
+    const Layout L = layout(W, H, wtot);
+    if (!ws || ws_bytes < L.total || W <= 0 || H <= 0) {
+        set_error("amh_render_workspace_init: invalid argument");
+        return AMH_E_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char *base = static_cast<char *>(ws);
+    if (int r = check_hip(hipMemsetAsync(base, 0, L.total, s), "amh_render_workspace_init")) return r;
+    hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((L.npx + 255) / 256)), dim3(256), 0, s,
+                       reinterpret_cast<int *>(base + L.bmin), L.npx, 0x7FFFFFFF);
+    AMH_LAUNCH_CHECK("amh_render_workspace_init");
+    return AMH_OK;
+}
+
+extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t ws_bytes, uint32_t *out,
+                                void *stream) {
+    if (!R || !ws || !out || R->width <= 0 || R->height <= 0 || R->nkeys <= 0 || R->nchains < 0 ||
+        !R->keys_dev || (R->wtot > 0 && (!R->pts_dev || !R->chain_off_dev || !R->chain_w_dev || !R->rank_dev)) ||
+        (R->fading == kPerlin && (!R->lag_perm_dev || !R->slope_perm_dev)) || R->width > 65535 ||
+        R->height > 65535) {
+        set_error("amh_render_frame: invalid argument");
+        return AMH_E_ARG;
+    }
+    const Layout L = layout(R->width, R->height, R->wtot);
+    if (ws_bytes < L.total) {
+        set_error("amh_render_frame: workspace %zu < %zu bytes", ws_bytes, L.total);
+        return AMH_E_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    char *base = static_cast<char *>(ws);
+    int *bmin = reinterpret_cast<int *>(base + L.bmin);
+    int *ovcount = reinterpret_cast<int *>(base + L.ovcount);
+    int *ovstart = reinterpret_cast<int *>(base + L.ovstart);
+    int *sums = reinterpret_cast<int *>(base + L.sums);
+    int *offs = reinterpret_cast<int *>(base + L.offs);
+    Rec *recs = reinterpret_cast<Rec *>(base + L.recs);
+    Ov *ov = reinterpret_cast<Ov *>(base + L.ov);
+    const Acc acc = acc_of(base, L);
+
+    // morph::normalize_time / get_frame_key (morph.cpp:404-450, 460-464)
+    Frame F;
+    double integ;
+    F.tn = modf(t, &integ);
+    if (F.tn < 0.0) F.tn += 1.0;
+    const size_t fk = size_t(F.tn * R->nkeys);
+    amh_render Rc = *R;
+    if (fk >= size_t(R->nkeys) || R->wtot == 0) {
+        // nothing to draw: the background only
+        Rc.wtot = 0;
+    }
+    F.fk = int(fk < size_t(R->nkeys) ? fk : 0);
+    F.fnext = (F.fk + 1) % R->nkeys;
+    const double dt = 1.0 / double(R->nkeys);
+    F.tl = fmax(0.0, (F.tn - (double(F.fk) * dt)) / dt);
+    const uint32_t gpx = uint32_t((L.npx + 255) / 256);
+    const uint32_t gat = uint32_t((Rc.wtot + 255) / 256);
+    if (Rc.wtot > 0) {
+        hipLaunchKernelGGL(k_prepare, dim3(gat), dim3(256), 0, s, Rc, F, recs, bmin);
+        hipLaunchKernelGGL(k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc, ovcount);
+    }
+    hipLaunchKernelGGL(k_blk_sum, dim3(uint32_t(L.nblk)), dim3(256), 0, s, ovcount, L.npx, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, int(L.nblk), offs);
+    hipLaunchKernelGGL(k_blk_apply, dim3(uint32_t(L.nblk)), dim3(256), 0, s, ovcount, L.npx, offs, int(L.nblk),
+                       ovstart);
+    if (Rc.wtot > 0)
+        hipLaunchKernelGGL(k_overflow, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, ovcount, ovstart,
+                           ov);
+    const int separate_clear = Rc.feather > 0 ? 1 : 0;
+    hipLaunchKernelGGL(k_composite, dim3(gpx), dim3(256), 0, s, Rc, F, bmin, acc, ovstart, ov, out,
+                       separate_clear ? 0 : 1);
+    if (separate_clear)
+        hipLaunchKernelGGL(k_fill_i32, dim3(gpx), dim3(256), 0, s, bmin, L.npx, 0x7FFFFFFF);
+    AMH_LAUNCH_CHECK("amh_render_frame");
+    return AMH_OK;
+}

@@ -170,6 +170,52 @@ int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int
                        int32_t tile, uint64_t pass, int32_t rounds, uint64_t seed,
                        uint32_t *queue_dev, amh_refine_stats *stats_dev, void *stream);
 
+/* ----------------------------------------------------- frame rendering -- */
+/* morph::get_pixels(t, image) on the atom path (morph.cpp:1405-1421 with
+ * draw_atoms 1302-1403, get_pixels(blob) 452-678, get_background 1431-1465,
+ * interpolate 1467-1515).  Per atom: trajectory (NONE / LINEAR / closed
+ * Catmull-Rom SPLINE), colour fade (NONE/LINEAR, COSINE, PERLIN lag/slope),
+ * bilinear splat with the reference's /255 weights and guards; per blob and
+ * pixel the weighted mean, alpha * min(1, n/density), optional inward
+ * feather; blobs composited with "over" in blob order; optional background
+ * cross-dissolve.  Sums are exact integers (the bilinear weights are
+ * integers over 255^2), so the result does not depend on atom order; it
+ * agrees with the reference's FP64 sums within 1 LSB per channel. */
+typedef struct amh_render {
+    int32_t width, height;          /* set_resolution                        */
+    int32_t nkeys;                  /* key frames                            */
+    int32_t nblobs;                 /* blobs per key frame (blob map width)  */
+    int32_t bbox_x1, bbox_y1, bbox_x2, bbox_y2;
+    int32_t motion;                 /* 2 NONE, 3 LINEAR, 4 SPLINE            */
+    int32_t fading;                 /* 2 NONE, 3 LINEAR, 5 COSINE, 6 PERLIN  */
+    int32_t show_blobs;             /* 0 TEXTURE, 1 AVERAGE, 2 DISTINCT      */
+    int32_t density;
+    int32_t feather;
+    int32_t keep_background;
+    int32_t blend_blobs;
+    int32_t nchains;
+    int64_t wtot;
+    const uint32_t *keys_dev;       /* [nkeys][height*width] get_pixel colour */
+    const uint64_t *pts_dev;        /* [nkeys][wtot] chain points             */
+    const int64_t *chain_off_dev;   /* [nchains]                              */
+    const int32_t *chain_w_dev;     /* [nchains]                              */
+    const int32_t *rank_dev;        /* [nkeys][nchains] blob index of chain c
+                                       in key frame y (composite order), -1:
+                                       not drawn                              */
+    const uint32_t *blob_rgba_dev;  /* [nkeys][nblobs] AVERAGE / DISTINCT
+                                       colours (NULL for TEXTURE)             */
+    const int32_t *lag_perm_dev;    /* PerlinNoise(seed).p[512]               */
+    const int32_t *slope_perm_dev;  /* PerlinNoise(seed + 1).p[512]           */
+} amh_render;
+size_t amh_render_workspace_bytes(int32_t width, int32_t height, int64_t wtot);
+/* Prepares a workspace once; every amh_render_frame leaves it clean again. */
+int amh_render_workspace_init(void *ws_dev, size_t ws_bytes, int32_t width, int32_t height, int64_t wtot,
+                              void *stream);
+/* Renders global time t (any real; normalised like morph::normalize_time)
+ * into out_dev[height*width] packed RGBA8. */
+int amh_render_frame(const amh_render *r, double t, void *ws_dev, size_t ws_bytes, uint32_t *out_dev,
+                     void *stream);
+
 #ifdef __cplusplus
 }
 #endif
