@@ -74,6 +74,14 @@ PROTOTYPES = {
     "amh_refine_tiles": (C.c_int, [C.POINTER(i32), vp, vp, i32]),
     "amh_refine_items": (C.c_int, [vp, i32, i32, vp, i32, C.POINTER(i32)]),
     "amh_refine_batched": (C.c_int, [vp, i32, i64, vp, vp, vp, i32, i32, u64, i32, u64, vp, vp, vp]),
+    "amh_refine_graph_create": (C.c_int, [C.POINTER(vp), vp, i32, i64, vp, vp, vp, i32, i32, i32, i32, u64, vp,
+                                          vp, vp]),
+    "amh_graph_launch": (C.c_int, [vp, vp]),
+    "amh_graph_destroy": (C.c_int, [vp]),
+    "amh_rgb_to_hsp": (u32, [u32]),
+    "amh_hsp_to_rgb": (u32, [u32]),
+    "amh_rgb2hsp": (None, [f64, f64, f64, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64)]),
+    "amh_hsp2rgb": (None, [f64, f64, f64, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64)]),
     "amh_render_workspace_bytes": (sz, [i32, i32, i64]),
     "amh_render_workspace_init": (C.c_int, [vp, sz, i32, i32, i64, vp]),
     "amh_render_frame": (C.c_int, [C.POINTER(RenderArgs), f64, vp, sz, vp, vp]),
@@ -92,6 +100,13 @@ def load(path=HIP_LIB):
     global _lib
     if _lib is not None:
         return _lib
+    # One HIP runtime per process: torch ships its own libamdhip64.so.7 (same
+    # soname as ROCm's).  Whichever is loaded first serves every library, and
+    # torch only works with its own, so let torch load first when present.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(path):
         raise AmhError(f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
     lib = C.CDLL(path)

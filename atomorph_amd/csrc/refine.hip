@@ -117,9 +117,12 @@ __global__ __launch_bounds__(kRoundThreads) void k_refine_round(uint64_t *__rest
 }
 
 // ------------------------------------- batched: persistent, LDS resident
-// queue[0] = next item, queue[1] = workgroups finished; the last workgroup
-// to finish resets both, so the kernel can be replayed (hipGraph) without a
-// memset node.
+// queue[0] = next item, queue[1] = workgroups finished, queue[2] = pass
+// counter.  The last workgroup to finish resets the first two and advances
+// the counter, so the kernel can be replayed (hipGraph) without a memset
+// node; with pass == kPassFromQueue every replay runs the next pass.
+constexpr uint64_t kPassFromQueue = ~0ull;
+
 template <int CAP, int NT>
 __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, int h, int64_t wtot,
                                                     const int64_t *__restrict__ off, const int32_t *__restrict__ cw,
@@ -129,7 +132,13 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
     __shared__ uint64_t C[CAP];
     __shared__ int32_t SX[CAP], SY[CAP];
     __shared__ int item_s;
+    __shared__ uint32_t pass_s;
     const int tid = threadIdx.x;
+    if (pass == kPassFromQueue) {
+        if (tid == 0) pass_s = __hip_atomic_load(&queue[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        pass = pass_s;
+    }
     uint64_t att = 0, acc = 0;
     int64_t gain = 0;
     for (;;) {
@@ -192,6 +201,7 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
         if (done == gridDim.x - 1) {
             atomicExch(&queue[0], 0u);
             atomicExch(&queue[1], 0u);
+            atomicAdd(&queue[2], 1u);
         }
     }
 }
@@ -326,5 +336,65 @@ extern "C" int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot
                        chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed, queue_dev,
                        stats_dev);
     AMH_LAUNCH_CHECK("amh_refine_batched");
+    return AMH_OK;
+}
+
+// ------------------------------------------------------- graph replay
+// A hipGraph of `npasses` batched launches that take their pass number from
+// the queue's counter: one replay = npasses consecutive, distinct passes.
+struct amh_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int amh_refine_graph_create(void **graph, uint64_t *pts_dev, int32_t nkeys, int64_t wtot,
+                                       const int64_t *chain_off_dev, const int32_t *chain_w_dev,
+                                       const int32_t *items_dev, int32_t nitems, int32_t tile, int32_t npasses,
+                                       int32_t rounds, uint64_t seed, uint32_t *queue_dev,
+                                       amh_refine_stats *stats_dev, void *stream) {
+    if (!graph || !stream || npasses <= 0 || !pts_dev || !chain_off_dev || !chain_w_dev || !items_dev ||
+        !queue_dev || !stats_dev || tile < 0 || tile >= kNumTiles || nkeys < 2 || nitems <= 0 || rounds <= 0) {
+        set_error("amh_refine_graph_create: invalid argument (a created stream is required)");
+        return AMH_E_ARG;
+    }
+    const int grid = std::min(tile_grid(tile), int(nitems));
+    if (grid <= 0) return check_hip(hipGetLastError(), "amh_refine_graph_create occupancy");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    amh_graph_s *g = new amh_graph_s;
+    int r = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (!r) {
+        for (int p = 0; p < npasses; ++p)
+            hipLaunchKernelGGL(kTiles[tile].fn, dim3(grid), dim3(kTiles[tile].threads), 0, s, pts_dev, nkeys, wtot,
+                               chain_off_dev, chain_w_dev, items_dev, int(nitems), kPassFromQueue, int(rounds), seed,
+                               queue_dev, stats_dev);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ee = hipStreamEndCapture(s, &g->graph);
+        r = check_hip(le != hipSuccess ? le : ee, "graph capture");
+    }
+    if (!r) r = check_hip(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (r) {
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return r;
+    }
+    *graph = g;
+    return AMH_OK;
+}
+
+extern "C" int amh_graph_launch(void *graph, void *stream) {
+    amh_graph_s *g = static_cast<amh_graph_s *>(graph);
+    if (!g || !g->exec) {
+        set_error("amh_graph_launch: NULL graph");
+        return AMH_E_ARG;
+    }
+    return check_hip(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+}
+
+extern "C" int amh_graph_destroy(void *graph) {
+    amh_graph_s *g = static_cast<amh_graph_s *>(graph);
+    if (!g) return AMH_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return AMH_OK;
 }

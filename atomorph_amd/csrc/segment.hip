@@ -550,3 +550,9 @@ extern "C" int amh_build_chains(const amh_chain_build *a, uint64_t *pts, void *s
     AMH_LAUNCH_CHECK("amh_build_chains");
     return AMH_OK;
 }
+
+// Host-callable colour conversions with the kernels' arithmetic.
+extern "C" uint32_t amh_rgb_to_hsp(uint32_t rgba) { return rgb_to_hsp(rgba); }
+extern "C" uint32_t amh_hsp_to_rgb(uint32_t hspa) { return hsp_to_rgb(hspa); }
+extern "C" void amh_rgb2hsp(double r, double g, double b, double *h, double *s, double *p) { rgb2hsp(r, g, b, h, s, p); }
+extern "C" void amh_hsp2rgb(double h, double s, double p, double *r, double *g, double *b) { hsp2rgb(h, s, p, r, g, b); }

@@ -161,7 +161,10 @@ int amh_refine_chain(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, int64_t cha
  * writes the row back.  Chains wider than the tile capacity are split into
  * groups by a random slot permutation that changes every pass.
  * items_dev[2*i], [2*i+1] = (chain, group); build it with
- * amh_refine_items.  `tile` selects the kernel variant (amh_refine_tiles). */
+ * amh_refine_items.  `tile` selects the kernel variant (amh_refine_tiles).
+ * queue_dev: 4 uint32 words, zeroed once before first use; the kernel
+ * leaves words 0-1 at zero and counts completed passes in word 2.  With
+ * pass == UINT64_MAX the pass number is read from word 2 instead. */
 int amh_refine_tiles(int32_t *count, int32_t *capacity, int32_t *threads, int32_t max_count);
 int amh_refine_items(const int32_t *chain_w, int32_t nchains, int32_t tile, int32_t *items,
                      int32_t max_items, int32_t *nitems);
@@ -169,6 +172,22 @@ int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int
                        const int32_t *chain_w_dev, const int32_t *items_dev, int32_t nitems,
                        int32_t tile, uint64_t pass, int32_t rounds, uint64_t seed,
                        uint32_t *queue_dev, amh_refine_stats *stats_dev, void *stream);
+/* hipGraph of `npasses` batched launches whose pass numbers come from the
+ * queue's counter, so every amh_graph_launch runs the next npasses passes
+ * with one submission.  Capture needs a stream made by amh_stream_create. */
+int amh_refine_graph_create(void **graph, uint64_t *pts_dev, int32_t nkeys, int64_t wtot,
+                            const int64_t *chain_off_dev, const int32_t *chain_w_dev, const int32_t *items_dev,
+                            int32_t nitems, int32_t tile, int32_t npasses, int32_t rounds, uint64_t seed,
+                            uint32_t *queue_dev, amh_refine_stats *stats_dev, void *stream);
+int amh_graph_launch(void *graph, void *stream);
+int amh_graph_destroy(void *graph);
+
+/* ------------------------------------------------------ host colour ----- */
+/* The kernels' colour arithmetic on the host (color.cpp:31-59, 71-178). */
+uint32_t amh_rgb_to_hsp(uint32_t rgba);
+uint32_t amh_hsp_to_rgb(uint32_t hspa);
+void amh_rgb2hsp(double r, double g, double b, double *h, double *s, double *p);
+void amh_hsp2rgb(double h, double s, double p, double *r, double *g, double *b);
 
 /* ----------------------------------------------------- frame rendering -- */
 /* morph::get_pixels(t, image) on the atom path (morph.cpp:1405-1421 with

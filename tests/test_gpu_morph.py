@@ -1,0 +1,170 @@
+"""End-to-end am::morph on the GPU: the reference's driver sequence
+(set options, add frames, iterate states, render) through the C++ engine."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from atomorph_amd import morph as M
+from atomorph_amd import synth
+from oracle import oracle_py as op
+from tests import gpu_util as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _pipeline(frames, seed=1, cycle=1 << 16, match_steps=2000, **opts):
+    m = M.Morph()
+    m.set(seed=seed, cycle_length=cycle, **opts)
+    for i, f in enumerate(frames):
+        m.add_frame_rgba(i, f)
+    hh, ww = frames[0].shape[:2]
+    m.set_resolution(ww, hh)
+    G.log("morph: detection")
+    assert m.run(1) == M.STATE_BLOB_UNIFICATION
+    G.log("morph: matching")
+    st = m.run(match_steps + 1)
+    assert st == M.STATE_BLOB_MATCHING
+    m.next_state()
+    G.log("morph: chain build")
+    assert m.run(1) == M.STATE_ATOM_MORPHING
+    return m
+
+
+def test_detection_matches_reference_blobs(golden_cfg1):
+    g = golden_cfg1
+    m = M.Morph()
+    m.set(seed=1)
+    for i, f in enumerate(g["frames"]):
+        m.add_frame_rgba(i, f)
+    m.set_resolution(64, 64)
+    G.log("cfg1 detection")
+    assert m.run(1) == M.STATE_BLOB_UNIFICATION
+    for f in range(2):
+        n = m.blob_count(f)
+        comps = set()
+        for b in range(n):
+            s = m.blob(f, b)["surface"]
+            idx = (s // 65536) * 64 + (s % 65536)
+            comps.add(tuple(sorted(idx.tolist())))
+        lab = g["labels"][f]
+        want = set()
+        for r in np.unique(lab[lab >= 0]):
+            want.add(tuple(np.nonzero(lab == r)[0].tolist()))
+        assert comps == want
+
+
+def test_chains_follow_init_morph(golden_cfg1):
+    """Row y of every chain: the blob's pixels in row-major order (HAS_PIXEL |
+    HAS_FLUID, zero fractions), then duplicates (HAS_PIXEL only) of those
+    pixels; an empty blob gives one HAS_FLUID point and flag-less copies."""
+    m = _pipeline(list(golden_cfg1["frames"]))
+    pts, widths = m.chains()
+    h = pts.shape[0]
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]])
+    assert widths.sum() == pts.shape[1]
+    for f in range(h):
+        groups = {}
+        for b in range(m.blob_count(f)):
+            bl = m.blob(f, b)
+            groups[bl["group"]] = bl
+        for c, (o, w) in enumerate(zip(off, widths)):
+            row = pts[f, o:o + w]
+            surf = groups[c]["surface"]
+            fl = (row >> 48) & 255
+            xy = (row & 0xFFFFFFFF)
+            pos = (xy >> 16) * 65536 + (xy & 0xFFFF)
+            n = len(surf)
+            if n:
+                assert (pos[:n] == surf).all()
+                assert (fl[:n] == 3).all() and (((row[:n] >> 32) & 0xFFFF) == 0).all()
+                assert (fl[n:] == 1).all() and set(pos[n:].tolist()) <= set(surf.tolist())
+            else:
+                assert fl[0] == 2 and (fl[1:] == 0).all()
+            assert w == max(len(groups_f[c]["surface"]) for groups_f in [
+                {m.blob(ff, bb)["group"]: m.blob(ff, bb) for bb in range(m.blob_count(ff))} for ff in range(h)])
+
+
+def test_refinement_plain_then_batched_then_graph(golden_cfg1):
+    m = _pipeline(list(golden_cfg1["frames"]))
+    c0 = m.cost()
+    pts0, widths = m.chains()
+    G.log(f"plain refine from cost {c0}")
+    m.run(3 * len(widths))
+    c1 = m.cost()
+    assert c1 <= c0
+    a1, acc1 = m.counters()
+    assert a1 >= 3 * (1 << 16) * 0.99
+    m.set_refine(M.REFINE_BATCHED, rounds=16, tile=2, graph_passes=0)
+    G.log("batched refine")
+    m.run(20)
+    c2 = m.cost()
+    assert c2 <= c1
+    m.set_refine(M.REFINE_BATCHED, rounds=16, tile=3, graph_passes=8)
+    G.log("graph refine")
+    m.run(5)
+    c3 = m.cost()
+    assert c3 <= c2
+    a3, _ = m.counters()
+    assert a3 > a1
+    # rows remain permutations of the initial rows
+    pts3, _ = m.chains()
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]])
+    for o, w in zip(off, widths):
+        for y in range(pts3.shape[0]):
+            assert (np.sort(pts3[y, o:o + w]) == np.sort(pts0[y, o:o + w])).all()
+    # exact cost equals the oracle's recomputation
+    am = G.atom_major(pts3.ravel(), pts3.shape[0])
+    h = pts3.shape[0]
+    assert c3 == sum(op.chain_cost(am[o * h:(o + w) * h], h) for o, w in zip(off, widths))
+    G.log(f"costs: {c0} -> {c1} -> {c2} -> {c3}")
+
+
+def test_render_matches_oracle_on_engine_chains(golden_cfg1):
+    """Frames from am::morph equal the oracle's rendering of the engine's own
+    chains and blob order within 1 LSB; the pixel-vector API gives the same
+    image as the bulk one."""
+    m = _pipeline(list(golden_cfg1["frames"]))
+    m.run(6)
+    pts, widths = m.chains()
+    h = pts.shape[0]
+    nb = m.blob_count(0)
+    order = np.zeros((h, nb), np.int32)
+    for f in range(h):
+        for b in range(nb):
+            order[f, b] = m.blob(f, b)["group"]
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]]).astype(np.int64)
+    keys = np.ascontiguousarray(golden_cfg1["shown"]).view(np.uint8).reshape(h, 64, 64, 4)
+    lag, slope = op.perlin_perm(1), op.perlin_perm(2)
+    bbox = golden_cfg1["bbox"]
+    cfg = op.RenderCfg(64, 64, h, int(bbox[0]), int(bbox[1]), int(bbox[2]), int(bbox[3]), 4, 6, 0, 1, 0, 0, 0, 0,
+                       op.ptr(lag), op.ptr(slope))
+    am = G.atom_major(pts.ravel(), h)
+    w64 = widths.astype(np.int64)
+    out = np.zeros((64, 64, 4), np.uint8)
+    worst = 0
+    for i in range(16):
+        t = m.get_time(i, 16)
+        G.log(f"engine render frame {i}")
+        got = m.render(t)
+        op.oracle().or_render_atoms(C.byref(cfg), op.ptr(keys), op.ptr(am), op.ptr(off), op.ptr(w64), len(widths),
+                                    op.ptr(order), nb, None, float(t), op.ptr(out))
+        worst = max(worst, int(np.abs(got.astype(int) - out.astype(int)).max()))
+        if i % 5 == 0:
+            assert (m.render_pixels(t) == got).all()
+    assert worst <= 1
+
+
+def test_blobs512_pipeline_runs():
+    frames = synth.workload("blobs512")
+    m = _pipeline(frames, cycle=1 << 14, match_steps=500)
+    pts, widths = m.chains()
+    assert len(widths) == 40 and pts.shape == (2, widths.sum())
+    c0 = m.cost()
+    m.set_refine(M.REFINE_BATCHED, rounds=32, tile=4, graph_passes=16)
+    G.log("blobs512 graph refine")
+    m.run(8)
+    c1 = m.cost()
+    assert c1 < c0
+    img = m.render(0.25)
+    assert img.shape == (512, 512, 4) and (img[..., 3] > 0).sum() > 50000

@@ -109,7 +109,7 @@ def test_refine_batched_bitexact_all_tiles():
             items, nitems = _items(wid, tile)
             dev = G.to_dev(G.key_major(pts, h))
             off_d, w_d, it_d = G.to_dev(off), G.to_dev(wid.astype(np.int32)), G.to_dev(items)
-            queue = G.zeros(2, T.int32)
+            queue = G.zeros(4, T.int32)
             stats = G.zeros(4, T.int64)
             want = pts.copy()
             att_w = acc_w = gain_w = 0
@@ -124,8 +124,8 @@ def test_refine_batched_bitexact_all_tiles():
             assert (got == want).all(), (h, tile)
             st = G.to_host(stats, np.int64)
             assert (st[0], st[1], st[2]) == (att_w, acc_w, gain_w)
-            # the queue resets itself for the next launch
-            assert (G.to_host(queue, np.int32) == 0).all()
+            # the queue resets itself for the next launch and counts passes
+            assert G.to_host(queue, np.int32).tolist() == [0, 0, 3, 0]
 
 
 def test_refine_batched_full_size_properties():
@@ -142,7 +142,7 @@ def test_refine_batched_full_size_properties():
     items, nitems = _items(wid, tile)
     dev = G.to_dev(G.key_major(pts, h))
     off_d, w_d, it_d = G.to_dev(off), G.to_dev(wid.astype(np.int32)), G.to_dev(items)
-    queue = G.zeros(2, T.int32)
+    queue = G.zeros(4, T.int32)
     stats = G.zeros(4, T.int64)
     c0 = sum(_gpu_cost(T, dev, h, wtot, off_d, w_d, len(wid)))
     prev = c0
