@@ -131,7 +131,7 @@ struct Options {  // the setters, as the worker sees them (morph.h:97-119)
     bool keep_background = false, blend_blobs = false, finite = false;
     unsigned fluidsteps = 0, show_blobs = TEXTURE;
     unsigned seed = 0;
-    unsigned refine_mode = REFINE_PLAIN, refine_rounds = 8, graph_passes = 0;
+    unsigned refine_mode = REFINE_PLAIN, refine_rounds = 0, graph_passes = 0;  // rounds 0: default
     int refine_tile = 2;
     size_t match_budget = 0;
 };
@@ -630,7 +630,7 @@ bool morph::engine::morph_step() {
     if (nch == 0 || h < 2) return false;
     if (wopt.refine_mode == REFINE_BATCHED) {
         const int tile = wopt.refine_tile < 0 ? 2 : wopt.refine_tile;
-        const int rounds = int(std::max(1u, wopt.refine_rounds));
+        const int rounds = wopt.refine_rounds > 0 ? int(wopt.refine_rounds) : 8;
         if (items_tile != tile) {
             int32_t n = 0;
             if (amh_refine_items(cw.data(), int32_t(nch), tile, nullptr, 0, &n)) return fail("refine items");
@@ -669,8 +669,10 @@ bool morph::engine::morph_step() {
     const size_t c = counter % nch;
     if (cw[c] <= 1) return false;
     const int32_t y = int32_t(rng() % uint64_t(h));
+    // rounds per visit: fixed when set, else cycle_length attempts
     const uint64_t pairs = uint64_t(cw[c] / 2);
-    const uint64_t rounds = std::max<uint64_t>(1, (wopt.cycle_length + pairs - 1) / pairs);
+    const uint64_t rounds = wopt.refine_rounds > 0 ? uint64_t(wopt.refine_rounds)
+                                                   : std::max<uint64_t>(1, (wopt.cycle_length + pairs - 1) / pairs);
     if (amh_refine_chain(d_pts.as<uint64_t>(), h, wtot, off[c], cw[c], y, rounds, mix_seed(),
                          (uint64_t(c) << 32) ^ visits, d_rstats.as<amh_refine_stats>(), cstream))
         return fail("refine chain");
@@ -1408,7 +1410,7 @@ void morph::set_resolution(uint16_t w, uint16_t h) {
 
 void morph::set_refine(unsigned mode, unsigned rounds, int tile, unsigned graph_passes) {
     E->opt.refine_mode = mode;
-    E->opt.refine_rounds = rounds ? rounds : 1;
+    E->opt.refine_rounds = rounds;
     E->opt.refine_tile = tile;
     E->opt.graph_passes = graph_passes;
 }

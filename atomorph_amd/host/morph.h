@@ -121,8 +121,9 @@ class morph {
     bool        get_frame_rgba      (double t, uint8_t *out);
     // Exact transport cost of the synchronized chains (128-bit).
     bool        transport_cost      (uint64_t *lo, uint64_t *hi);
-    // Refinement schedule: REFINE_PLAIN (default) or REFINE_BATCHED with
-    // `rounds` rounds per pass, kernel variant `tile` (-1 = autotune) and
+    // Refinement schedule: REFINE_PLAIN (default; `rounds` rounds per chain
+    // visit, 0 = cycle_length attempts like the reference) or REFINE_BATCHED
+    // (`rounds` rounds per pass, 0 = 8), kernel variant `tile` and
     // `graph_passes` passes per hipGraph replay (0 = no graph).
     void        set_refine          (unsigned mode, unsigned rounds, int tile, unsigned graph_passes);
     // Matching ends by itself after this many steps (0 = only at

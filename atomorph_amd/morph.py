@@ -124,6 +124,10 @@ class Morph:
             raise _capi.AmhError(self.error())
         return st
 
+    def restart(self):
+        """Back to blob detection at the next run (frames and options kept)."""
+        self.lib.am_morph_restart(self.h)
+
     def run_timed(self, n):
         """n steps; returns (state, device milliseconds of those steps)."""
         ms = C.c_float()

@@ -1,0 +1,291 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X morph engine on one GPU.
+
+One step = one full morph job on the 1024^2 workload (4 RGBA8 key frames of
+150 noise-thresholded blobs each): blob detection, blob matching, chain build,
+batched optimal-transport refinement (hipGraph replays) and 60 output frames
+rendered into HBM.  The key frames are uploaded before the timed region.
+
+Printed (one JSON line): frames/s of the whole job, plus
+  time_to_cost  plain GPU descent (one chain at a time) run to convergence vs
+                the batched persistent descent (and the learned warm start when
+                enabled) timed until they reach the plain descent's final cost;
+  roofline      the dominant kernel's achieved bytes/s against 8 TB/s;
+  cpu_baseline  the compiled reference (oracle/_ref) on this host, one core,
+                on a bounded sample of the same job, scaled to frames/s.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FRAMES_OUT = 60
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_morph(M, frames, seed=1):
+    m = M.Morph()
+    m.set(seed=seed, cycle_length=1 << 16)
+    for i, f in enumerate(frames):
+        m.add_frame_rgba(i, f)
+    h, w = frames[0].shape[:2]
+    m.set_resolution(w, h)
+    return m
+
+
+def to_morphing(m, M, match_steps):
+    m.run(1)                        # blob detection (+ unification)
+    m.run(match_steps + 1)          # matching
+    m.next_state()
+    st = m.run(1)                   # chain build
+    if st != M.STATE_ATOM_MORPHING:
+        raise RuntimeError(f"engine did not reach ATOM_MORPHING ({st}): {m.error()}")
+
+
+def time_to_cost(m, M, log_every=20):
+    """Plain descent to convergence, then the batched descent from the same
+    initial chains until it reaches the plain final cost.  Device time of the
+    refinement steps only (HIP events on the engine's compute stream)."""
+    pts0, widths = m.chains()
+    nch = len(widths)
+    c0 = m.cost()
+    m.set_refine(M.REFINE_PLAIN, rounds=4)   # 4 rounds (~2w attempts) per chain visit
+    t_plain, hist = 0.0, [(0.0, c0)]
+    for sweep in range(2000):
+        _, ms = m.run_timed(nch)
+        t_plain += ms
+        hist.append((t_plain, m.cost()))
+        gain = c0 - hist[-1][1]
+        # converged: the last 20 sweeps gained < 0.1% of the gain so far
+        if len(hist) > 21 and hist[-21][1] - hist[-1][1] <= 1e-3 * gain:
+            break
+        if t_plain > 60000.0:
+            break
+        if sweep % log_every == 0:
+            log(f"  plain sweep {sweep}: {t_plain:.1f} ms cost {hist[-1][1]}")
+    c_plain = hist[-1][1]
+    gain = c0 - c_plain
+    out = {"c0": c0, "plain_cost": c_plain, "plain_ms": round(t_plain, 3), "plain_sweeps": len(hist) - 1}
+
+    def race(label, setup):
+        m.put_chains(pts0)
+        setup()
+        t, reach_lit, reach_gain = 0.0, None, None
+        if c0 <= 1.005 * c_plain:
+            reach_lit = 0.0
+        for _ in range(4000):
+            _, ms = m.run_timed(1)
+            t += ms
+            c = m.cost()
+            if reach_lit is None and c <= 1.005 * c_plain:
+                reach_lit = t
+            if reach_gain is None and c <= c_plain + 0.005 * gain:
+                reach_gain = t
+            if reach_gain is not None and reach_lit is not None:
+                break
+            if t > 60000.0:
+                break
+        return {"ms_to_within_0.5pct_of_plain_cost": reach_lit, "ms_to_99.5pct_of_plain_gain": reach_gain,
+                "final_cost": c}
+
+    out["batched"] = race("batched", lambda: m.set_refine(M.REFINE_BATCHED, rounds=32, tile=4, graph_passes=16))
+    b = out["batched"]["ms_to_99.5pct_of_plain_gain"]
+    out["ratio_batched_vs_plain"] = (b / t_plain) if (b is not None and t_plain > 0) else None
+    return out
+
+
+def cpu_baseline(frames, gpu_attempts, match_steps):
+    """The compiled reference on this host, one core (threads = 0), on a
+    bounded sample of the same job: full blob detection, matching and chain
+    init; a timed refinement sample scaled to the GPU job's attempt count;
+    3 rendered frames scaled to 60."""
+    from oracle import oracle_py as op
+    if not op.have_ref():
+        return None
+    import numpy as np
+    r = op.ref()
+    m = r.ref_morph_new()
+    for k, v in {"seed": 1, "threads": 0, "cycle_length": 1 << 16}.items():
+        r.ref_morph_set_option(m, k.encode(), float(v))
+    for i, f in enumerate(frames):
+        a = np.ascontiguousarray(f)
+        r.ref_morph_add_rgba(m, i, op.ptr(a), f.shape[1], f.shape[0])
+    H, W = frames[0].shape[:2]
+    r.ref_morph_set_resolution(m, W, H)
+    st = r.ref_morph_run(m, 0)
+    t0 = time.perf_counter()
+    while st == 0:
+        st = r.ref_morph_run(m, 50000)
+    t_detect = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r.ref_morph_run(m, match_steps + 1)
+    r.ref_morph_next_state(m)
+    r.ref_morph_run(m, 1)
+    t_match_init = time.perf_counter() - t0
+    n = r.ref_morph_chain_count(m)
+    t0 = time.perf_counter()
+    r.ref_morph_run(m, n)           # one 2^16-attempt visit per chain
+    t_visits = time.perf_counter() - t0
+    rate = n * (1 << 16) / t_visits
+    r.ref_morph_next_state(m)
+    r.ref_morph_run(m, 1)
+    out = np.zeros((H, W, 4), np.uint8)
+    nr = 3
+    t0 = time.perf_counter()
+    for i in range(nr):
+        r.ref_morph_render(m, r.ref_morph_get_time(m, i * 20 + 7, FRAMES_OUT), op.ptr(out))
+    t_frame = (time.perf_counter() - t0) / nr
+    r.ref_morph_free(m)
+    total = t_detect + t_match_init + gpu_attempts / rate + FRAMES_OUT * t_frame
+    return {"value": round(FRAMES_OUT / total, 5), "unit": "frames/s", "cores": 1, "kind": "reference",
+            "sample": (f"compiled reference on this host, threads=0: full blob detection ({t_detect:.1f} s), "
+                       f"{match_steps} matching steps + chain init ({t_match_init:.1f} s), {n} chain visits of 2^16 "
+                       f"attempts ({rate / 1e6:.2f} M attempts/s, scaled to the GPU job's {gpu_attempts} attempts), "
+                       f"{nr} rendered frames ({t_frame:.2f} s each, scaled to {FRAMES_OUT}); job estimate "
+                       f"{total:.1f} s"),
+            "phases_s": {"detect": round(t_detect, 3), "match_init": round(t_match_init, 3),
+                         "refine": round(gpu_attempts / rate, 3), "render": round(FRAMES_OUT * t_frame, 3)}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="blobs1024")
+    ap.add_argument("--match-steps", type=int, default=20000)
+    ap.add_argument("--graphs", type=int, default=8, help="refinement hipGraph replays per job (16 passes each)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-time-to-cost", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the morph engine on a "
+              f"single MI355X. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from atomorph_amd import morph as M
+    from atomorph_amd import synth
+
+    torch.cuda.set_device(0)
+    frames = synth.workload(args.workload)
+    H, W = frames[0].shape[:2]
+    m = build_morph(M, frames)
+    out = torch.empty(FRAMES_OUT * H * W * 4, dtype=torch.uint8, device="cuda")
+    times = [m.get_time(i, FRAMES_OUT) for i in range(FRAMES_OUT)]
+
+    # time-to-cost (once, before the throughput steps)
+    ttc = None
+    if not args.no_time_to_cost:
+        m.run(0)
+        to_morphing(m, M, args.match_steps)
+        log("time-to-cost: plain descent to convergence, then batched")
+        ttc = time_to_cost(m, M)
+        log(f"  {ttc}")
+
+    m.set(cycle_length=1 << 16)
+    m.set_refine(M.REFINE_BATCHED, rounds=32, tile=4, graph_passes=16)
+    phase = {"detect_ms": [], "refine_gpu_ms": [], "render_gpu_ms": [], "job_ms": []}
+    attempts_job = accepted_job = 0
+
+    def job(record):
+        nonlocal attempts_job, accepted_job
+        m.restart()
+        m.run(0)                                   # key frames into HBM (untimed)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, ms_det = m.run_timed(1)
+        m.run(args.match_steps + 1)
+        m.next_state()
+        if m.run(1) != M.STATE_ATOM_MORPHING:
+            raise RuntimeError(m.error())
+        a0, acc0 = m.counters()
+        _, ms_ref = m.run_timed(args.graphs)
+        a1, acc1 = m.counters()
+        ms_ren = m.render_device(times, out.data_ptr())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if record:
+            phase["detect_ms"].append(ms_det)
+            phase["refine_gpu_ms"].append(ms_ref)
+            phase["render_gpu_ms"].append(ms_ren)
+            phase["job_ms"].append(dt * 1e3)
+            attempts_job, accepted_job = a1 - a0, acc1 - acc0
+        return dt
+
+    for _ in range(args.warmup):
+        job(False)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    total = 0.0
+    for _ in range(args.steps):
+        total += job(True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    ms_per_step = 1e3 * total / args.steps
+    value = FRAMES_OUT * args.steps / total
+
+    pts, widths = m.chains()
+    h = pts.shape[0]
+    passes = 16 * args.graphs
+    # roofline of the refinement pass kernel: per attempt it reads the two
+    # slots' points in rows y-1, y, y+1 (8 B each; rows y-1 = y+1 when h = 2)
+    # and writes 16 B per accepted swap.
+    per_attempt = 8 * 2 * (2 + (1 if h > 2 else 0))
+    refine_bytes = attempts_job * per_attempt + accepted_job * 16
+    refine_ms = float(np.mean(phase["refine_gpu_ms"]))
+    launch_s = refine_ms / 1e3 / passes
+    achieved = refine_bytes / passes / launch_s / 1e9
+    # rendering, for comparison: per atom and frame h points + 2 colours read,
+    # per output pixel 4 B written
+    render_bytes = FRAMES_OUT * (int(widths.sum()) * (8 * h + 8) + H * W * 4)
+    render_gbs = render_bytes / (float(np.mean(phase["render_gpu_ms"])) / 1e3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: compiled reference, bounded sample")
+        cpu = cpu_baseline(frames, attempts_job, args.match_steps)
+
+    res = {
+        "metric": "time to equal OT cost, learned+batched vs plain GPU, 1024²; frames/s",
+        "value": round(value, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: 4 key frames 1024x1024 RGBA8, 150 fBm-thresholded blobs each, "
+                               f"atom trajectories (SPLINE, PERLIN fade), fluid off, {FRAMES_OUT} frames rendered",
+                   "atoms": int(widths.sum()), "chains": int(len(widths)), "key_frames": h,
+                   "match_steps": args.match_steps, "refine_passes": passes, "refine_attempts": attempts_job},
+        "phases_ms": {k: round(float(np.mean(v)), 3) for k, v in phase.items()},
+        "roofline": {"kernel": "k_refine_pass (batched refinement pass)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(refine_bytes / passes), "launch_us": round(launch_s * 1e6, 3)},
+        "render_gbs": round(render_gbs, 1),
+        "time_to_cost": ttc,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()
