@@ -44,6 +44,11 @@ class RenderArgs(C.Structure):
                 ("rank_dev", vp), ("blob_rgba_dev", vp), ("lag_perm_dev", vp), ("slope_perm_dev", vp)]
 
 
+class FluidFrame(C.Structure):
+    _fields_ = [("r", C.POINTER(RenderArgs)), ("rank_dev", vp), ("surface_dev", vp), ("blob_col_dev", vp),
+                ("sparseness", f64), ("fluidsteps", i32), ("pad_", i32)]
+
+
 # name -> (restype, argtypes); every entry point declared in include/*.h
 PROTOTYPES = {
     "amh_version": (C.c_char_p, []),
@@ -85,6 +90,13 @@ PROTOTYPES = {
     "amh_render_workspace_bytes": (sz, [i32, i32, i64]),
     "amh_render_workspace_init": (C.c_int, [vp, sz, i32, i32, i64, vp]),
     "amh_render_frame": (C.c_int, [C.POINTER(RenderArgs), f64, vp, sz, vp, vp]),
+    "amh_fluid_create": (C.c_int, [C.POINTER(vp), i32, i32, i32, i32, i32, i32, vp]),
+    "amh_fluid_destroy": (C.c_int, [vp]),
+    "amh_fluid_reset": (C.c_int, [vp, f64, vp]),
+    "amh_fluid_set_particles": (C.c_int, [vp, vp, vp]),
+    "amh_fluid_get_particles": (C.c_int, [vp, vp, vp]),
+    "amh_fluid_step": (C.c_int, [vp, i32, f64, u64, vp]),
+    "amh_fluid_render": (C.c_int, [vp, C.POINTER(FluidFrame), f64, vp, vp]),
 }
 
 

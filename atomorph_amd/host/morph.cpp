@@ -134,6 +134,7 @@ struct Options {  // the setters, as the worker sees them (morph.h:97-119)
     unsigned refine_mode = REFINE_PLAIN, refine_rounds = 0, graph_passes = 0;  // rounds 0: default
     int refine_tile = 2;
     size_t match_budget = 0;
+    double fluid_sparseness = 1.0;  // grid cells per pixel (the reference's sparseness)
 };
 
 struct HostFrame {  // caller-side pixels of one key frame, in insertion order
@@ -212,6 +213,15 @@ struct morph::engine {
     std::vector<int32_t> rank;                    // [h][nchains]
     std::vector<uint32_t> brgba;                  // [h][bw]
     bool have_snap = false;
+    // fluid path: the model persists across frames; its per-segment inputs
+    // are rebuilt after every snapshot
+    void *fluid = nullptr;
+    double fluid_sp = 0.0;
+    bool fluid_inputs = false;
+    DevBuf d_frank, d_fsurf, d_fbcol;
+    bool prepare_fluid();
+    bool render_fluid(double t, uint32_t *out_dev);
+    void fill_render(amh_render &r) const;
 
     // ---------------------------------------------------- worker thread
     std::thread th;
@@ -270,6 +280,7 @@ morph::engine::~engine() {
     cv.notify_all();
     if (th.joinable()) th.join();
     if (graph) amh_graph_destroy(graph);
+    if (fluid) amh_fluid_destroy(fluid);
     for (auto &v : out_blobs)
         for (blob *b : v) delete b;
     // device buffers free themselves; streams last
@@ -310,7 +321,105 @@ void morph::engine::reset_worker() {
         graph = nullptr;
     }
     have_snap = false;
+    fluid_inputs = false;
+    if (fluid) {  // a restart deletes the fluid model (morph.cpp:131-132)
+        amh_fluid_destroy(fluid);
+        fluid = nullptr;
+    }
     rng.seed(wopt.seed);
+}
+
+void morph::engine::fill_render(amh_render &r) const {
+    std::memset(&r, 0, sizeof(r));
+    r.width = CW; r.height = CH; r.nkeys = h; r.nblobs = int32_t(bw);
+    r.bbox_x1 = bbox_x1; r.bbox_y1 = bbox_y1; r.bbox_x2 = bbox_x2; r.bbox_y2 = bbox_y2;
+    r.motion = opt.motion; r.fading = opt.fading; r.show_blobs = int32_t(opt.show_blobs);
+    r.density = opt.density; r.feather = int32_t(opt.feather);
+    r.keep_background = opt.keep_background ? 1 : 0; r.blend_blobs = opt.blend_blobs ? 1 : 0;
+    r.nchains = int32_t(cw.size()); r.wtot = wtot;
+    r.keys_dev = d_shown.as<uint32_t>(); r.pts_dev = d_snap.as<uint64_t>();
+    r.chain_off_dev = d_off.as<int64_t>(); r.chain_w_dev = d_w.as<int32_t>();
+    r.rank_dev = d_rank.as<int32_t>(); r.blob_rgba_dev = d_brgba.as<uint32_t>();
+    r.lag_perm_dev = d_lag.as<int32_t>(); r.slope_perm_dev = d_slope.as<int32_t>();
+}
+
+// Per key segment: which atoms carry particles (rank: both ends hold fluid,
+// then source only, then destination only, hash order inside each class),
+// the blob sizes |A|, |B| and the AVERAGE / DISTINCT colours
+// (update_particle, morph.cpp:760-777).
+bool morph::engine::prepare_fluid() {
+    const size_t nch = cw.size();
+    if (!have_snap || nch == 0 || wtot == 0) return false;
+    const double sp = opt.fluid_sparseness > 0.0 ? opt.fluid_sparseness : 1.0;
+    if (!fluid || fluid_sp != sp) {
+        if (fluid) amh_fluid_destroy(fluid);
+        fluid = nullptr;
+        const int32_t gsx = int32_t(CW * sp) + 20, gsy = int32_t(CH * sp) + 20;
+        if (amh_fluid_create(&fluid, gsx, gsy, int32_t(wtot), CW, CH, h, rstream)) return fail("fluid create");
+        if (amh_fluid_reset(fluid, sp, rstream)) return fail("fluid reset");
+        fluid_sp = sp;
+    }
+    if (fluid_inputs) return true;
+    std::vector<uint64_t> pts(size_t(wtot) * h);
+    if (amh_memcpy_d2h(pts.data(), d_snap.p, 8 * pts.size(), rstream) || amh_stream_sync(rstream))
+        return fail("fluid chains");
+    std::vector<int32_t> rk(size_t(h) * wtot);
+    std::vector<int32_t> surf(size_t(h) * nch);
+    std::vector<double> bcol(size_t(h) * nch * 4, 0.0);
+    std::vector<std::pair<uint64_t, int32_t>> keyx;
+    for (int y = 0; y < h; ++y) {
+        const int yn = (y + 1) % h;
+        for (size_t c = 0; c < nch; ++c) {
+            const int64_t o = off[c];
+            keyx.clear();
+            for (int32_t x = 0; x < cw[c]; ++x) {
+                const uint64_t p1 = pts[size_t(y) * wtot + o + x], p2 = pts[size_t(yn) * wtot + o + x];
+                const bool f1 = (p1 >> 48) & HAS_FLUID, f2 = (p2 >> 48) & HAS_FLUID;
+                const uint64_t cls = f1 && f2 ? 0 : f1 ? 1 : f2 ? 2 : 3;
+                uint64_t hsh = (uint64_t(c) << 40) ^ (uint64_t(y) << 32) ^ uint64_t(x);
+                hsh += 0x9E3779B97F4A7C15ull;
+                hsh = (hsh ^ (hsh >> 30)) * 0xBF58476D1CE4E5B9ull;
+                hsh = (hsh ^ (hsh >> 27)) * 0x94D049BB133111EBull;
+                keyx.emplace_back((cls << 62) | ((hsh ^ (hsh >> 31)) >> 2), x);
+            }
+            std::sort(keyx.begin(), keyx.end());
+            for (size_t i = 0; i < keyx.size(); ++i) rk[size_t(y) * wtot + o + keyx[i].second] = int32_t(i);
+            const BlobInfo &bi = fb[y][best_map[c * h + y]];
+            surf[size_t(y) * nch + c] = int32_t(bi.size);
+            double *q = &bcol[(size_t(y) * nch + c) * 4];
+            if (opt.show_blobs == DISTINCT) {
+                std::mt19937 gen(static_cast<std::mt19937::result_type>(c));
+                std::uniform_real_distribution<double> u(0.0, 1.0);
+                q[0] = u(gen); q[1] = u(gen); q[2] = u(gen); q[3] = 1.0;
+            } else {
+                q[0] = bi.r; q[1] = bi.g; q[2] = bi.b; q[3] = bi.a;
+            }
+        }
+    }
+    if (!d_frank.ensure(4 * rk.size()) || !d_fsurf.ensure(4 * surf.size()) || !d_fbcol.ensure(8 * bcol.size()))
+        return fail("device allocation");
+    if (amh_memcpy_h2d(d_frank.p, rk.data(), 4 * rk.size(), rstream) ||
+        amh_memcpy_h2d(d_fsurf.p, surf.data(), 4 * surf.size(), rstream) ||
+        amh_memcpy_h2d(d_fbcol.p, bcol.data(), 8 * bcol.size(), rstream) || amh_stream_sync(rstream))
+        return fail("fluid inputs");
+    fluid_inputs = true;
+    return true;
+}
+
+bool morph::engine::render_fluid(double t, uint32_t *out_dev) {
+    if (!prepare_fluid()) return false;
+    amh_render r;
+    fill_render(r);
+    amh_fluid_frame fa;
+    std::memset(&fa, 0, sizeof(fa));
+    fa.r = &r;
+    fa.rank_dev = d_frank.as<int32_t>();
+    fa.surface_dev = d_fsurf.as<int32_t>();
+    fa.blob_col_dev = opt.show_blobs == TEXTURE ? nullptr : d_fbcol.as<double>();
+    fa.sparseness = fluid_sp;
+    fa.fluidsteps = int32_t(opt.fluidsteps);
+    if (amh_fluid_render(fluid, &fa, t, out_dev, rstream)) return fail("fluid render");
+    return true;
 }
 
 // Key frames onto the device canvas (morph::synchronize -> worker.set_frame,
@@ -770,6 +879,7 @@ bool morph::engine::snapshot() {
     if (!d_out.ensure(4 * size_t(CW) * CH)) return fail("device allocation");
     if (amh_stream_sync(cstream)) return fail("snapshot sync");
     have_snap = true;
+    fluid_inputs = false;  // chains moved: the fluid's per-segment inputs are rebuilt on use
     return true;
 }
 
@@ -1143,26 +1253,21 @@ bool morph::render_device(const double *times, size_t n, void *out_dev, float *g
         return false;
     }
     amh_render r;
-    std::memset(&r, 0, sizeof(r));
-    r.width = e.CW; r.height = e.CH; r.nkeys = e.h; r.nblobs = int32_t(e.bw);
-    r.bbox_x1 = e.bbox_x1; r.bbox_y1 = e.bbox_y1; r.bbox_x2 = e.bbox_x2; r.bbox_y2 = e.bbox_y2;
-    r.motion = e.opt.motion; r.fading = e.opt.fading; r.show_blobs = int32_t(e.opt.show_blobs);
-    r.density = e.opt.density; r.feather = int32_t(e.opt.feather);
-    r.keep_background = e.opt.keep_background ? 1 : 0; r.blend_blobs = e.opt.blend_blobs ? 1 : 0;
-    r.nchains = int32_t(e.cw.size()); r.wtot = e.wtot;
-    r.keys_dev = e.d_shown.as<uint32_t>(); r.pts_dev = e.d_snap.as<uint64_t>();
-    r.chain_off_dev = e.d_off.as<int64_t>(); r.chain_w_dev = e.d_w.as<int32_t>();
-    r.rank_dev = e.d_rank.as<int32_t>(); r.blob_rgba_dev = e.d_brgba.as<uint32_t>();
-    r.lag_perm_dev = e.d_lag.as<int32_t>(); r.slope_perm_dev = e.d_slope.as<int32_t>();
+    e.fill_render(r);
+    const bool fluid = e.opt.fluidsteps > 0;
+    if (fluid && !e.prepare_fluid()) return false;
     void *timer = nullptr;
     if (amh_timer_begin(&timer, e.rstream)) return e.fail("timer");
     const size_t npx = size_t(e.CW) * e.CH;
-    for (size_t i = 0; i < n; ++i)
-        if (amh_render_frame(&r, times[i], e.d_rws.p, e.d_rws.n, static_cast<uint32_t *>(out_dev) + i * npx,
-                             e.rstream)) {
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t *dst = static_cast<uint32_t *>(out_dev) + i * npx;
+        const bool ok = fluid ? e.render_fluid(times[i], dst)
+                              : amh_render_frame(&r, times[i], e.d_rws.p, e.d_rws.n, dst, e.rstream) == AMH_OK;
+        if (!ok) {
             amh_timer_free(timer);
-            return e.fail("render");
+            return fluid ? false : e.fail("render");
         }
+    }
     float ms = 0.0f;
     const int rc = amh_timer_end(timer, e.rstream, &ms);
     amh_timer_free(timer);
@@ -1262,23 +1367,16 @@ bool morph::get_frame_rgba(double t, uint8_t *out) {
     const size_t npx = size_t(e.CW) * e.CH;
     if (e.have_snap) {
         amh_render r;
-        std::memset(&r, 0, sizeof(r));
-        r.width = e.CW; r.height = e.CH; r.nkeys = e.h; r.nblobs = int32_t(e.bw);
-        r.bbox_x1 = e.bbox_x1; r.bbox_y1 = e.bbox_y1; r.bbox_x2 = e.bbox_x2; r.bbox_y2 = e.bbox_y2;
-        r.motion = e.opt.motion; r.fading = e.opt.fading; r.show_blobs = int32_t(e.opt.show_blobs);
-        r.density = e.opt.density; r.feather = int32_t(e.opt.feather);
-        r.keep_background = e.opt.keep_background ? 1 : 0; r.blend_blobs = e.opt.blend_blobs ? 1 : 0;
-        r.nchains = int32_t(e.cw.size()); r.wtot = e.wtot;
-        r.keys_dev = e.d_shown.as<uint32_t>(); r.pts_dev = e.d_snap.as<uint64_t>();
-        r.chain_off_dev = e.d_off.as<int64_t>(); r.chain_w_dev = e.d_w.as<int32_t>();
-        r.rank_dev = e.d_rank.as<int32_t>(); r.blob_rgba_dev = e.d_brgba.as<uint32_t>();
-        r.lag_perm_dev = e.d_lag.as<int32_t>(); r.slope_perm_dev = e.d_slope.as<int32_t>();
+        e.fill_render(r);
         if (e.opt.fading == PERLIN && (e.opt.seed != e.wopt.seed)) {  // perms follow the caller's seed
             amh_memcpy_h2d(e.d_lag.p, e.lag_perm, 2048, e.rstream);
             amh_memcpy_h2d(e.d_slope.p, e.slope_perm, 2048, e.rstream);
         }
-        if (amh_render_frame(&r, t, e.d_rws.p, e.d_rws.n, e.d_out.as<uint32_t>(), e.rstream))
+        if (e.opt.fluidsteps > 0) {  // draw_fluid (morph.cpp:1418)
+            if (!e.render_fluid(t, e.d_out.as<uint32_t>())) return false;
+        } else if (amh_render_frame(&r, t, e.d_rws.p, e.d_rws.n, e.d_out.as<uint32_t>(), e.rstream)) {
             return e.fail("render");
+        }
         img.resize(npx);
         if (amh_memcpy_d2h(img.data(), e.d_out.p, npx * 4, e.rstream) || amh_stream_sync(e.rstream))
             return e.fail("render download");
@@ -1415,6 +1513,9 @@ void morph::set_refine(unsigned mode, unsigned rounds, int tile, unsigned graph_
     E->opt.graph_passes = graph_passes;
 }
 void morph::set_match_budget(size_t steps) { E->opt.match_budget = steps; }
+void morph::set_fluid_sparseness(double s) {
+    if (s > 0.0) E->opt.fluid_sparseness = s;
+}
 void morph::get_refine_counters(uint64_t *attempts, uint64_t *accepted) {
     if (attempts) *attempts = E->counters[0];
     if (accepted) *accepted = E->counters[1];

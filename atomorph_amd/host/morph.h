@@ -129,6 +129,9 @@ class morph {
     // Matching ends by itself after this many steps (0 = only at
     // best energy 0 or next_state(), as in the reference).
     void        set_match_budget    (size_t steps);
+    // Fluid grid cells per pixel (the reference fixes it at 1: a
+    // (width+20) x (height+20) grid); 0.25 gives a quarter-resolution grid.
+    void        set_fluid_sparseness(double s);
     // Swap attempts and accepted swaps so far (synchronized).
     void        get_refine_counters (uint64_t *attempts, uint64_t *accepted);
     // Atom trajectories of the synchronized chains: total atoms, key frames,

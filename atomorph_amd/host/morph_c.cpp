@@ -41,6 +41,7 @@ int am_morph_set(void *mp, const char *name, double v) {
     else if (n == "fluid") m->set_fluid(unsigned(v));
     else if (n == "density") m->set_density(uint16_t(v));
     else if (n == "match_budget") m->set_match_budget(size_t(v));
+    else if (n == "fluid_sparseness") m->set_fluid_sparseness(v);
     else return -1;
     return 0;
 }

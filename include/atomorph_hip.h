@@ -235,6 +235,42 @@ int amh_render_workspace_init(void *ws_dev, size_t ws_bytes, int32_t width, int3
 int amh_render_frame(const amh_render *r, double t, void *ws_dev, size_t ws_bytes, uint32_t *out_dev,
                      void *stream);
 
+/* --------------------------------------------------------- fluid path -- */
+/* The fluid renderer (fluidsteps > 0): morph::step_fluid, FluidModel::step
+ * and morph::draw_fluid (morph.cpp:807-1300, fluidmodel.cpp:107-580).  The
+ * model holds `count` particles (= atoms of the chains; morph.cpp:235-240
+ * sizes it by the chains' max_surface, which equals the atoms at density 1)
+ * on a gsx x gsy node grid (the reference uses width*s+20 x height*s+20 with
+ * s = 1), plus width x height render buffers.  State persists from frame to
+ * frame: rendering advances the simulation, as in the reference. */
+int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t count, int32_t width, int32_t height,
+                     int32_t nkeys, void *stream);
+int amh_fluid_destroy(void *fluid);
+/* Particles back to the state morph::synchronize gives a new model
+ * (morph.cpp:243-262): centred, inactive, green. */
+int amh_fluid_reset(void *fluid, double sparseness, void *stream);
+/* Particle state as 20 doubles per particle: x y u v gravity_x gravity_y
+ * freedom_r R G B A r g b a strength active mature, 2 unused (synchronous). */
+int amh_fluid_set_particles(void *fluid, const double *host, void *stream);
+int amh_fluid_get_particles(void *fluid, double *host, void *stream);
+/* One FluidModel::step (fluidmodel.cpp:165-580); the wall bounce jitter
+ * (rand() in the reference) comes from `seed`. */
+int amh_fluid_step(void *fluid, int32_t steps_left, double freedom_radius, uint64_t seed, void *stream);
+typedef struct amh_fluid_frame {
+    const amh_render *r;          /* chains, key frames, options, Perlin     */
+    const int32_t *rank_dev;      /* [nkeys][wtot]: activation order of each
+                                     atom within its chain, per key segment  */
+    const int32_t *surface_dev;   /* [nkeys][nchains]: blob sizes            */
+    const double *blob_col_dev;   /* [nkeys][nchains][4] AVERAGE / DISTINCT
+                                     colours in [0,1] (NULL for TEXTURE)     */
+    double sparseness;            /* grid cells per pixel (reference: 1)     */
+    int32_t fluidsteps;           /* FluidModel::step calls per frame        */
+    int32_t pad_;
+} amh_fluid_frame;
+/* One frame at global time t: particle update, fluidsteps model steps,
+ * splat, resolve, outward feather, background; out_dev[width*height]. */
+int amh_fluid_render(void *fluid, const amh_fluid_frame *frame, double t, uint32_t *out_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

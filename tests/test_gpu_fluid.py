@@ -1,0 +1,193 @@
+"""GPU fluid path through the C ABI: FluidModel::step against the reference's
+own outputs (via the fixture) and the FP64 oracle, and the frame path's
+particle bookkeeping and rendering."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from atomorph_amd import _capi as A
+from atomorph_amd import morph as M
+from oracle import oracle_py as op
+from tests import gpu_util as G
+from tests.test_gpu_render import GpuRenderer
+from tests.test_oracle import render_inputs
+
+pytestmark = pytest.mark.gpu
+
+# FP32 grid against FP64: positions within 1e-4 cell, velocities 1e-4, colours 1e-4
+TOL = {0: 1e-4, 1: 1e-4, 2: 1e-4, 3: 1e-4, 11: 1e-4, 12: 1e-4, 13: 1e-4, 14: 1e-4}
+
+
+def _model(n, gsx, gsy, W=64, H=64):
+    lib = G.lib()
+    f = C.c_void_p()
+    G.check(lib.amh_fluid_create(C.byref(f), gsx, gsy, n, W, H, 2, G.stream()), "amh_fluid_create")
+    return f
+
+
+def _step_gpu(P, gsx, gsy, steps, radius):
+    lib = G.lib()
+    n = P.shape[0]
+    f = _model(n, gsx, gsy)
+    try:
+        Q = np.ascontiguousarray(P.copy())
+        G.check(lib.amh_fluid_set_particles(f, op.ptr(Q), G.stream()), "set")
+        outs = []
+        for k in range(steps):
+            G.check(lib.amh_fluid_step(f, steps - 1 - k, radius, 0, G.stream()), "step")
+            G.check(lib.amh_fluid_get_particles(f, op.ptr(Q), G.stream()), "get")
+            outs.append(Q.copy())
+        return outs
+    finally:
+        lib.amh_fluid_destroy(f)
+
+
+def test_fluid_step_matches_reference_fixture(golden_functions):
+    g = golden_functions
+    gx, gy, fr = (int(g["fluid_meta"][0]), int(g["fluid_meta"][1]), float(g["fluid_meta"][2]))
+    G.log("fluid step vs fixture")
+    outs = _step_gpu(g["fluid_in"], gx, gy, 3, fr)
+    for k in range(3):
+        want = g["fluid_out"][k]
+        for col, tol in TOL.items():
+            d = np.abs(outs[k][:, col] - want[:, col]).max()
+            assert d <= tol * (k + 1), (k, col, d)
+
+
+def test_fluid_step_matches_oracle_many_particles():
+    rng = np.random.default_rng(4)
+    n, gsx, gsy = 20000, 276, 276
+    P = np.zeros((n, 20))
+    P[:, 0] = rng.uniform(8, 268, n)
+    P[:, 1] = rng.uniform(8, 268, n)
+    P[:, 2:4] = rng.normal(0, 0.2, (n, 2))
+    P[:, 4] = P[:, 0] + rng.normal(0, 3, n)
+    P[:, 5] = P[:, 1] + rng.normal(0, 3, n)
+    P[:, 6] = 1.0
+    P[:, 7:15] = rng.uniform(0, 1, (n, 8))
+    P[:, 15] = rng.choice([0.1, 1.0], n)
+    P[:, 16] = (rng.uniform(0, 1, n) < 0.9).astype(float)
+    P[:, 17] = rng.integers(0, 2, n)
+    G.log("fluid step 20k particles")
+    outs = _step_gpu(P, gsx, gsy, 3, 20.0)
+    Q = P.copy()
+    o = op.oracle()
+    for k in range(3):
+        o.or_fluid_step(op.ptr(Q), n, gsx, gsy, 2 - k, 20.0, 0)
+        for col in (0, 1, 2, 3):
+            d = np.abs(outs[k][:, col] - Q[:, col]).max()
+            assert d <= 2e-3, (k, col, d)
+        assert (outs[k][:, 16] == Q[:, 16]).all()
+
+
+def _fluid_frame(g, variant, t, fluidsteps, sp=1.0, reset=True, fluid=None, rend=None):
+    """Render one fluid frame of the cfg1 chains through the C ABI."""
+    lib = G.lib()
+    cfg, keys, off, widths, order, nb, blob_rgba, (lag, slope) = render_inputs(g, variant)
+    if rend is None:
+        vals = {"bbox": [cfg.bbox_x1, cfg.bbox_y1, cfg.bbox_x2, cfg.bbox_y2], "motion": cfg.motion,
+                "fading": cfg.fading, "show": cfg.show_blobs, "density": cfg.density, "feather": cfg.feather,
+                "keep_bg": cfg.keep_background}
+        rend = GpuRenderer(keys, g["chain_after"], off, widths, order, blob_rgba, vals, lag, slope)
+    h = 2
+    wtot = int(np.sum(widths))
+    pts = g["chain_after"].reshape(-1, h)
+    nch = len(widths)
+    # activation ranks and blob sizes per key segment, as the engine builds them
+    rank = np.zeros((h, wtot), np.int32)
+    surf = np.zeros((h, nch), np.int32)
+    for y in range(h):
+        yn = (y + 1) % h
+        for c, (o_, w) in enumerate(zip(off, widths)):
+            f1 = (pts[o_:o_ + w, y] >> 48) & 2
+            f2 = (pts[o_:o_ + w, yn] >> 48) & 2
+            cls = np.where((f1 > 0) & (f2 > 0), 0, np.where(f1 > 0, 1, np.where(f2 > 0, 2, 3)))
+            order_x = np.lexsort((np.arange(w), cls))
+            rank[y, o_ + order_x] = np.arange(w)
+            surf[y, c] = int((f1 > 0).sum())
+    keep = [G.to_dev(rank.ravel()), G.to_dev(surf.ravel())]
+    fa = A.FluidFrame(C.pointer(rend.args), keep[0].data_ptr(), keep[1].data_ptr(), None, sp, fluidsteps, 0)
+    if fluid is None:
+        fluid = C.c_void_p()
+        G.check(lib.amh_fluid_create(C.byref(fluid), int(64 * sp) + 20, int(64 * sp) + 20, wtot, 64, 64, h,
+                                     G.stream()), "create")
+        G.check(lib.amh_fluid_reset(fluid, sp, G.stream()), "reset")
+    G.check(lib.amh_fluid_render(fluid, C.byref(fa), float(t), G.dptr(rend.out), G.stream()), "render")
+    img = G.to_host(rend.out, np.uint32).view(np.uint8).reshape(64, 64, 4).copy()
+    return img, fluid, rend, surf, rank
+
+
+def test_fluid_frame_without_steps_reproduces_key_frame(golden_cfg1):
+    """fluidsteps = 0 at t = 0: each of the |A| active particles of a chain
+    sits on its source pixel (grid = pixel + 10, weight 1), so the frame is
+    key frame 0 as get_pixel returns it."""
+    g = golden_cfg1
+    G.log("fluid frame t=0, no steps")
+    img, fluid, rend, surf, rank = _fluid_frame(g, "default", 0.0, 0)
+    try:
+        key = np.ascontiguousarray(g["shown"][0]).view(np.uint8).reshape(64, 64, 4)
+        assert (img == key).all()
+        # active particles per chain = round((1-t)|A| + t|B|) at t = 0
+        lib = G.lib()
+        n = int(rank.shape[1])
+        Q = np.zeros((n, 20))
+        G.check(lib.amh_fluid_get_particles(fluid, op.ptr(Q), G.stream()), "get")
+        widths = g["chain_w"]
+        off = np.concatenate([[0], np.cumsum(widths)[:-1]])
+        for c, (o_, w) in enumerate(zip(off, widths)):
+            assert int(Q[o_:o_ + w, 16].sum()) == surf[0, c]
+    finally:
+        G.lib().amh_fluid_destroy(fluid)
+
+
+def test_fluid_frames_advance_and_count(golden_cfg1):
+    """Frames in order with 4 model steps each: the active count follows the
+    interpolated blob areas and the image stays covered and opaque where the
+    fluid is."""
+    g = golden_cfg1
+    fluid = rend = None
+    widths = g["chain_w"]
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]])
+    lib = G.lib()
+    try:
+        for i in range(16):
+            t = i / 16
+            G.log(f"fluid frame {i}")
+            img, fluid, rend, surf, rank = _fluid_frame(g, "default", t, 4, fluid=fluid, rend=rend)
+            n = int(rank.shape[1])
+            Q = np.zeros((n, 20))
+            G.check(lib.amh_fluid_get_particles(fluid, op.ptr(Q), G.stream()), "get")
+            fk = int(t * 2)
+            tl = (t - fk / 2) * 2
+            for c, (o_, w) in enumerate(zip(off, widths)):
+                want = int(np.floor((1 - tl) * surf[fk, c] + tl * surf[(fk + 1) % 2, c] + 0.5))
+                assert int(Q[o_:o_ + w, 16].sum()) == want
+            cover = int((img[..., 3] > 0).sum())
+            assert cover > 0.5 * min(surf[0].sum(), surf[1].sum())
+            assert np.isfinite(Q[:, :4]).all()
+    finally:
+        if fluid is not None:
+            lib.amh_fluid_destroy(fluid)
+
+
+def test_engine_fluid_render():
+    """am::morph with set_fluid: frames render through the fluid path and
+    differ from the atom path."""
+    from atomorph_amd import synth
+    frames = synth.workload("ellipses64")
+    m = M.Morph()
+    m.set(seed=1, cycle_length=1 << 14)
+    for i, f in enumerate(frames):
+        m.add_frame_rgba(i, f)
+    m.set_resolution(64, 64)
+    m.run(1)
+    m.run(300)
+    m.next_state()
+    assert m.run(1) == M.STATE_ATOM_MORPHING
+    m.run(4)
+    atoms = m.render(0.3)
+    m.set(fluid=5, feather=1)
+    imgs = [m.render(i / 16) for i in range(16)]
+    assert all((im[..., 3] > 0).sum() > 100 for im in imgs)
+    assert not (imgs[5] == atoms).all()
