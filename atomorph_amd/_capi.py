@@ -66,6 +66,9 @@ PROTOTYPES = {
     "amh_timer_begin": (C.c_int, [C.POINTER(vp), vp]),
     "amh_timer_end": (C.c_int, [vp, vp, C.POINTER(f32)]),
     "amh_timer_free": (C.c_int, [vp]),
+    "amh_profile_enable": (C.c_int, [C.c_int]),
+    "amh_profile_read": (C.c_int, [C.c_char_p, C.POINTER(u64), C.POINTER(f64)]),
+    "amh_profile_reset": (C.c_int, []),
     "amh_ingest_frame": (C.c_int, [vp, i32, i32, i32, vp, vp, vp]),
     "amh_segment": (C.c_int, [vp, i32, i32, vp, vp]),
     "amh_blob_workspace_bytes": (sz, [i32, i32]),

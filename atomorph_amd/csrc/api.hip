@@ -2,9 +2,14 @@
 #include "common.h"
 #include "../../include/atomorph_hip.h"
 
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
 
 namespace amh {
 
@@ -23,10 +28,88 @@ int check_hip(hipError_t e, const char *what) {
     return e == hipErrorOutOfMemory ? AMH_E_NOMEM : AMH_E_HIP;
 }
 
+// ---------------------------------------------------- kernel profiling
+namespace prof {
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+static std::mutex mu;
+static std::atomic<bool> on{false};
+static std::vector<Pending> pending;
+static std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+static std::map<std::string, std::pair<uint64_t, double>> totals;  // launches, ms
+static thread_local std::vector<Pending> open;
+}  // namespace prof
+
+bool profiling() { return prof::on.load(std::memory_order_relaxed); }
+
+void prof_begin(const char *kernel, hipStream_t s) {
+    prof::Pending p;
+    p.name = kernel;
+    {
+        std::lock_guard<std::mutex> lk(prof::mu);
+        if (!prof::pool.empty()) {
+            p.a = prof::pool.back().first;
+            p.b = prof::pool.back().second;
+            prof::pool.pop_back();
+        } else if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess) {
+            return;
+        }
+    }
+    if (hipEventRecord(p.a, s) != hipSuccess) return;
+    prof::open.push_back(p);
+}
+
+void prof_end(const char *kernel, hipStream_t s) {
+    for (size_t i = prof::open.size(); i-- > 0;) {
+        if (prof::open[i].name != kernel) continue;
+        prof::Pending p = prof::open[i];
+        prof::open.erase(prof::open.begin() + long(i));
+        if (hipEventRecord(p.b, s) != hipSuccess) return;
+        std::lock_guard<std::mutex> lk(prof::mu);
+        prof::pending.push_back(p);
+        return;
+    }
+}
+
 }  // namespace amh
 
 using amh::check_hip;
 using amh::set_error;
+
+extern "C" int amh_profile_enable(int on) {
+    amh::prof::on.store(on != 0);
+    return AMH_OK;
+}
+
+// Device time of the launches recorded under `kernel` since the last reset
+// (synchronous: waits for their events).
+extern "C" int amh_profile_read(const char *kernel, uint64_t *launches, double *ms) {
+    std::lock_guard<std::mutex> lk(amh::prof::mu);
+    for (auto &p : amh::prof::pending) {
+        float t = 0.0f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&t, p.a, p.b) == hipSuccess) {
+            auto &e = amh::prof::totals[p.name];
+            e.first += 1;
+            e.second += t;
+        }
+        amh::prof::pool.emplace_back(p.a, p.b);
+    }
+    amh::prof::pending.clear();
+    auto it = amh::prof::totals.find(kernel ? kernel : "");
+    if (launches) *launches = it == amh::prof::totals.end() ? 0 : it->second.first;
+    if (ms) *ms = it == amh::prof::totals.end() ? 0.0 : it->second.second;
+    return AMH_OK;
+}
+
+extern "C" int amh_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(amh::prof::mu);
+    for (auto &p : amh::prof::pending) amh::prof::pool.emplace_back(p.a, p.b);
+    amh::prof::pending.clear();
+    amh::prof::totals.clear();
+    return AMH_OK;
+}
 
 extern "C" {
 

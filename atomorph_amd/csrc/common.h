@@ -129,6 +129,22 @@ __device__ __forceinline__ void atomic_add_u128(unsigned long long *lo, unsigned
 namespace amh {
 void set_error(const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
+
+// Optional per-kernel device timing (amh_profile_enable): when on, the
+// entry points bracket their kernels with HIP events on the launch stream.
+bool profiling();
+void prof_begin(const char *kernel, hipStream_t s);
+void prof_end(const char *kernel, hipStream_t s);
+struct ProfScope {
+    const char *k;
+    hipStream_t s;
+    ProfScope(const char *kernel, hipStream_t st) : k(kernel), s(st) {
+        if (profiling()) prof_begin(k, s);
+    }
+    ~ProfScope() {
+        if (profiling()) prof_end(k, s);
+    }
+};
 }  // namespace amh
 
 #define AMH_LAUNCH_CHECK(what)                                              \

@@ -406,13 +406,25 @@ extern "C" int amh_fluid_step(void *fluid, int32_t steps_left, double freedom_ra
     // the grid's 13 fields are contiguous: clear them in one pass
     hipLaunchKernelGGL(k_fill_f32, dim3(uint32_t((nn * 13 + kT - 1) / kT)), dim3(kT), 0, s, f->G.m, nn * 13);
     if (f->count > 0) {
-        hipLaunchKernelGGL(k_p2g, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
-        hipLaunchKernelGGL(k_forces, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
+        {
+            amh::ProfScope ps_("k_p2g", s);
+            hipLaunchKernelGGL(k_p2g, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
+        }
+        {
+            amh::ProfScope ps_("k_forces", s);
+            hipLaunchKernelGGL(k_forces, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
+        }
         hipLaunchKernelGGL(k_node_div, dim3(gn), dim3(kT), 0, s, f->G.m, f->G.ax, f->G.ay, nn);
-        hipLaunchKernelGGL(k_g2p, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
+        {
+            amh::ProfScope ps_("k_g2p", s);
+            hipLaunchKernelGGL(k_g2p, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
+        }
         hipLaunchKernelGGL(k_node_div, dim3(gn), dim3(kT), 0, s, f->G.m, f->G.u, f->G.v, nn);
-        hipLaunchKernelGGL(k_advect, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy, int(steps_left),
-                           freedom_radius, seed);
+        {
+            amh::ProfScope ps_("k_advect", s);
+            hipLaunchKernelGGL(k_advect, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy, int(steps_left),
+                               freedom_radius, seed);
+        }
     }
     AMH_LAUNCH_CHECK("amh_fluid_step");
     return AMH_OK;
@@ -670,8 +682,11 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
     const uint32_t gpx = uint32_t((npx + kT - 1) / kT);
     if (F.fk < R.nkeys && f->count > 0) {
         const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
-        hipLaunchKernelGGL(k_fluid_update, dim3(gp), dim3(kT), 0, s, R, F, f->P, fa->rank_dev, fa->surface_dev,
-                           fa->blob_col_dev, fa->sparseness);
+        {
+            amh::ProfScope ps_("k_fluid_update", s);
+            hipLaunchKernelGGL(k_fluid_update, dim3(gp), dim3(kT), 0, s, R, F, f->P, fa->rank_dev, fa->surface_dev,
+                               fa->blob_col_dev, fa->sparseness);
+        }
         // morph.cpp:1050-1054
         const double freedom = sqrt(double(f->W * f->W + f->H * f->H)) * fa->sparseness / 4.0;
         const double radius = freedom * (1.0 - F.tl);
@@ -679,8 +694,11 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
             const int r = amh_fluid_step(f, fa->fluidsteps - (k + 1), radius, mix64(f->frame * 131 + k), stream);
             if (r) return r;
         }
-        hipLaunchKernelGGL(k_fluid_splat, dim3(gp), dim3(kT), 0, s, f->P, f->count, F.tl, f->W, f->H, fa->sparseness,
-                           R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->acc);
+        {
+            amh::ProfScope ps_("k_fluid_splat", s);
+            hipLaunchKernelGGL(k_fluid_splat, dim3(gp), dim3(kT), 0, s, f->P, f->count, F.tl, f->W, f->H, fa->sparseness,
+                               R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->acc);
+        }
     }
     hipLaunchKernelGGL(k_fluid_resolve, dim3(gpx), dim3(kT), 0, s, f->acc, npx, F.tl, R.keep_background, f->img,
                        f->mask);

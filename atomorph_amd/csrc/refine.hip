@@ -332,9 +332,12 @@ extern "C" int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot
     const int grid = std::min(tile_grid(tile), int(nitems));
     if (grid <= 0) return check_hip(hipGetLastError(), "amh_refine_batched occupancy");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(kTiles[tile].fn, dim3(grid), dim3(kTiles[tile].threads), 0, s, pts_dev, nkeys, wtot,
-                       chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed, queue_dev,
-                       stats_dev);
+    {
+        amh::ProfScope ps_("k_refine_pass", s);
+        hipLaunchKernelGGL(kTiles[tile].fn, dim3(grid), dim3(kTiles[tile].threads), 0, s, pts_dev, nkeys, wtot,
+                           chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed, queue_dev,
+                           stats_dev);
+    }
     AMH_LAUNCH_CHECK("amh_refine_batched");
     return AMH_OK;
 }
@@ -387,6 +390,7 @@ extern "C" int amh_graph_launch(void *graph, void *stream) {
         set_error("amh_graph_launch: NULL graph");
         return AMH_E_ARG;
     }
+    amh::ProfScope ps_("refine_graph", static_cast<hipStream_t>(stream));
     return check_hip(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
 }
 

@@ -512,8 +512,14 @@ extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t 
     const uint32_t gpx = uint32_t((L.npx + 255) / 256);
     const uint32_t gat = uint32_t((Rc.wtot + 255) / 256);
     if (Rc.wtot > 0) {
-        hipLaunchKernelGGL(k_prepare, dim3(gat), dim3(256), 0, s, Rc, F, recs, bmin);
-        hipLaunchKernelGGL(k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc, ovcount);
+        {
+            amh::ProfScope ps_("k_prepare", s);
+            hipLaunchKernelGGL(k_prepare, dim3(gat), dim3(256), 0, s, Rc, F, recs, bmin);
+        }
+        {
+            amh::ProfScope ps_("k_splat", s);
+            hipLaunchKernelGGL(k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc, ovcount);
+        }
     }
     hipLaunchKernelGGL(k_blk_sum, dim3(uint32_t(L.nblk)), dim3(256), 0, s, ovcount, L.npx, sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, int(L.nblk), offs);
@@ -523,8 +529,11 @@ extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t 
         hipLaunchKernelGGL(k_overflow, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, ovcount, ovstart,
                            ov);
     const int separate_clear = Rc.feather > 0 ? 1 : 0;
-    hipLaunchKernelGGL(k_composite, dim3(gpx), dim3(256), 0, s, Rc, F, bmin, acc, ovstart, ov, out,
-                       separate_clear ? 0 : 1);
+    {
+        amh::ProfScope ps_("k_composite", s);
+        hipLaunchKernelGGL(k_composite, dim3(gpx), dim3(256), 0, s, Rc, F, bmin, acc, ovstart, ov, out,
+                           separate_clear ? 0 : 1);
+    }
     if (separate_clear)
         hipLaunchKernelGGL(k_fill_i32, dim3(gpx), dim3(256), 0, s, bmin, L.npx, 0x7FFFFFFF);
     AMH_LAUNCH_CHECK("amh_render_frame");

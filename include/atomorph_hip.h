@@ -56,6 +56,15 @@ int amh_stream_sync(void *stream);
 int amh_timer_begin(void **timer, void *stream);
 int amh_timer_end(void *timer, void *stream, float *ms);
 int amh_timer_free(void *timer);
+/* Per-kernel device timing.  While enabled, the entry points bracket their
+ * main kernels (k_refine_pass, refine_graph, k_prepare, k_splat,
+ * k_composite, k_p2g, k_forces, k_g2p, k_advect, k_fluid_update,
+ * k_fluid_splat, k_blob_stats) with HIP events on the launch stream.
+ * amh_profile_read waits for them and returns the number of launches and
+ * the summed device milliseconds recorded under `kernel`. */
+int amh_profile_enable(int on);
+int amh_profile_read(const char *kernel, uint64_t *launches, double *ms);
+int amh_profile_reset(void);
 
 /* -------------------------------------------------------------- ingest -- */
 /* One key frame: per pixel, `stored` = what morph::add_pixel keeps
