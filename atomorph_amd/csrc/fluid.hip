@@ -16,10 +16,12 @@
 #include "colour.h"
 #include "common.h"
 #include "../../include/atomorph_hip.h"
+#include "scan.h"
 #include "shade.h"
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 using namespace amh;
@@ -33,12 +35,20 @@ struct Particles {  // structure of arrays, device
     double *x, *y, *u, *v, *gx, *gy, *fr;
     double *R, *G, *B, *A, *r, *g, *b, *a;
     double *strength;
+    double *u2, *v2;  // the velocity between G2P and advection
     int32_t *active, *mature, *owner, *key;  // owner: first at its source
+    int32_t *atom;  // in the cell-ordered copy: the particle (atom) in each slot
 };
+constexpr int kParticleDoubles = 18;
+constexpr int kParticleInts = 5;
 
-struct Grid {  // FP32 node fields, [gsx * gsy], node (i, j) at i * gsy + j
-    float *m, *d, *gx, *gy, *u, *v, *ax, *ay, *r, *g, *b, *a, *w;
+// FP32 node fields, [gsx * gsy], node (i, j) at i * gsy + j.  FluidModel's
+// node density accumulates the same weights as its mass, so one field (m)
+// serves both.
+struct Grid {
+    float *m, *gx, *gy, *u, *v, *ax, *ay, *r, *g, *b, *a, *w;
 };
+constexpr int kGridFields = 12;
 
 __device__ __forceinline__ void weights(double x, double y, int &cx, int &cy, double px[3], double py[3],
                                         double gx[3], double gy[3]) {
@@ -61,47 +71,153 @@ __device__ __forceinline__ void weights(double x, double y, int &cx, int &cy, do
 
 __device__ __forceinline__ bool node_ok(int i, int j, int gsx, int gsy) { return i >= 0 && j >= 0 && i < gsx && j < gsy; }
 
-__global__ void k_p2g(Particles P, int n, Grid Gd, int gsx, int gsy) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n || !P.active[i]) return;
-    int cx, cy;
-    double px[3], py[3], gx[3], gy[3];
-    weights(P.x[i], P.y[i], cx, cy, px, py, gx, gy);
-    const bool col = P.mature[i] && P.strength[i] > 0.0;
-    const double s = P.strength[i];
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-            if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
-            const int64_t k = int64_t(cx + a) * gsy + cy + b;
-            const double phi = px[a] * py[b];
-            atomicAdd(&Gd.m[k], float(phi));
-            atomicAdd(&Gd.d[k], float(phi));
-            atomicAdd(&Gd.gx[k], float(gx[a] * py[b]));
-            atomicAdd(&Gd.gy[k], float(px[a] * gy[b]));
-            if (col) {
-                // strength-weighted running mean == sum(s c) / sum(s)
-                atomicAdd(&Gd.r[k], float(s * P.r[i]));
-                atomicAdd(&Gd.g[k], float(s * P.g[i]));
-                atomicAdd(&Gd.b[k], float(s * P.b[i]));
-                atomicAdd(&Gd.a[k], float(s * P.a[i]));
-                atomicAdd(&Gd.w[k], float(s));
-            }
-        }
+// ---------------------------------------------------- stencil scatter
+// Particles scatter into the 3x3 nodes around them.  Neighbouring atoms of a
+// chain stay close together, so the nodes one block's particles touch form a
+// small box: the block sums its contributions into an LDS copy of that box
+// (LDS atomics) and then adds each touched node to the grid once per field.
+// That turns ~9 contended global atomics per particle and field into one per
+// node.  A block whose box exceeds the tile scatters straight to the grid.
+struct Box {
+    int x0, y0, w, h;
+    bool fits;
+};
+
+// Bounding box of the E x E stencils at (cx, cy) of the block's valid
+// particles; fits when it has at most NODES entries.
+template <int NODES, int E = 3>
+__device__ Box block_box(bool valid, int cx, int cy, int *red) {
+    int mnx = valid ? cx : 0x7FFFFFFF, mny = valid ? cy : 0x7FFFFFFF;
+    int mxx = valid ? cx : -0x7FFFFFFF, mxy = valid ? cy : -0x7FFFFFFF;
+    for (int off = 32; off > 0; off >>= 1) {
+        mnx = min(mnx, __shfl_xor(mnx, off, 64));
+        mny = min(mny, __shfl_xor(mny, off, 64));
+        mxx = max(mxx, __shfl_xor(mxx, off, 64));
+        mxy = max(mxy, __shfl_xor(mxy, off, 64));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wave * 4 + 0] = mnx;
+        red[wave * 4 + 1] = mny;
+        red[wave * 4 + 2] = mxx;
+        red[wave * 4 + 3] = mxy;
+    }
+    __syncthreads();
+    for (int w = 0; w < kT / 64; ++w) {
+        mnx = min(mnx, red[w * 4 + 0]);
+        mny = min(mny, red[w * 4 + 1]);
+        mxx = max(mxx, red[w * 4 + 2]);
+        mxy = max(mxy, red[w * 4 + 3]);
+    }
+    Box b;
+    b.x0 = mnx;
+    b.y0 = mny;
+    const int64_t w = int64_t(mxx) - mnx + E, h = int64_t(mxy) - mny + E;
+    b.fits = mnx <= mxx && w * h <= NODES;
+    b.w = b.fits ? int(w) : 0;
+    b.h = b.fits ? int(h) : 0;
+    return b;
+}
+
+// Runs of consecutive lanes whose particles share a stencil origin form
+// segments (the cell order makes such runs common).  seg_sum leaves each
+// segment's total in its last lane, so one lane per segment scatters.
+struct Seg {
+    int lane, start;  // start: first lane of this lane's segment
+    bool tail;        // last lane of its segment
+};
+
+__device__ __forceinline__ Seg segments(bool valid, int cx, int cy) {
+    Seg s;
+    s.lane = threadIdx.x & 63;
+    const uint64_t vm = __ballot(valid);
+    const int pcx = __shfl_up(cx, 1, 64), pcy = __shfl_up(cy, 1, 64);
+    const int ncx = __shfl_down(cx, 1, 64), ncy = __shfl_down(cy, 1, 64);
+    const bool pv = s.lane > 0 && ((vm >> (s.lane - 1)) & 1ull);
+    const bool nv = s.lane < 63 && ((vm >> (s.lane + 1)) & 1ull);
+    const bool head = !valid || !pv || pcx != cx || pcy != cy;
+    s.tail = !valid || !nv || ncx != cx || ncy != cy;
+    int st = head ? s.lane : 0;
+    for (int off = 1; off < 64; off <<= 1) st = max(st, __shfl_up(st, off, 64));
+    s.start = st;
+    return s;
+}
+
+// DPP move of v: row_shr:k (0x110 + k), row_bcast:15 (0x142) or
+// row_bcast:31 (0x143); lanes without a source (or outside row_mask) get 0.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
+}
+
+// Inclusive segmented sum over lanes [start, lane]; every lane of the wave
+// must take part.
+__device__ __forceinline__ float seg_sum(float v, const Seg &s) {
+    float t;
+    t = dpp<0x111, 0xF>(v);
+    if (s.lane - 1 >= s.start) v += t;
+    t = dpp<0x112, 0xF>(v);
+    if (s.lane - 2 >= s.start) v += t;
+    t = dpp<0x114, 0xF>(v);
+    if (s.lane - 4 >= s.start) v += t;
+    t = dpp<0x118, 0xF>(v);
+    if (s.lane - 8 >= s.start) v += t;
+    t = dpp<0x142, 0xA>(v);  // lane 15 -> row 1, lane 47 -> row 3
+    if ((s.lane & 16) && s.start <= (s.lane & ~15) - 1) v += t;
+    t = dpp<0x143, 0xC>(v);  // lane 31 -> rows 2 and 3
+    if (s.lane >= 32 && s.start <= 31) v += t;
+    return v;
+}
+
+// ------------------------------------------------------------ P2G
+// Mass (= density), density gradient and strength-weighted colour
+// (fluidmodel.cpp:191-249).  With e = x - cx - 1 in [-0.5, 0.5) the
+// quadratic B-spline weights and their gradients are polynomials in e:
+//   px[a] = sum_p kW[a][p] e^p,   gx[a] = sum_p kD[a][p] e^p,
+// so the node sums of px[a] py[b], gx[a] py[b] and px[a] gy[b] over the
+// particles of one cell follow from the cell's nine moments
+// M[p][q] = sum e^p h^q (h likewise in y).  A particle therefore adds 9
+// moments and 5 colour sums to its one cell instead of 8 values to each of 9
+// nodes; the nodes are assembled from the 3x3 cells around them.
+__constant__ float kW[3][3] = {{0.125f, -0.5f, 0.5f}, {0.75f, 0.0f, -1.0f}, {0.125f, 0.5f, 0.5f}};
+__constant__ float kD[3][3] = {{0.5f, -1.0f, 0.0f}, {0.0f, 2.0f, 0.0f}, {-0.5f, -1.0f, 0.0f}};
+constexpr int kCellSums = 14;  // 9 moments (p-major), then s*r, s*g, s*b, s*a, s
+
+// The 8 node values (m, gx, gy, r, g, b, a, w) that the sums c[] of cell
+// (node - (a, b)) contribute to the node.
+__device__ __forceinline__ void cell_to_node(const float *c, int a, int b, float *v) {
+    float xw[3], xd[3];  // sum_p kW[a][p] M[p][q], sum_p kD[a][p] M[p][q]
+    for (int q = 0; q < 3; ++q) {
+        xw[q] = kW[a][0] * c[q] + kW[a][1] * c[3 + q] + kW[a][2] * c[6 + q];
+        xd[q] = kD[a][0] * c[q] + kD[a][1] * c[3 + q] + kD[a][2] * c[6 + q];
+    }
+    v[0] = kW[b][0] * xw[0] + kW[b][1] * xw[1] + kW[b][2] * xw[2];
+    v[1] = kW[b][0] * xd[0] + kW[b][1] * xd[1] + kW[b][2] * xd[2];
+    v[2] = kD[b][0] * xw[0] + kD[b][1] * xw[1] + kD[b][2] * xw[2];
+    for (int k = 0; k < 5; ++k) v[3 + k] = c[9 + k];
 }
 
 __device__ __forceinline__ float nd(const float *f, int i, int j, int gsx, int gsy) {
     return node_ok(i, j, gsx, gsy) ? f[int64_t(i) * gsy + j] : 0.0f;
 }
 
+// A node's acceleration and velocity as FluidModel reads them after dividing
+// them by the node mass where it is positive (fluidmodel.cpp:359-364,
+// 437-443); the readers divide, so no pass over the grid is needed.
+__device__ __forceinline__ float per_mass(float v, float m) { return m > 0.0f ? v / m : v; }
+__device__ __forceinline__ float nd_m(const float *f, const float *m, int i, int j, int gsx, int gsy) {
+    if (!node_ok(i, j, gsx, gsy)) return 0.0f;
+    const int64_t k = int64_t(i) * gsy + j;
+    return per_mass(f[k], m[k]);
+}
+
 // pressure from the bicubic density interpolation, wall forces
 // (fluidmodel.cpp:251-357)
-__global__ void k_forces(Particles P, int n, Grid Gd, int gsx, int gsy) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n || !P.active[i]) return;
-    const double x = P.x[i], y = P.y[i];
+__device__ __forceinline__ void particle_forces(double x, double y, const Grid &Gd, int gsx, int gsy,
+                                                double &pressure_out, double &fx_out, double &fy_out) {
     const int cx = static_cast<int>(x), cy = static_cast<int>(y);
-    const double d01 = nd(Gd.d, cx, cy, gsx, gsy), d02 = nd(Gd.d, cx, cy + 1, gsx, gsy);
-    const double d11 = nd(Gd.d, cx + 1, cy, gsx, gsy), d12 = nd(Gd.d, cx + 1, cy + 1, gsx, gsy);
+    const double d01 = nd(Gd.m, cx, cy, gsx, gsy), d02 = nd(Gd.m, cx, cy + 1, gsx, gsy);
+    const double d11 = nd(Gd.m, cx + 1, cy, gsx, gsy), d12 = nd(Gd.m, cx + 1, cy + 1, gsx, gsy);
     const double gx01 = nd(Gd.gx, cx, cy, gsx, gsy), gx02 = nd(Gd.gx, cx, cy + 1, gsx, gsy);
     const double gx11 = nd(Gd.gx, cx + 1, cy, gsx, gsy), gx12 = nd(Gd.gx, cx + 1, cy + 1, gsx, gsy);
     const double gy01 = nd(Gd.gy, cx, cy, gsx, gsy), gy02 = nd(Gd.gy, cx, cy + 1, gsx, gsy);
@@ -129,27 +245,9 @@ __global__ void k_forces(Particles P, int n, Grid Gd, int gsx, int gsy) {
     else if (x > gsx - 5) fx += (gsx - 5 - x);
     if (y < 4.0) fy += (4.0 - y);
     else if (y > gsy - 5) fy += (gsy - 5 - y);
-    int pcx, pcy;
-    double px[3], py[3], gx[3], gy[3];
-    weights(x, y, pcx, pcy, px, py, gx, gy);
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-            if (!node_ok(pcx + a, pcy + b, gsx, gsy)) continue;
-            const int64_t k = int64_t(pcx + a) * gsy + pcy + b;
-            const double phi = px[a] * py[b];
-            atomicAdd(&Gd.ax[k], float(-((gx[a] * py[b]) * pressure) + fx * phi));
-            atomicAdd(&Gd.ay[k], float(-((px[a] * gy[b]) * pressure) + fy * phi));
-        }
-}
-
-__global__ void k_node_div(const float *m, float *f1, float *f2, int64_t nn) {
-    const int64_t k = int64_t(blockIdx.x) * kT + threadIdx.x;
-    if (k >= nn) return;
-    const float mm = m[k];
-    if (mm > 0.0f) {
-        f1[k] /= mm;
-        f2[k] /= mm;
-    }
+    pressure_out = pressure;
+    fx_out = fx;
+    fy_out = fy;
 }
 
 // Move (x1, y1) towards (x2, y2) by `a` (capped at the distance), the
@@ -171,38 +269,278 @@ __device__ __forceinline__ void toward(double x1, double y1, double x2, double y
     }
 }
 
-__global__ void k_g2p(Particles P, int n, Grid Gd, int gsx, int gsy) {
+// ------------------------------------------------------------------ tiles
+// The grid's nodes are owned by kTile x kTile tiles.  Each step every active
+// particle is binned into each tile that holds a node of its 3x3 stencil (1,
+// 2 or 4 tiles; a counting sort of slot numbers); the entry for the tile of
+// its own cell is marked as its home.  The block of a tile reads its bin,
+// sums what those particles add to its nodes in LDS and stores every node
+// once.  No phase of the step uses a global atomic (device-wide atomics are
+// performed past the per-XCD L2) and no phase needs the grid cleared first.
+constexpr int kTile = 8;
+constexpr int kReach = kTile + 2;       // cells per axis that reach a tile's nodes
+constexpr int kHome = int(0x80000000);  // bin entry flag: the particle's home tile
+
+struct Bins {
+    int32_t ntx, nty;  // tiles per axis
+    int32_t *count;    // [tiles]; zero between steps
+    int32_t *start;    // [tiles + 1]
+    int32_t *rank;     // [4][n]: place of particle i's j-th entry in its bin
+    int32_t *list;     // [<= 4n] slot | kHome, grouped by bin
+};
+
+// The tiles holding the stencil nodes of a particle at (x, y): tx0..tx1 x
+// ty0..ty1 (nodes clamped to the grid); (tx0, ty0) is its home.
+__device__ __forceinline__ void stencil_tiles(double x, double y, int gsx, int gsy, int &tx0, int &tx1, int &ty0,
+                                              int &ty1) {
+    const int cx = static_cast<int>(x - 0.5), cy = static_cast<int>(y - 0.5);
+    tx0 = min(max(cx, 0), gsx - 1) / kTile;
+    tx1 = min(max(cx + 2, 0), gsx - 1) / kTile;
+    ty0 = min(max(cy, 0), gsy - 1) / kTile;
+    ty1 = min(max(cy + 2, 0), gsy - 1) / kTile;
+}
+
+// Entry j (0..3) of a particle: tile (tx0 + (j >> 1), ty0 + (j & 1)) when it
+// exists, else -1.
+__device__ __forceinline__ int entry_tile(int j, int tx0, int tx1, int ty0, int ty1, int nty) {
+    const int bx = tx0 + (j >> 1), by = ty0 + (j & 1);
+    return (bx <= tx1 && by <= ty1) ? bx * nty + by : -1;
+}
+
+__global__ __launch_bounds__(kT) void k_bin_count(Particles P, int n, int gsx, int gsy, Bins B) {
     const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n || !P.active[i]) return;
-    int cx, cy;
-    double px[3], py[3], gx[3], gy[3];
-    const double x = P.x[i], y = P.y[i];
-    weights(x, y, cx, cy, px, py, gx, gy);
-    double cax, cay;  // attractor, 0.03 per step (fluidmodel.cpp:383-410)
-    toward(x, y, P.gx[i], P.gy[i], 0.03, cax, cay);
-    double u = P.u[i], v = P.v[i];
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-            const double phi = px[a] * py[b];
-            const double ax = nd(Gd.ax, cx + a, cy + b, gsx, gsy), ay = nd(Gd.ay, cx + a, cy + b, gsx, gsy);
-            u += phi * (ax + cax);
-            v += phi * (ay + cay);
+    const bool act = i < n && P.active[i];
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    if (act) stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
+    const int lane = threadIdx.x & 63;
+    for (int j = 0; j < 4; ++j) {
+        const int t = act ? entry_tile(j, tx0, tx1, ty0, ty1, B.nty) : -1;
+        // one atomic per distinct bin of the wave
+        uint64_t todo = __ballot(t >= 0);
+        int rank = 0;
+        while (todo) {
+            const int leader = __ffsll((unsigned long long)todo) - 1;
+            const int tl = __shfl(t, leader, 64);
+            const uint64_t same = __ballot(t == tl);
+            int base = 0;
+            if (lane == leader) base = atomicAdd(&B.count[tl], int(__popcll(same)));
+            base = __shfl(base, leader, 64);
+            if (t == tl) rank = base + int(__popcll(same & ((1ull << lane) - 1ull)));
+            todo &= ~same;
         }
-    P.u[i] = u;
-    P.v[i] = v;
-    double mu = u, mv = v;
-    if (!P.mature[i]) {
-        mu *= 0.0;
-        mv *= 0.0;
+        if (i < n) B.rank[int64_t(j) * n + i] = rank;
     }
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-            if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
-            const int64_t k = int64_t(cx + a) * gsy + cy + b;
-            const double phi = px[a] * py[b];
-            atomicAdd(&Gd.u[k], float(phi * mu));
-            atomicAdd(&Gd.v[k], float(phi * mv));
+}
+
+// Particles into their bins (the tiles recomputed from the unchanged
+// positions); clears the counts for the next step.  The grid covers
+// max(n, tiles) threads.
+__global__ __launch_bounds__(kT) void k_bin_place(Particles P, int n, int gsx, int gsy, Bins B, int tiles) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i < tiles) B.count[i] = 0;
+    if (i >= n || !P.active[i]) return;
+    int tx0, tx1, ty0, ty1;
+    stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
+    for (int j = 0; j < 4; ++j) {
+        const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
+        if (t >= 0) B.list[B.start[t] + B.rank[int64_t(j) * n + i]] = i | (j == 0 ? kHome : 0);
+    }
+}
+
+// Calls body(slot, home) for the entries of bin t, 256 at a time: slot is -1
+// for lanes past the end.  Trip counts are uniform over the block.
+template <class Body>
+__device__ __forceinline__ void for_bin(const Bins &B, int t, const Body &body) {
+    const int s0 = B.start[t], s1 = B.start[t + 1];
+    for (int base = s0; base < s1; base += kT) {
+        const int k = base + int(threadIdx.x);
+        int slot = -1;
+        bool home = false;
+        if (k < s1) {
+            const int e = B.list[k];
+            slot = e & 0x7FFFFFFF;
+            home = e < 0;
         }
+        body(slot, home);
+    }
+}
+
+__device__ __forceinline__ bool in_reach(int cx, int cy, int x0, int y0) {
+    return cx >= x0 - 2 && cx < x0 + kTile && cy >= y0 - 2 && cy < y0 + kTile;
+}
+
+// Mass (= density), density gradient and strength-weighted colour of the
+// tile's nodes from the moments of the cells that reach them.
+__global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
+    __shared__ float cs[kCellSums][kReach * kReach];
+    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
+    const int x0 = tx * kTile, y0 = ty * kTile;
+    for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
+    __syncthreads();
+    for_bin(B, int(blockIdx.x), [&](int slot, bool) {
+        int cx = 0, cy = 0;
+        bool valid = false;
+        float c[kCellSums];
+        for (int k = 0; k < kCellSums; ++k) c[k] = 0.0f;
+        if (slot >= 0) {
+            const double x = P.x[slot], y = P.y[slot];
+            cx = static_cast<int>(x - 0.5);
+            cy = static_cast<int>(y - 0.5);
+            valid = in_reach(cx, cy, x0, y0);
+            if (valid) {
+                const double e = x - cx - 1.0, h = y - cy - 1.0;
+                const double ep[3] = {1.0, e, e * e}, hp[3] = {1.0, h, h * h};
+                for (int p = 0; p < 3; ++p)
+                    for (int q = 0; q < 3; ++q) c[p * 3 + q] = float(ep[p] * hp[q]);
+                const double s = P.strength[slot];
+                if (P.mature[slot] && s > 0.0) {
+                    // strength-weighted running mean == sum(s c) / sum(s)
+                    c[9] = float(s * P.r[slot]);
+                    c[10] = float(s * P.g[slot]);
+                    c[11] = float(s * P.b[slot]);
+                    c[12] = float(s * P.a[slot]);
+                    c[13] = float(s);
+                }
+            }
+        }
+        const Seg S = segments(valid, cx, cy);
+        for (int k = 0; k < kCellSums; ++k) c[k] = seg_sum(c[k], S);
+        if (valid && S.tail) {
+            const int idx = (cx - x0 + 2) * kReach + (cy - y0 + 2);
+            for (int k = 0; k < kCellSums; ++k)
+                if (c[k] != 0.0f) atomicAdd(&cs[k][idx], c[k]);
+        }
+    });
+    __syncthreads();
+    for (int t = threadIdx.x; t < kTile * kTile; t += kT) {
+        const int ni = x0 + t / kTile, nj = y0 + t % kTile;
+        if (ni >= gsx || nj >= gsy) continue;
+        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                const int cell = (ni - a - x0 + 2) * kReach + (nj - b - y0 + 2);
+                if (cs[0][cell] == 0.0f) continue;  // no particle (each adds 1 to M[0][0])
+                float m[kCellSums];
+                for (int k = 0; k < kCellSums; ++k) m[k] = cs[k][cell];
+                float v[8];
+                cell_to_node(m, a, b, v);
+                for (int f = 0; f < 8; ++f) acc[f] += v[f];
+            }
+        const int64_t k = int64_t(ni) * gsy + nj;
+        Gd.m[k] = acc[0]; Gd.gx[k] = acc[1]; Gd.gy[k] = acc[2];
+        Gd.r[k] = acc[3]; Gd.g[k] = acc[4]; Gd.b[k] = acc[5]; Gd.a[k] = acc[6]; Gd.w[k] = acc[7];
+    }
+}
+
+// Sums v (zeros for lanes that do not contribute) over runs of lanes with
+// the same stencil origin and adds the run totals to node (cx + a, cy + b)
+// of the tile's LDS sums when that node belongs to the tile.
+__device__ __forceinline__ void tile_add2(float (&acc)[2][kTile * kTile], const Seg &S, bool valid, int cx, int cy,
+                                          int x0, int y0, int a, int b, float v0, float v1) {
+    v0 = seg_sum(v0, S);
+    v1 = seg_sum(v1, S);
+    const int li = cx + a - x0, lj = cy + b - y0;
+    if (valid && S.tail && li >= 0 && li < kTile && lj >= 0 && lj < kTile) {
+        if (v0 != 0.0f) atomicAdd(&acc[0][li * kTile + lj], v0);
+        if (v1 != 0.0f) atomicAdd(&acc[1][li * kTile + lj], v1);
+    }
+}
+
+__device__ __forceinline__ void store_tile2(const float (&acc)[2][kTile * kTile], float *f0, float *f1, int x0, int y0,
+                                            int gsx, int gsy) {
+    for (int t = threadIdx.x; t < kTile * kTile; t += kT) {
+        const int ni = x0 + t / kTile, nj = y0 + t % kTile;
+        if (ni >= gsx || nj >= gsy) continue;
+        const int64_t k = int64_t(ni) * gsy + nj;
+        f0[k] = acc[0][t];
+        f1[k] = acc[1][t];
+    }
+}
+
+// Node accelerations: pressure and wall forces (fluidmodel.cpp:251-357).
+__global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
+    __shared__ float acc[2][kTile * kTile];
+    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
+    const int x0 = tx * kTile, y0 = ty * kTile;
+    for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
+    __syncthreads();
+    for_bin(B, int(blockIdx.x), [&](int slot, bool) {
+        int cx = 0, cy = 0;
+        bool valid = false;
+        double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0};
+        double pressure = 0.0, fx = 0.0, fy = 0.0;
+        if (slot >= 0) {
+            const double x = P.x[slot], y = P.y[slot];
+            weights(x, y, cx, cy, px, py, gx, gy);
+            valid = in_reach(cx, cy, x0, y0);
+            if (valid) particle_forces(x, y, Gd, gsx, gsy, pressure, fx, fy);
+        }
+        const Seg S = segments(valid, cx, cy);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                const double phi = px[a] * py[b];
+                const float v0 = valid ? float(-((gx[a] * py[b]) * pressure) + fx * phi) : 0.0f;
+                const float v1 = valid ? float(-((px[a] * gy[b]) * pressure) + fy * phi) : 0.0f;
+                tile_add2(acc, S, valid, cx, cy, x0, y0, a, b, v0, v1);
+            }
+    });
+    __syncthreads();
+    store_tile2(acc, Gd.ax, Gd.ay, x0, y0, gsx, gsy);
+}
+
+// Particle velocity from node accelerations plus the attractor (written by
+// the particle's home tile), then the momentum of the tile's nodes
+// (fluidmodel.cpp:359-437).
+__global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
+    __shared__ float acc[2][kTile * kTile];
+    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
+    const int x0 = tx * kTile, y0 = ty * kTile;
+    for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
+    __syncthreads();
+    for_bin(B, int(blockIdx.x), [&](int slot, bool home) {
+        int cx = 0, cy = 0;
+        bool valid = false;
+        double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3], gy[3];
+        double mu = 0.0, mv = 0.0;
+        if (slot >= 0) {
+            const double x = P.x[slot], y = P.y[slot];
+            weights(x, y, cx, cy, px, py, gx, gy);
+            valid = in_reach(cx, cy, x0, y0);
+            if (valid || home) {
+                double cax, cay;  // attractor, 0.03 per step (fluidmodel.cpp:383-410)
+                toward(x, y, P.gx[slot], P.gy[slot], 0.03, cax, cay);
+                double u = P.u[slot], v = P.v[slot];
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) {
+                        const double phi = px[a] * py[b];
+                        const double ax = nd_m(Gd.ax, Gd.m, cx + a, cy + b, gsx, gsy);
+                        const double ay = nd_m(Gd.ay, Gd.m, cx + a, cy + b, gsx, gsy);
+                        u += phi * (ax + cax);
+                        v += phi * (ay + cay);
+                    }
+                if (home) {  // other tiles still read the old velocity
+                    P.u2[slot] = u;
+                    P.v2[slot] = v;
+                }
+                mu = u;
+                mv = v;
+                if (!P.mature[slot]) {
+                    mu *= 0.0;
+                    mv *= 0.0;
+                }
+            }
+        }
+        const Seg S = segments(valid, cx, cy);
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                const double phi = px[a] * py[b];
+                const float v0 = valid ? float(phi * mu) : 0.0f;
+                const float v1 = valid ? float(phi * mv) : 0.0f;
+                tile_add2(acc, S, valid, cx, cy, x0, y0, a, b, v0, v1);
+            }
+    });
+    __syncthreads();
+    store_tile2(acc, Gd.u, Gd.v, x0, y0, gsx, gsy);
 }
 
 __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
@@ -210,10 +548,13 @@ __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
     return double(mix64(seed ^ (uint64_t(i) << 3) ^ uint64_t(k)) >> 11) * (1.0 / 9007199254740992.0) * 0.01;
 }
 
+// atom: the particle held in each slot when P is the cell-ordered copy
+// (NULL: slot i holds particle i); the wall jitter is keyed by particle.
 __global__ void k_advect(Particles P, int n, Grid Gd, int gsx, int gsy, int steps_left, double freedom_radius,
-                         uint64_t seed) {
+                         uint64_t seed, const int32_t *__restrict__ atom) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= n || !P.active[i]) return;
+    const int id = atom ? atom[i] : i;
     int cx, cy;
     double px[3], py[3], gx[3], gy[3];
     double x = P.x[i], y = P.y[i];
@@ -224,8 +565,9 @@ __global__ void k_advect(Particles P, int n, Grid Gd, int gsx, int gsy, int step
             if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
             const int64_t k = int64_t(cx + a) * gsy + cy + b;
             const double phi = px[a] * py[b];
-            gu += phi * Gd.u[k];
-            gv += phi * Gd.v[k];
+            const float mk = Gd.m[k];
+            gu += phi * per_mass(Gd.u[k], mk);
+            gv += phi * per_mass(Gd.v[k], mk);
             const double w = Gd.w[k];
             if (w > 0.0) {
                 // node colour = sum(s c)/sum(s); weighted by sum(s)
@@ -262,19 +604,82 @@ __global__ void k_advect(Particles P, int n, Grid Gd, int gsx, int gsy, int step
             y += my * w;
         }
     }
-    double u = P.u[i], v = P.v[i];
+    double u = P.u2[i], v = P.v2[i];
     u += gu - u;
     v += gv - v;
-    if (x < 1.0) { x = 1.0 + jitter(seed, i, 0); u = 0.0; }
-    else if (x > gsx - 2) { x = gsx - 2 - jitter(seed, i, 1); u = 0.0; }
-    if (y < 1.0) { y = 1.0 + jitter(seed, i, 2); v = 0.0; }
-    else if (y > gsy - 2) { y = gsy - 2 - jitter(seed, i, 3); v = 0.0; }
+    if (x < 1.0) { x = 1.0 + jitter(seed, id, 0); u = 0.0; }
+    else if (x > gsx - 2) { x = gsx - 2 - jitter(seed, id, 1); u = 0.0; }
+    if (y < 1.0) { y = 1.0 + jitter(seed, id, 2); v = 0.0; }
+    else if (y > gsy - 2) { y = gsy - 2 - jitter(seed, id, 3); v = 0.0; }
     P.x[i] = x; P.y[i] = y; P.u[i] = u; P.v[i] = v;
 }
 
-__global__ void k_fill_f32(float *p, int64_t n) {
+__global__ void k_fill_i32(int32_t *p, int64_t n) {
     const int64_t k = int64_t(blockIdx.x) * kT + threadIdx.x;
-    if (k < n) p[k] = 0.0f;
+    if (k < n) p[k] = 0;
+}
+
+// --------------------------------------------------------- cell order
+// Once per frame the particles are put in cell order (a counting sort by
+// the Morton rank of the cell of their stencil origin), so a block's
+// particles fill a compact patch of the grid: the LDS tiles of the scatters
+// stay small and the node reads of a wave share cache lines.  Inactive
+// particles go to one bucket after the last cell.
+__device__ __forceinline__ int cell_of(double x, double y, int gsx, int gsy) {
+    const int cx = min(max(static_cast<int>(x - 0.5), 0), gsx - 1);
+    const int cy = min(max(static_cast<int>(y - 0.5), 0), gsy - 1);
+    return cx * gsy + cy;
+}
+
+__global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, int gsy,
+                                                  const int32_t *__restrict__ cell_rank, int32_t *__restrict__ cell,
+                                                  int32_t *__restrict__ pos, int32_t *__restrict__ cnt) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    const bool in = i < n;
+    const bool act = in && P.active[i];
+    const int nn = gsx * gsy;
+    // the inactive bucket is one counter: a wave takes its slots at once
+    const uint64_t idle = __ballot(in && !act);
+    const int lane = threadIdx.x & 63;
+    const int leader = idle ? __ffsll((unsigned long long)idle) - 1 : 0;
+    int base = 0;
+    if (idle && lane == leader) base = atomicAdd(&cnt[nn], __popcll(idle));
+    base = __shfl(base, leader, 64);
+    if (!in) return;
+    if (act) {
+        const int c = cell_rank[cell_of(P.x[i], P.y[i], gsx, gsy)];
+        cell[i] = c;
+        pos[i] = atomicAdd(&cnt[c], 1);
+    } else {
+        cell[i] = nn;
+        pos[i] = base + __popcll(idle & ((1ull << lane) - 1ull));
+    }
+}
+
+__global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, const int32_t *__restrict__ cell,
+                                                   const int32_t *__restrict__ pos,
+                                                   const int32_t *__restrict__ start, Particles S) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    const int s = start[cell[i]] + pos[i];
+    S.x[s] = P.x[i]; S.y[s] = P.y[i]; S.u[s] = P.u[i]; S.v[s] = P.v[i];
+    S.gx[s] = P.gx[i]; S.gy[s] = P.gy[i]; S.fr[s] = P.fr[i];
+    S.R[s] = P.R[i]; S.G[s] = P.G[i]; S.B[s] = P.B[i]; S.A[s] = P.A[i];
+    S.r[s] = P.r[i]; S.g[s] = P.g[i]; S.b[s] = P.b[i]; S.a[s] = P.a[i];
+    S.strength[s] = P.strength[i];
+    S.active[s] = P.active[i];
+    S.mature[s] = P.mature[i];
+    S.atom[s] = i;
+}
+
+// What the steps and the splat change goes back to particle order.
+__global__ __launch_bounds__(kT) void k_sort_scatter(Particles S, int n, Particles P) {
+    const int s = blockIdx.x * kT + threadIdx.x;
+    if (s >= n || !S.active[s]) return;
+    const int i = S.atom[s];
+    P.x[i] = S.x[s]; P.y[i] = S.y[s]; P.u[i] = S.u[s]; P.v[i] = S.v[s];
+    P.r[i] = S.r[s]; P.g[i] = S.g[s]; P.b[i] = S.b[s]; P.a[i] = S.a[s];
+    P.mature[i] = S.mature[s];
 }
 
 }  // namespace
@@ -283,8 +688,15 @@ __global__ void k_fill_f32(float *p, int64_t n) {
 struct amh_fluid_s {
     int32_t gsx = 0, gsy = 0, count = 0;
     void *buf = nullptr;       // one allocation for particles + grid + render
-    Particles P;
+    Particles P;               // particle i = atom i
+    Particles S;               // the frame's cell-ordered copy
     Grid G;
+    // cell ordering: per particle its cell and rank in it; per cell counts
+    // and starts ([nn + 1]: the last bucket holds the inactive particles)
+    int32_t *cell = nullptr, *pos = nullptr, *cnt = nullptr, *start = nullptr;
+    int32_t *scan_sums = nullptr, *scan_offs = nullptr;
+    int32_t *cell_rank = nullptr;  // [nn] sort key of each cell (Morton order)
+    Bins bins;                     // per-step binning by node tile
     // render buffers
     double *acc = nullptr;     // [npx][5] sum w, sum w*c
     int32_t *mask = nullptr;   // [npx] blob membership / feather state
@@ -296,8 +708,20 @@ struct amh_fluid_s {
 namespace {
 size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     const size_t n = size_t(count), nn = size_t(gsx) * gsy, npx = size_t(W) * H;
+    const size_t nb = size_t(scan::blocks(int64_t(nn) + 1));
+    const size_t tiles = size_t((gsx + kTile - 1) / kTile) * size_t((gsy + kTile - 1) / kTile);
     (void)nkeys;
-    return n * 8 * 16 + n * 4 * 4 + nn * 4 * 13 + npx * 8 * 5 + npx * 4 * 2 + 8192;
+    return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + 2) * 4 * 2 +
+           (2 * nb + 1) * 4 + nn * 4 + (2 * tiles + 1) * 4 + n * 4 * 8 + nn * 4 * kGridFields + npx * 8 * 5 +
+           npx * 4 * 2 + 64 * 256;
+}
+
+void carve_particles(Particles &Q, size_t n, const std::function<char *(size_t)> &take) {
+    double **d[kParticleDoubles] = {&Q.x, &Q.y, &Q.u, &Q.v, &Q.gx, &Q.gy, &Q.fr, &Q.R, &Q.G,
+                                    &Q.B, &Q.A, &Q.r, &Q.g, &Q.b, &Q.a, &Q.strength, &Q.u2, &Q.v2};
+    for (double **q : d) *q = reinterpret_cast<double *>(take(n * 8));
+    int32_t **ints[kParticleInts] = {&Q.active, &Q.mature, &Q.owner, &Q.key, &Q.atom};
+    for (int32_t **q : ints) *q = reinterpret_cast<int32_t *>(take(n * 4));
 }
 }  // namespace
 
@@ -317,21 +741,54 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     char *p = static_cast<char *>(f->buf);
     auto take = [&](size_t b) { char *q = p; p += (b + 255) & ~size_t(255); return q; };
     const size_t n = size_t(count), nn = size_t(gsx) * gsy, npx = size_t(width) * height;
-    double **d[] = {&f->P.x, &f->P.y, &f->P.u, &f->P.v, &f->P.gx, &f->P.gy, &f->P.fr, &f->P.R, &f->P.G, &f->P.B,
-                    &f->P.A, &f->P.r, &f->P.g, &f->P.b, &f->P.a, &f->P.strength};
-    for (double **q : d) *q = reinterpret_cast<double *>(take(n * 8));
-    f->P.active = reinterpret_cast<int32_t *>(take(n * 4));
-    f->P.mature = reinterpret_cast<int32_t *>(take(n * 4));
-    f->P.owner = reinterpret_cast<int32_t *>(take(n * 4));
-    f->P.key = reinterpret_cast<int32_t *>(take(n * 4));
-    float **g[] = {&f->G.m, &f->G.d, &f->G.gx, &f->G.gy, &f->G.u, &f->G.v, &f->G.ax, &f->G.ay,
-                   &f->G.r, &f->G.g, &f->G.b, &f->G.a, &f->G.w};
-    float *grid = reinterpret_cast<float *>(take(nn * 4 * 13));  // contiguous: cleared in one pass
-    for (int k = 0; k < 13; ++k) *g[k] = grid + size_t(k) * nn;
+    carve_particles(f->P, n, take);
+    carve_particles(f->S, n, take);
+    f->cell = reinterpret_cast<int32_t *>(take(n * 4));
+    f->pos = reinterpret_cast<int32_t *>(take(n * 4));
+    f->cnt = reinterpret_cast<int32_t *>(take((nn + 2) * 4));
+    f->start = reinterpret_cast<int32_t *>(take((nn + 2) * 4));
+    const size_t nb = size_t(scan::blocks(int64_t(nn) + 1));
+    f->scan_sums = reinterpret_cast<int32_t *>(take(nb * 4));
+    f->scan_offs = reinterpret_cast<int32_t *>(take((nb + 1) * 4));
+    f->cell_rank = reinterpret_cast<int32_t *>(take(nn * 4));
+    f->bins.ntx = (gsx + kTile - 1) / kTile;
+    f->bins.nty = (gsy + kTile - 1) / kTile;
+    const size_t tiles = size_t(f->bins.ntx) * size_t(f->bins.nty);
+    f->bins.count = reinterpret_cast<int32_t *>(take(tiles * 4));
+    f->bins.start = reinterpret_cast<int32_t *>(take((tiles + 1) * 4));
+    f->bins.rank = reinterpret_cast<int32_t *>(take(n * 4 * 4));
+    f->bins.list = reinterpret_cast<int32_t *>(take(n * 4 * 4));
+    float **g[kGridFields] = {&f->G.m, &f->G.gx, &f->G.gy, &f->G.u, &f->G.v, &f->G.ax,
+                              &f->G.ay, &f->G.r, &f->G.g, &f->G.b, &f->G.a, &f->G.w};
+    float *grid = reinterpret_cast<float *>(take(nn * 4 * kGridFields));
+    for (int k = 0; k < kGridFields; ++k) *g[k] = grid + size_t(k) * nn;
     f->acc = reinterpret_cast<double *>(take(npx * 8 * 5));
     f->mask = reinterpret_cast<int32_t *>(take(npx * 4));
     f->img = reinterpret_cast<uint32_t *>(take(npx * 4));
-    if (int r = check_hip(hipMemsetAsync(f->buf, 0, bytes, static_cast<hipStream_t>(stream)), "amh_fluid_create")) {
+    // cells ranked in Morton (Z) order of (x, y): a run of consecutive ranks
+    // is a compact patch of the grid
+    std::vector<int32_t> by_code(nn), rank(nn);
+    std::vector<uint32_t> code(nn);
+    auto spread = [](uint32_t v) {
+        v &= 0xFFFFu;
+        v = (v | (v << 8)) & 0x00FF00FFu;
+        v = (v | (v << 4)) & 0x0F0F0F0Fu;
+        v = (v | (v << 2)) & 0x33333333u;
+        v = (v | (v << 1)) & 0x55555555u;
+        return v;
+    };
+    for (size_t c = 0; c < nn; ++c) {
+        code[c] = spread(uint32_t(c / size_t(gsy))) | (spread(uint32_t(c % size_t(gsy))) << 1);
+        by_code[c] = int32_t(c);
+    }
+    std::sort(by_code.begin(), by_code.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
+    for (size_t r = 0; r < nn; ++r) rank[by_code[r]] = int32_t(r);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int r = check_hip(hipMemsetAsync(f->buf, 0, bytes, st), "amh_fluid_create");
+    if (!r) r = check_hip(hipMemcpyAsync(f->cell_rank, rank.data(), nn * 4, hipMemcpyHostToDevice, st),
+                          "amh_fluid_create");
+    if (!r) r = check_hip(hipStreamSynchronize(st), "amh_fluid_create");
+    if (r) {
         (void)hipFree(f->buf);
         delete f;
         return r;
@@ -396,38 +853,66 @@ extern "C" int amh_fluid_get_particles(void *fluid, double *host, void *stream) 
     return AMH_OK;
 }
 
-// One FluidModel::step (fluidmodel.cpp:165-580).
-extern "C" int amh_fluid_step(void *fluid, int32_t steps_left, double freedom_radius, uint64_t seed, void *stream) {
-    amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);
-    if (!f) { set_error("amh_fluid_step: NULL"); return AMH_E_ARG; }
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const int64_t nn = int64_t(f->gsx) * f->gsy;
-    const uint32_t gn = uint32_t((nn + kT - 1) / kT), gp = uint32_t((f->count + kT - 1) / kT);
-    // the grid's 13 fields are contiguous: clear them in one pass
-    hipLaunchKernelGGL(k_fill_f32, dim3(uint32_t((nn * 13 + kT - 1) / kT)), dim3(kT), 0, s, f->G.m, nn * 13);
-    if (f->count > 0) {
-        {
-            amh::ProfScope ps_("k_p2g", s);
-            hipLaunchKernelGGL(k_p2g, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
-        }
-        {
-            amh::ProfScope ps_("k_forces", s);
-            hipLaunchKernelGGL(k_forces, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
-        }
-        hipLaunchKernelGGL(k_node_div, dim3(gn), dim3(kT), 0, s, f->G.m, f->G.ax, f->G.ay, nn);
-        {
-            amh::ProfScope ps_("k_g2p", s);
-            hipLaunchKernelGGL(k_g2p, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy);
-        }
-        hipLaunchKernelGGL(k_node_div, dim3(gn), dim3(kT), 0, s, f->G.m, f->G.u, f->G.v, nn);
-        {
-            amh::ProfScope ps_("k_advect", s);
-            hipLaunchKernelGGL(k_advect, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->G, f->gsx, f->gsy, int(steps_left),
-                               freedom_radius, seed);
-        }
+namespace {
+// One FluidModel::step (fluidmodel.cpp:165-580) over the particles in Q
+// (particle order, or the cell-ordered copy with its atom map).
+int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32_t steps_left,
+                  double freedom_radius, uint64_t seed, hipStream_t s) {
+    amh::ProfScope step_("fluid_step", s);
+    if (f->count == 0) return AMH_OK;
+    const Bins &B = f->bins;
+    const int tiles = B.ntx * B.nty;
+    const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
+    {
+        amh::ProfScope ps_("fluid_bins", s);
+        hipLaunchKernelGGL(k_bin_count, dim3(gp), dim3(kT), 0, s, Q, f->count, f->gsx, f->gsy, B);
+        hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count, tiles, B.start);
+        const int span = std::max(f->count, tiles);
+        hipLaunchKernelGGL(k_bin_place, dim3(uint32_t((span + kT - 1) / kT)), dim3(kT), 0, s, Q, f->count, f->gsx,
+                           f->gsy, B, tiles);
+    }
+    {
+        amh::ProfScope ps_("k_p2g", s);
+        hipLaunchKernelGGL(k_p2g, dim3(uint32_t(tiles)), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+    }
+    {
+        amh::ProfScope ps_("k_forces", s);
+        hipLaunchKernelGGL(k_forces, dim3(uint32_t(tiles)), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+    }
+    {
+        amh::ProfScope ps_("k_g2p", s);
+        hipLaunchKernelGGL(k_g2p, dim3(uint32_t(tiles)), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+    }
+    {
+        amh::ProfScope ps_("k_advect", s);
+        hipLaunchKernelGGL(k_advect, dim3(gp), dim3(kT), 0, s, Q, f->count, f->G, f->gsx, f->gsy, int(steps_left),
+                           freedom_radius, seed, atom);
     }
     AMH_LAUNCH_CHECK("amh_fluid_step");
     return AMH_OK;
+}
+
+// f->S = f->P in cell order (f->S.atom maps slots back to particles).
+int sort_particles(amh_fluid_s *f, hipStream_t s) {
+    amh::ProfScope ps_("fluid_sort", s);
+    const int64_t nn = int64_t(f->gsx) * f->gsy;
+    const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
+    hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((nn + 1 + kT - 1) / kT)), dim3(kT), 0, s, f->cnt, nn + 1);
+    hipLaunchKernelGGL(k_sort_count, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->gsx, f->gsy, f->cell_rank,
+                       f->cell, f->pos, f->cnt);
+    scan::exclusive(f->cnt, nn + 1, f->scan_sums, f->scan_offs, f->start, s);
+    hipLaunchKernelGGL(k_sort_gather, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->cell, f->pos, f->start, f->S);
+    AMH_LAUNCH_CHECK("fluid sort");
+    return AMH_OK;
+}
+}  // namespace
+
+// One FluidModel::step (fluidmodel.cpp:165-580) on the particles as they are
+// stored (particle order).
+extern "C" int amh_fluid_step(void *fluid, int32_t steps_left, double freedom_radius, uint64_t seed, void *stream) {
+    amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);
+    if (!f) { set_error("amh_fluid_step: NULL"); return AMH_E_ARG; }
+    return fluid_step_on(f, f->P, nullptr, steps_left, freedom_radius, seed, static_cast<hipStream_t>(stream));
 }
 
 // ----------------------------------------------------------- frame path
@@ -522,45 +1007,82 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, Frame F, Part
     P.active[a] = 1;
 }
 
-// draw_fluid, splat (morph.cpp:1080-1167)
-__global__ void k_fluid_splat(Particles P, int n, double t, int W, int H, double sp, int bx1, int by1, int bx2,
-                              int by2, double *acc) {
+// draw_fluid, splat (morph.cpp:1080-1167).  Bilinear taps into per-pixel
+// FP64 sums (weight, weight * channel); like the grid scatters, a block sums
+// its taps in an LDS copy of the pixels it touches when they fit.
+constexpr int kSplatPx = 1024;  // 5 FP64 sums: 40 KB of LDS
+
+__global__ __launch_bounds__(kT) void k_fluid_splat(Particles P, int n, double t, int W, int H, double sp, int bx1,
+                                                   int by1, int bx2, int by2, double *acc) {
+    __shared__ double tile[5][kSplatPx];
+    __shared__ int red[4 * (kT / 64)];
     const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n || !P.active[i]) return;
-    const double s = P.strength[i];
-    const double w = (1.0 / (100.0 * pow((t - (1.0 * (1.0 - s))), 2.0) + 1.0)) * pow(s, 2.0);
-    const double r = (w * P.R[i]) + (1.0 - w) * P.r[i], g = (w * P.G[i]) + (1.0 - w) * P.g[i];
-    const double b = (w * P.B[i]) + (1.0 - w) * P.b[i], al = (w * P.A[i]) + (1.0 - w) * P.a[i];
-    P.r[i] = r; P.g[i] = g; P.b[i] = b; P.a[i] = al;
-    double px, py;
-    const double x = P.x[i], y = P.y[i];
-    if (x <= 10.0) px = 0.0;
-    else if (x >= W * sp + 10.0) px = (W - 1);
-    else px = (x - 10.0) / sp;
-    if (y <= 10.0) py = 0.0;
-    else if (y >= H * sp + 10.0) py = (H - 1);
-    else py = (y - 10.0) / sp;
-    const int X = min(int(px), W - 1), Y = min(int(py), H - 1);
-    P.mature[i] = 1;
-    double ix, iy;
-    const double xf = modf(px, &ix), yf = modf(py, &iy);
-    if (ix >= W || iy >= H) {
-        if (ix > bx2 || ix < bx1 || iy > by2 || iy < by1) return;
+    bool valid = i < n && P.active[i];
+    int X = 0, Y = 0;
+    double wt[4] = {0.0, 0.0, 0.0, 0.0};  // taps (X,Y) (X+1,Y) (X,Y+1) (X+1,Y+1)
+    uint32_t col = 0;
+    if (valid) {
+        const double s = P.strength[i];
+        const double w = (1.0 / (100.0 * pow((t - (1.0 * (1.0 - s))), 2.0) + 1.0)) * pow(s, 2.0);
+        const double r = (w * P.R[i]) + (1.0 - w) * P.r[i], g = (w * P.G[i]) + (1.0 - w) * P.g[i];
+        const double b = (w * P.B[i]) + (1.0 - w) * P.b[i], al = (w * P.A[i]) + (1.0 - w) * P.a[i];
+        P.r[i] = r; P.g[i] = g; P.b[i] = b; P.a[i] = al;
+        double px, py;
+        const double x = P.x[i], y = P.y[i];
+        if (x <= 10.0) px = 0.0;
+        else if (x >= W * sp + 10.0) px = (W - 1);
+        else px = (x - 10.0) / sp;
+        if (y <= 10.0) py = 0.0;
+        else if (y >= H * sp + 10.0) py = (H - 1);
+        else py = (y - 10.0) / sp;
+        X = min(int(px), W - 1);
+        Y = min(int(py), H - 1);
+        P.mature[i] = 1;
+        double ix, iy;
+        const double xf = modf(px, &ix), yf = modf(py, &iy);
+        if ((ix >= W || iy >= H) && (ix > bx2 || ix < bx1 || iy > by2 || iy < by1)) valid = false;
+        const double w11 = ((1.0 - xf) * (1.0 - yf)), w21 = (xf * (1.0 - yf));
+        const double w12 = ((1.0 - xf) * yf), w22 = (xf * yf);
+        col = rgba_pack(narrow8(round(r * 255.0)), narrow8(round(g * 255.0)), narrow8(round(b * 255.0)),
+                        narrow8(round(al * 255.0)));
+        // a tap is drawn when its weight is positive, its guard holds and it
+        // lies inside the image
+        if (w11 > 0.0) wt[0] = w11;
+        if ((X < bx2 || X + 1 < W) && w21 > 0.0 && X + 1 < W) wt[1] = w21;
+        if ((Y < by2 || Y + 1 < H) && w12 > 0.0 && Y + 1 < H) wt[2] = w12;
+        if (w22 > 0.0 && ((Y < by2 && X < bx2) || (Y + 1 < H && X + 1 < W)) && X + 1 < W && Y + 1 < H) wt[3] = w22;
     }
-    const double w11 = ((1.0 - xf) * (1.0 - yf)), w21 = (xf * (1.0 - yf));
-    const double w12 = ((1.0 - xf) * yf), w22 = (xf * yf);
-    const uint32_t col = rgba_pack(narrow8(round(r * 255.0)), narrow8(round(g * 255.0)), narrow8(round(b * 255.0)),
-                                   narrow8(round(al * 255.0)));
-    auto add = [&](int qx, int qy, double wt) {
-        if (qx < 0 || qy < 0 || qx >= W || qy >= H) return;
-        double *q = acc + (int64_t(qy) * W + qx) * 5;
-        atomicAdd(&q[0], wt);
-        for (int k = 0; k < 4; ++k) atomicAdd(&q[1 + k], double(ch(col, k)) * wt);
-    };
-    if (w11 > 0.0) add(X, Y, w11);
-    if ((X < bx2 || X + 1 < W) && w21 > 0.0) add(X + 1, Y, w21);
-    if ((Y < by2 || Y + 1 < H) && w12 > 0.0) add(X, Y + 1, w12);
-    if (w22 > 0.0 && ((Y < by2 && X < bx2) || (Y + 1 < H && X + 1 < W))) add(X + 1, Y + 1, w22);
+    // X, Y are in [0, W) x [0, H) here: the box is in pixels, row-major
+    const Box B = block_box<kSplatPx, 2>(valid, X, Y, red);
+    if (!B.fits) {  // block-uniform
+        if (!valid) return;
+        for (int q = 0; q < 4; ++q) {
+            if (wt[q] <= 0.0) continue;
+            double *d = acc + (int64_t(Y + (q >> 1)) * W + X + (q & 1)) * 5;
+            atomicAdd(&d[0], wt[q]);
+            for (int k = 0; k < 4; ++k) atomicAdd(&d[1 + k], double(ch(col, k)) * wt[q]);
+        }
+        return;
+    }
+    const int area = B.w * B.h;  // B.w columns (x) by B.h rows (y)
+    for (int idx = threadIdx.x; idx < area; idx += kT)
+        for (int f = 0; f < 5; ++f) tile[f][idx] = 0.0;
+    __syncthreads();
+    if (valid) {
+        for (int q = 0; q < 4; ++q) {
+            if (wt[q] <= 0.0) continue;
+            const int idx = (Y + (q >> 1) - B.y0) * B.w + (X + (q & 1) - B.x0);
+            atomicAdd(&tile[0][idx], wt[q]);
+            for (int k = 0; k < 4; ++k) atomicAdd(&tile[1 + k][idx], double(ch(col, k)) * wt[q]);
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < area; idx += kT) {
+        if (tile[0][idx] == 0.0) continue;
+        const int qx = B.x0 + idx % B.w, qy = B.y0 + idx / B.w;
+        double *d = acc + (int64_t(qy) * W + qx) * 5;
+        for (int f = 0; f < 5; ++f) atomicAdd(&d[f], tile[f][idx]);
+    }
 }
 
 // resolve (morph.cpp:1169-1209); clears the accumulators
@@ -677,6 +1199,7 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
         return AMH_E_ARG;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
+    amh::ProfScope whole_("fluid_render", s);
     const Frame F = frame_of(t, R.nkeys);
     const int64_t npx = int64_t(f->W) * f->H;
     const uint32_t gpx = uint32_t((npx + kT - 1) / kT);
@@ -690,15 +1213,19 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
         // morph.cpp:1050-1054
         const double freedom = sqrt(double(f->W * f->W + f->H * f->H)) * fa->sparseness / 4.0;
         const double radius = freedom * (1.0 - F.tl);
+        // the steps and the splat run on the cell-ordered copy
+        if (int r = sort_particles(f, s)) return r;
         for (int k = 0; k < fa->fluidsteps; ++k) {
-            const int r = amh_fluid_step(f, fa->fluidsteps - (k + 1), radius, mix64(f->frame * 131 + k), stream);
+            const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), radius,
+                                        mix64(f->frame * 131 + k), s);
             if (r) return r;
         }
         {
             amh::ProfScope ps_("k_fluid_splat", s);
-            hipLaunchKernelGGL(k_fluid_splat, dim3(gp), dim3(kT), 0, s, f->P, f->count, F.tl, f->W, f->H, fa->sparseness,
-                               R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->acc);
+            hipLaunchKernelGGL(k_fluid_splat, dim3(gp), dim3(kT), 0, s, f->S, f->count, F.tl, f->W, f->H,
+                               fa->sparseness, R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->acc);
         }
+        hipLaunchKernelGGL(k_sort_scatter, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->P);
     }
     hipLaunchKernelGGL(k_fluid_resolve, dim3(gpx), dim3(kT), 0, s, f->acc, npx, F.tl, R.keep_background, f->img,
                        f->mask);

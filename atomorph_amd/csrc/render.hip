@@ -20,6 +20,7 @@
 #include "colour.h"
 #include "common.h"
 #include "../../include/atomorph_hip.h"
+#include "scan.h"
 #include "shade.h"
 
 using namespace amh;
@@ -28,7 +29,6 @@ using namespace amh::shade;
 namespace {
 
 constexpr uint32_t kNoPos = 0xFFFFFFFFu;
-constexpr int kScan = 1024;  // elements per scan block
 // --------------------------------------------------------------- prepare
 struct Rec {       // one atom's splat, 16 bytes
     uint32_t pos;  // y * W + x of the base tap, kNoPos if not drawn
@@ -169,70 +169,6 @@ __global__ __launch_bounds__(256) void k_overflow(const Rec *__restrict__ recs, 
             ov[ovstart[p] + slot] = o;
         }
     }
-}
-
-// ----------------------------------------------------------------- scan
-__global__ __launch_bounds__(256) void k_blk_sum(const int *__restrict__ in, int64_t n, int *__restrict__ sums) {
-    __shared__ int part[4];
-    const int64_t base = int64_t(blockIdx.x) * kScan;
-    int s = 0;
-    for (int l = threadIdx.x; l < kScan; l += 256) {
-        const int64_t i = base + l;
-        if (i < n) s += in[i];
-    }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) sums[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
-}
-__global__ __launch_bounds__(1024) void k_scan_sums(const int *__restrict__ in, int nb, int *__restrict__ out) {
-    __shared__ int buf[1024];
-    __shared__ int carry;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < nb; base += 1024) {
-        const int i = base + threadIdx.x;
-        const int v = i < nb ? in[i] : 0;
-        buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
-            __syncthreads();
-            buf[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < nb) out[i] = carry + buf[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += buf[1023];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[nb] = carry;
-}
-// exclusive scan inside each block of kScan, plus the block's offset;
-// out[n] = total.
-__global__ __launch_bounds__(256) void k_blk_apply(const int *__restrict__ in, int64_t n,
-                                                   const int *__restrict__ offs, int nb, int *__restrict__ out) {
-    __shared__ int wsum[4];
-    const int64_t base = int64_t(blockIdx.x) * kScan;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int running = offs[blockIdx.x];
-    for (int chunk = 0; chunk < kScan / 256; ++chunk) {
-        const int64_t i = base + chunk * 256 + threadIdx.x;
-        const int v = i < n ? in[i] : 0;
-        int incl = v;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int t = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += t;
-        }
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        int before = running;
-        for (int w = 0; w < wave; ++w) before += wsum[w];
-        if (i < n) out[i] = before + incl - v;
-        running += wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        __syncthreads();
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = offs[nb];
 }
 
 // ------------------------------------------------------------ composite
@@ -424,7 +360,7 @@ struct Layout {
 Layout layout(int32_t W, int32_t H, int64_t wtot) {
     Layout L;
     L.npx = int64_t(W) * H;
-    L.nblk = (L.npx + kScan - 1) / kScan;
+    L.nblk = scan::blocks(L.npx);
     auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
     size_t o = 0;
     L.bmin = o; o = al(o + sizeof(int) * L.npx);
@@ -484,6 +420,7 @@ extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t 
         return AMH_E_ARG;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
+    amh::ProfScope whole_("render_frame", s);
     char *base = static_cast<char *>(ws);
     int *bmin = reinterpret_cast<int *>(base + L.bmin);
     int *ovcount = reinterpret_cast<int *>(base + L.ovcount);
@@ -521,10 +458,7 @@ extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t 
             hipLaunchKernelGGL(k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc, ovcount);
         }
     }
-    hipLaunchKernelGGL(k_blk_sum, dim3(uint32_t(L.nblk)), dim3(256), 0, s, ovcount, L.npx, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, int(L.nblk), offs);
-    hipLaunchKernelGGL(k_blk_apply, dim3(uint32_t(L.nblk)), dim3(256), 0, s, ovcount, L.npx, offs, int(L.nblk),
-                       ovstart);
+    scan::exclusive(ovcount, L.npx, sums, offs, ovstart, s);
     if (Rc.wtot > 0)
         hipLaunchKernelGGL(k_overflow, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, ovcount, ovstart,
                            ov);

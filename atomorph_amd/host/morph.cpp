@@ -1541,6 +1541,19 @@ bool morph::put_chain_points(const std::vector<uint64_t> &pts) {
     return e.snapshot();
 }
 
+size_t morph::get_fluid_particles(std::vector<double> *out) {
+    engine &e = *E;
+    if (!e.fluid || e.wtot == 0) return 0;
+    if (out) {
+        out->resize(size_t(e.wtot) * 20);
+        if (amh_stream_sync(e.rstream) || amh_fluid_get_particles(e.fluid, out->data(), e.rstream)) {
+            out->clear();
+            return 0;
+        }
+    }
+    return size_t(e.wtot);
+}
+
 const char *morph::last_error() { return E->err.c_str(); }
 
 }  // namespace am

@@ -140,6 +140,10 @@ class morph {
                                      std::vector<size_t> *widths);
     // Replace the chains' refined points (same widths), e.g. a warm start.
     bool        put_chain_points    (const std::vector<uint64_t> &pts);
+    // Fluid particles of the last rendered fluid frame, 20 doubles each (x y
+    // u v gravity_x gravity_y freedom_r R G B A r g b a strength active
+    // mature - -, grid coordinates); returns the count (0: no fluid model).
+    size_t      get_fluid_particles (std::vector<double> *out);
     const char *last_error          ();
 
     struct engine;

@@ -3,6 +3,7 @@
  * tests and the benchmark, or any other FFI).  Each function forwards to the
  * am::morph member of the same name.
  */
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -155,6 +156,13 @@ size_t am_morph_chain_count(void *mp) {
 int am_morph_put_chains(void *mp, const uint64_t *pts, size_t n) {
     std::vector<uint64_t> p(pts, pts + n);
     return static_cast<am::morph *>(mp)->put_chain_points(p) ? 0 : -1;
+}
+size_t am_morph_fluid_particles(void *mp, double *out) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    std::vector<double> v;
+    const size_t n = m->get_fluid_particles(out ? &v : nullptr);
+    if (out && v.size() == n * 20) std::copy(v.begin(), v.end(), out);
+    return n;
 }
 const char *am_morph_error(void *m) { return static_cast<am::morph *>(m)->last_error(); }
 

@@ -46,6 +46,7 @@ _PROTOS = {
     "am_morph_energy": (C.c_double, [vp]),
     "am_morph_cost": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
     "am_morph_counters": (None, [vp, C.POINTER(u64), C.POINTER(u64)]),
+    "am_morph_fluid_particles": (C.c_size_t, [vp, vp]),
     "am_morph_get_time": (C.c_double, [vp, sz, sz]),
     "am_morph_render": (C.c_int, [vp, C.c_double, vp]),
     "am_morph_render_pixels": (C.c_int, [vp, C.c_double, vp, C.c_uint, C.c_uint]),
@@ -164,6 +165,15 @@ class Morph:
         a, b = u64(), u64()
         self.lib.am_morph_counters(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def fluid_particles(self):
+        """Particles of the last fluid frame, (count, 20) FP64 (see morph.h)."""
+        import numpy as np
+        n = self.lib.am_morph_fluid_particles(self.h, None)
+        out = np.zeros((n, 20))
+        if n:
+            self.lib.am_morph_fluid_particles(self.h, out.ctypes.data)
+        return out
 
     def get_time(self, i, n):
         return self.lib.am_morph_get_time(self.h, i, n)

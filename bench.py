@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FRAMES_OUT = 60
+PROFILED = ["k_blob_stats", "refine_graph", "k_refine_pass", "render_frame", "k_prepare", "k_splat", "k_composite",
+            "fluid_render", "k_fluid_update", "fluid_sort", "k_fluid_splat", "fluid_step", "fluid_bins", "k_p2g", "k_forces",
+            "k_node_div", "k_g2p", "k_advect"]
 
 
 def log(*a):
@@ -105,18 +108,19 @@ def time_to_cost(m, M, log_every=20):
     return out
 
 
-def cpu_baseline(frames, gpu_attempts, match_steps):
+def cpu_baseline(frames, gpu_attempts, match_steps, fluid_steps):
     """The compiled reference on this host, one core (threads = 0), on a
     bounded sample of the same job: full blob detection, matching and chain
     init; a timed refinement sample scaled to the GPU job's attempt count;
-    3 rendered frames scaled to 60."""
+    3 rendered frames scaled to 60.  With fluid on, the reference's fluid grid
+    is (W+20)x(H+20) FP64 (its sparseness is fixed at 1)."""
     from oracle import oracle_py as op
     if not op.have_ref():
         return None
     import numpy as np
     r = op.ref()
     m = r.ref_morph_new()
-    for k, v in {"seed": 1, "threads": 0, "cycle_length": 1 << 16}.items():
+    for k, v in {"seed": 1, "threads": 0, "cycle_length": 1 << 16, "fluid": fluid_steps}.items():
         r.ref_morph_set_option(m, k.encode(), float(v))
     for i, f in enumerate(frames):
         a = np.ascontiguousarray(f)
@@ -166,6 +170,9 @@ def main():
     ap.add_argument("--workload", default="blobs1024")
     ap.add_argument("--match-steps", type=int, default=20000)
     ap.add_argument("--graphs", type=int, default=8, help="refinement hipGraph replays per job (16 passes each)")
+    ap.add_argument("--fluid-steps", type=int, default=10, help="fluid model steps per output frame (0 = atom path)")
+    ap.add_argument("--fluid-sparseness", type=float, default=0.25,
+                    help="fluid grid scale: 0.25 gives a 256x256 grid (+ 20-node border) at 1024^2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-time-to-cost", action="store_true")
     args = ap.parse_args()
@@ -195,7 +202,7 @@ def main():
         ttc = time_to_cost(m, M)
         log(f"  {ttc}")
 
-    m.set(cycle_length=1 << 16)
+    m.set(cycle_length=1 << 16, fluid=args.fluid_steps, fluid_sparseness=args.fluid_sparseness)
     m.set_refine(M.REFINE_BATCHED, rounds=32, tile=4, graph_passes=16)
     phase = {"detect_ms": [], "refine_gpu_ms": [], "render_gpu_ms": [], "job_ms": []}
     attempts_job = accepted_job = 0
@@ -240,13 +247,37 @@ def main():
     pts, widths = m.chains()
     h = pts.shape[0]
     passes = 16 * args.graphs
+
+    # one more job with per-kernel device timing (HIP events on the engine's
+    # stream around each launch); not part of the timed steps
+    from atomorph_amd import _capi
+    lib = _capi.load()
+    lib.amh_profile_reset()
+    lib.amh_profile_enable(1)
+    job(False)
+    torch.cuda.synchronize()
+    kernels = {}
+    for name in PROFILED:
+        n, ms = C.c_uint64(0), C.c_double(0.0)
+        lib.amh_profile_read(name.encode(), C.byref(n), C.byref(ms))
+        if n.value:
+            kernels[name] = {"launches": int(n.value), "ms": round(ms.value, 4),
+                             "avg_us": round(1e3 * ms.value / n.value, 3)}
+    lib.amh_profile_enable(0)
+    lib.amh_profile_reset()
+    log("kernels (profiled job): " + json.dumps(kernels))
+
     # roofline of the refinement pass kernel: per attempt it reads the two
     # slots' points in rows y-1, y, y+1 (8 B each; rows y-1 = y+1 when h = 2)
-    # and writes 16 B per accepted swap.
+    # and writes 16 B per accepted swap.  The passes run inside hipGraph
+    # replays of 16 launches each.
     per_attempt = 8 * 2 * (2 + (1 if h > 2 else 0))
     refine_bytes = attempts_job * per_attempt + accepted_job * 16
-    refine_ms = float(np.mean(phase["refine_gpu_ms"]))
-    launch_s = refine_ms / 1e3 / passes
+    g = kernels.get("refine_graph")
+    if g:
+        launch_s = g["ms"] / 1e3 / (16 * g["launches"])
+    else:
+        launch_s = float(np.mean(phase["refine_gpu_ms"])) / 1e3 / passes
     achieved = refine_bytes / passes / launch_s / 1e9
     # rendering, for comparison: per atom and frame h points + 2 colours read,
     # per output pixel 4 B written
@@ -256,7 +287,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline: compiled reference, bounded sample")
-        cpu = cpu_baseline(frames, attempts_job, args.match_steps)
+        cpu = cpu_baseline(frames, attempts_job, args.match_steps, args.fluid_steps)
 
     res = {
         "metric": "time to equal OT cost, learned+batched vs plain GPU, 1024²; frames/s",
@@ -272,7 +303,11 @@ def main():
         "dtype": "int64",
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: 4 key frames 1024x1024 RGBA8, 150 fBm-thresholded blobs each, "
-                               f"atom trajectories (SPLINE, PERLIN fade), fluid off, {FRAMES_OUT} frames rendered",
+                               f"atom trajectories (SPLINE, PERLIN fade), "
+                               + (f"fluid on ({args.fluid_steps} steps/frame, "
+                                  f"{int(W * args.fluid_sparseness) + 20}x{int(H * args.fluid_sparseness) + 20} "
+                                  f"FP32 grid)" if args.fluid_steps else "fluid off")
+                               + f", {FRAMES_OUT} frames rendered",
                    "atoms": int(widths.sum()), "chains": int(len(widths)), "key_frames": h,
                    "match_steps": args.match_steps, "refine_passes": passes, "refine_attempts": attempts_job},
         "phases_ms": {k: round(float(np.mean(v)), 3) for k, v in phase.items()},
@@ -281,6 +316,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": int(refine_bytes / passes), "launch_us": round(launch_s * 1e6, 3)},
         "render_gbs": round(render_gbs, 1),
+        "kernels": kernels,
         "time_to_cost": ttc,
         "cpu_baseline": cpu,
     }

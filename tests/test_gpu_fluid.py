@@ -55,12 +55,24 @@ def test_fluid_step_matches_reference_fixture(golden_functions):
             assert d <= tol * (k + 1), (k, col, d)
 
 
-def test_fluid_step_matches_oracle_many_particles():
+@pytest.mark.parametrize("layout", ["uniform", "clustered", "cell_sorted"])
+def test_fluid_step_matches_oracle_many_particles(layout):
+    """uniform: every block's particles span the grid (direct global
+    scatter); clustered: runs of 256 particles around one centre, as chain
+    atoms are (the LDS-tiled scatter), with a few wide runs mixed in;
+    cell_sorted: the clustered particles ordered by cell, so runs of lanes
+    share a stencil and are summed across lanes before they scatter."""
     rng = np.random.default_rng(4)
-    n, gsx, gsy = 20000, 276, 276
+    n, gsx, gsy = 20480, 276, 276
     P = np.zeros((n, 20))
-    P[:, 0] = rng.uniform(8, 268, n)
-    P[:, 1] = rng.uniform(8, 268, n)
+    if layout == "uniform":
+        P[:, 0] = rng.uniform(8, 268, n)
+        P[:, 1] = rng.uniform(8, 268, n)
+    else:
+        centres = rng.uniform(20, 256, (n // 256, 2))
+        spread = np.where(np.arange(n // 256) % 10 == 0, 60.0, 3.0)
+        P[:, 0] = np.clip(np.repeat(centres[:, 0], 256) + rng.normal(0, 1, n) * np.repeat(spread, 256), 8, 268)
+        P[:, 1] = np.clip(np.repeat(centres[:, 1], 256) + rng.normal(0, 1, n) * np.repeat(spread, 256), 8, 268)
     P[:, 2:4] = rng.normal(0, 0.2, (n, 2))
     P[:, 4] = P[:, 0] + rng.normal(0, 3, n)
     P[:, 5] = P[:, 1] + rng.normal(0, 3, n)
@@ -69,7 +81,10 @@ def test_fluid_step_matches_oracle_many_particles():
     P[:, 15] = rng.choice([0.1, 1.0], n)
     P[:, 16] = (rng.uniform(0, 1, n) < 0.9).astype(float)
     P[:, 17] = rng.integers(0, 2, n)
-    G.log("fluid step 20k particles")
+    if layout == "cell_sorted":
+        cx, cy = (P[:, 0] - 0.5).astype(int), (P[:, 1] - 0.5).astype(int)
+        P = P[np.lexsort((cy, cx))]
+    G.log(f"fluid step 20k particles, {layout}")
     outs = _step_gpu(P, gsx, gsy, 3, 20.0)
     Q = P.copy()
     o = op.oracle()
