@@ -282,12 +282,13 @@ constexpr int kReach = kTile + 2;       // cells per axis that reach a tile's no
 constexpr int kHome = int(0x80000000);  // bin entry flag: the particle's home tile
 
 struct Bins {
-    int32_t ntx, nty;  // tiles per axis
-    int32_t *count;    // [tiles]; zero between steps
-    int32_t *start;    // [tiles + 1]
-    int32_t *rank;     // [4][n]: place of particle i's j-th entry in its bin
-    int32_t *list;     // [<= 4n] slot | kHome, grouped by bin
+    int32_t ntx, nty;   // tiles per axis
+    int32_t *count[2];  // [tiles] per step parity; the other parity is zero
+    int32_t *start;     // [tiles + 1]
+    int32_t *rank;      // [4][n]: place of particle i's j-th entry in its bin
+    int32_t *list;      // [<= 4n] slot | kHome, grouped by bin
 };
+constexpr int kLdsTiles = 4096;  // up to this many tiles the placing blocks scan the counts in LDS
 
 // The tiles holding the stencil nodes of a particle at (x, y): tx0..tx1 x
 // ty0..ty1 (nodes clamped to the grid); (tx0, ty0) is its home.
@@ -307,43 +308,87 @@ __device__ __forceinline__ int entry_tile(int j, int tx0, int tx1, int ty0, int 
     return (bx <= tx1 && by <= ty1) ? bx * nty + by : -1;
 }
 
-__global__ __launch_bounds__(kT) void k_bin_count(Particles P, int n, int gsx, int gsy, Bins B) {
+__global__ __launch_bounds__(kT) void k_bin_count(Particles P, int n, int gsx, int gsy, Bins B, int par) {
     const int i = blockIdx.x * kT + threadIdx.x;
     const bool act = i < n && P.active[i];
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
     if (act) stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
     const int lane = threadIdx.x & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    // Group the wave's lanes by bin, per entry: lane lead[j] leads the lanes
+    // holding bin t[j] and adds their number to the bin; all atomics of the
+    // wave are then in flight together.
+    int t[4], lead[4], pre[4], total[4];
     for (int j = 0; j < 4; ++j) {
-        const int t = act ? entry_tile(j, tx0, tx1, ty0, ty1, B.nty) : -1;
-        // one atomic per distinct bin of the wave
-        uint64_t todo = __ballot(t >= 0);
-        int rank = 0;
+        t[j] = act ? entry_tile(j, tx0, tx1, ty0, ty1, B.nty) : -1;
+        lead[j] = -1;
+        pre[j] = 0;
+        total[j] = 0;
+        uint64_t todo = __ballot(t[j] >= 0);
         while (todo) {
             const int leader = __ffsll((unsigned long long)todo) - 1;
-            const int tl = __shfl(t, leader, 64);
-            const uint64_t same = __ballot(t == tl);
-            int base = 0;
-            if (lane == leader) base = atomicAdd(&B.count[tl], int(__popcll(same)));
-            base = __shfl(base, leader, 64);
-            if (t == tl) rank = base + int(__popcll(same & ((1ull << lane) - 1ull)));
+            const int tl = __shfl(t[j], leader, 64);
+            const uint64_t same = __ballot(t[j] == tl);
+            if (t[j] == tl) {
+                lead[j] = leader;
+                pre[j] = int(__popcll(same & below));
+            }
+            if (lane == leader) total[j] = int(__popcll(same));
             todo &= ~same;
         }
-        if (i < n) B.rank[int64_t(j) * n + i] = rank;
+    }
+    int base[4];
+    for (int j = 0; j < 4; ++j) base[j] = total[j] > 0 ? atomicAdd(&B.count[par][t[j]], total[j]) : 0;
+    for (int j = 0; j < 4; ++j) {
+        const int b = __shfl(base[j], lead[j] < 0 ? lane : lead[j], 64);
+        if (i < n) B.rank[int64_t(j) * n + i] = lead[j] < 0 ? 0 : b + pre[j];
     }
 }
 
 // Particles into their bins (the tiles recomputed from the unchanged
-// positions); clears the counts for the next step.  The grid covers
+// positions) and the other parity's counts cleared for the next step.  With
+// kLds every block scans the counts into LDS (block 0 also stores them in
+// B.start); otherwise B.start holds the scan already.  The grid covers
 // max(n, tiles) threads.
-__global__ __launch_bounds__(kT) void k_bin_place(Particles P, int n, int gsx, int gsy, Bins B, int tiles) {
+template <bool kLds>
+__global__ __launch_bounds__(kT) void k_bin_place(Particles P, int n, int gsx, int gsy, Bins B, int tiles, int par) {
+    __shared__ int sstart[kLds ? kLdsTiles + 1 : 1];
+    __shared__ int wsum[kT / 64];
+    const int32_t *start = B.start;
+    if (kLds) {
+        const int per = (tiles + kT - 1) / kT;
+        const int lo = min(int(threadIdx.x) * per, tiles), hi = min(lo + per, tiles);
+        int sum = 0;
+        for (int k = lo; k < hi; ++k) {
+            const int c = B.count[par][k];
+            sstart[k] = sum;
+            sum += c;
+        }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        int incl = sum;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int off = incl - sum;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        for (int k = lo; k < hi; ++k) sstart[k] += off;
+        if (threadIdx.x == kT - 1) sstart[tiles] = off + sum;
+        __syncthreads();
+        if (blockIdx.x == 0)
+            for (int k = threadIdx.x; k <= tiles; k += kT) B.start[k] = sstart[k];
+        start = sstart;
+    }
+    for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) B.count[par ^ 1][k] = 0;
     const int i = blockIdx.x * kT + threadIdx.x;
-    if (i < tiles) B.count[i] = 0;
     if (i >= n || !P.active[i]) return;
     int tx0, tx1, ty0, ty1;
     stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
     for (int j = 0; j < 4; ++j) {
         const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
-        if (t >= 0) B.list[B.start[t] + B.rank[int64_t(j) * n + i]] = i | (j == 0 ? kHome : 0);
+        if (t >= 0) B.list[start[t] + B.rank[int64_t(j) * n + i]] = i | (j == 0 ? kHome : 0);
     }
 }
 
@@ -636,24 +681,28 @@ __global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, 
                                                   int32_t *__restrict__ pos, int32_t *__restrict__ cnt) {
     const int i = blockIdx.x * kT + threadIdx.x;
     const bool in = i < n;
-    const bool act = in && P.active[i];
     const int nn = gsx * gsy;
-    // the inactive bucket is one counter: a wave takes its slots at once
-    const uint64_t idle = __ballot(in && !act);
+    const int key = !in ? -1 : P.active[i] ? cell_rank[cell_of(P.x[i], P.y[i], gsx, gsy)] : nn;
+    // group the wave's lanes by key: one atomic per distinct key, all in flight together
     const int lane = threadIdx.x & 63;
-    const int leader = idle ? __ffsll((unsigned long long)idle) - 1 : 0;
-    int base = 0;
-    if (idle && lane == leader) base = atomicAdd(&cnt[nn], __popcll(idle));
-    base = __shfl(base, leader, 64);
-    if (!in) return;
-    if (act) {
-        const int c = cell_rank[cell_of(P.x[i], P.y[i], gsx, gsy)];
-        cell[i] = c;
-        pos[i] = atomicAdd(&cnt[c], 1);
-    } else {
-        cell[i] = nn;
-        pos[i] = base + __popcll(idle & ((1ull << lane) - 1ull));
+    int lead = lane, pre = 0, total = 0;
+    uint64_t todo = __ballot(in);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int kl = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(key == kl);
+        if (key == kl) {
+            lead = leader;
+            pre = int(__popcll(same & ((1ull << lane) - 1ull)));
+        }
+        if (lane == leader) total = int(__popcll(same));
+        todo &= ~same;
     }
+    const int base = total > 0 ? atomicAdd(&cnt[key], total) : 0;
+    const int b = __shfl(base, lead, 64);
+    if (!in) return;
+    cell[i] = key;
+    pos[i] = b + pre;
 }
 
 __global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, const int32_t *__restrict__ cell,
@@ -703,6 +752,7 @@ struct amh_fluid_s {
     uint32_t *img = nullptr;   // [npx] colours during feather
     int32_t W = 0, H = 0;
     uint64_t frame = 0;        // frames rendered (jitter seed)
+    int32_t bin_par = 0;       // parity of the next step's bin counts
 };
 
 namespace {
@@ -712,7 +762,7 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     const size_t tiles = size_t((gsx + kTile - 1) / kTile) * size_t((gsy + kTile - 1) / kTile);
     (void)nkeys;
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + 2) * 4 * 2 +
-           (2 * nb + 1) * 4 + nn * 4 + (2 * tiles + 1) * 4 + n * 4 * 8 + nn * 4 * kGridFields + npx * 8 * 5 +
+           (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + nn * 4 * kGridFields + npx * 8 * 5 +
            npx * 4 * 2 + 64 * 256;
 }
 
@@ -754,7 +804,8 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->bins.ntx = (gsx + kTile - 1) / kTile;
     f->bins.nty = (gsy + kTile - 1) / kTile;
     const size_t tiles = size_t(f->bins.ntx) * size_t(f->bins.nty);
-    f->bins.count = reinterpret_cast<int32_t *>(take(tiles * 4));
+    f->bins.count[0] = reinterpret_cast<int32_t *>(take(tiles * 4));
+    f->bins.count[1] = reinterpret_cast<int32_t *>(take(tiles * 4));
     f->bins.start = reinterpret_cast<int32_t *>(take((tiles + 1) * 4));
     f->bins.rank = reinterpret_cast<int32_t *>(take(n * 4 * 4));
     f->bins.list = reinterpret_cast<int32_t *>(take(n * 4 * 4));
@@ -865,11 +916,18 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
     {
         amh::ProfScope ps_("fluid_bins", s);
-        hipLaunchKernelGGL(k_bin_count, dim3(gp), dim3(kT), 0, s, Q, f->count, f->gsx, f->gsy, B);
-        hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count, tiles, B.start);
-        const int span = std::max(f->count, tiles);
-        hipLaunchKernelGGL(k_bin_place, dim3(uint32_t((span + kT - 1) / kT)), dim3(kT), 0, s, Q, f->count, f->gsx,
-                           f->gsy, B, tiles);
+        const int par = f->bin_par;
+        f->bin_par ^= 1;
+        hipLaunchKernelGGL(k_bin_count, dim3(gp), dim3(kT), 0, s, Q, f->count, f->gsx, f->gsy, B, par);
+        const uint32_t gb = uint32_t((std::max(f->count, tiles) + kT - 1) / kT);
+        if (tiles <= kLdsTiles) {
+            hipLaunchKernelGGL(k_bin_place<true>, dim3(gb), dim3(kT), 0, s, Q, f->count, f->gsx, f->gsy, B, tiles,
+                               par);
+        } else {
+            hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count[par], tiles, B.start);
+            hipLaunchKernelGGL(k_bin_place<false>, dim3(gb), dim3(kT), 0, s, Q, f->count, f->gsx, f->gsy, B, tiles,
+                               par);
+        }
     }
     {
         amh::ProfScope ps_("k_p2g", s);
