@@ -392,27 +392,49 @@ __global__ __launch_bounds__(kT) void k_bin_place(Particles P, int n, int gsx, i
     }
 }
 
-// Calls body(slot, home) for the entries of bin t, 256 at a time: slot is -1
-// for lanes past the end.  Trip counts are uniform over the block.
-template <class Body>
-__device__ __forceinline__ void for_bin(const Bins &B, int t, const Body &body) {
+// Calls body(slot, home, item) for the entries of bin t, kT at a time (slot
+// -1 past the end), item = fetch(slot) holding the particle fields the body
+// reads.  The next chunk's entries and fields are loaded before the current
+// chunk is processed, so a dense bin overlaps its memory latency.  Trip
+// counts are uniform over the block.
+template <class Fetch, class Body>
+__device__ __forceinline__ void for_bin(const Bins &B, int t, const Fetch &fetch, const Body &body) {
     const int s0 = B.start[t], s1 = B.start[t + 1];
-    for (int base = s0; base < s1; base += kT) {
+    auto entry = [&](int base, int &slot, bool &home) {
         const int k = base + int(threadIdx.x);
-        int slot = -1;
-        bool home = false;
+        slot = -1;
+        home = false;
         if (k < s1) {
             const int e = B.list[k];
             slot = e & 0x7FFFFFFF;
             home = e < 0;
         }
-        body(slot, home);
+    };
+    int slot, nslot = -1;
+    bool home, nhome = false;
+    entry(s0, slot, home);
+    auto item = fetch(slot);
+    for (int base = s0; base < s1; base += kT) {
+        auto next = item;
+        if (base + kT < s1) {
+            entry(base + kT, nslot, nhome);
+            next = fetch(nslot);
+        }
+        body(slot, home, item);
+        slot = nslot;
+        home = nhome;
+        item = next;
     }
 }
 
 __device__ __forceinline__ bool in_reach(int cx, int cy, int x0, int y0) {
     return cx >= x0 - 2 && cx < x0 + kTile && cy >= y0 - 2 && cy < y0 + kTile;
 }
+
+struct P2gItem {
+    double x, y, s, r, g, b, a;
+    int mature;
+};
 
 // Mass (= density), density gradient and strength-weighted colour of the
 // tile's nodes from the moments of the cells that reach them.
@@ -422,29 +444,36 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
     __syncthreads();
-    for_bin(B, int(blockIdx.x), [&](int slot, bool) {
+    auto fetch = [&](int slot) {
+        P2gItem it = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (slot >= 0) {
+            it.x = P.x[slot]; it.y = P.y[slot]; it.s = P.strength[slot];
+            it.r = P.r[slot]; it.g = P.g[slot]; it.b = P.b[slot]; it.a = P.a[slot];
+            it.mature = P.mature[slot];
+        }
+        return it;
+    };
+    for_bin(B, int(blockIdx.x), fetch, [&](int slot, bool, const P2gItem &it) {
         int cx = 0, cy = 0;
         bool valid = false;
         float c[kCellSums];
         for (int k = 0; k < kCellSums; ++k) c[k] = 0.0f;
         if (slot >= 0) {
-            const double x = P.x[slot], y = P.y[slot];
-            cx = static_cast<int>(x - 0.5);
-            cy = static_cast<int>(y - 0.5);
+            cx = static_cast<int>(it.x - 0.5);
+            cy = static_cast<int>(it.y - 0.5);
             valid = in_reach(cx, cy, x0, y0);
             if (valid) {
-                const double e = x - cx - 1.0, h = y - cy - 1.0;
+                const double e = it.x - cx - 1.0, h = it.y - cy - 1.0;
                 const double ep[3] = {1.0, e, e * e}, hp[3] = {1.0, h, h * h};
                 for (int p = 0; p < 3; ++p)
                     for (int q = 0; q < 3; ++q) c[p * 3 + q] = float(ep[p] * hp[q]);
-                const double s = P.strength[slot];
-                if (P.mature[slot] && s > 0.0) {
+                if (it.mature && it.s > 0.0) {
                     // strength-weighted running mean == sum(s c) / sum(s)
-                    c[9] = float(s * P.r[slot]);
-                    c[10] = float(s * P.g[slot]);
-                    c[11] = float(s * P.b[slot]);
-                    c[12] = float(s * P.a[slot]);
-                    c[13] = float(s);
+                    c[9] = float(it.s * it.r);
+                    c[10] = float(it.s * it.g);
+                    c[11] = float(it.s * it.b);
+                    c[12] = float(it.s * it.a);
+                    c[13] = float(it.s);
                 }
             }
         }
@@ -509,13 +538,18 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
     __syncthreads();
-    for_bin(B, int(blockIdx.x), [&](int slot, bool) {
+    auto fetch = [&](int slot) {
+        double2 it = {0.0, 0.0};
+        if (slot >= 0) it = double2{P.x[slot], P.y[slot]};
+        return it;
+    };
+    for_bin(B, int(blockIdx.x), fetch, [&](int slot, bool, const double2 &it) {
         int cx = 0, cy = 0;
         bool valid = false;
         double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0};
         double pressure = 0.0, fx = 0.0, fy = 0.0;
         if (slot >= 0) {
-            const double x = P.x[slot], y = P.y[slot];
+            const double x = it.x, y = it.y;
             weights(x, y, cx, cy, px, py, gx, gy);
             valid = in_reach(cx, cy, x0, y0);
             if (valid) particle_forces(x, y, Gd, gsx, gsy, pressure, fx, fy);
@@ -533,6 +567,11 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
     store_tile2(acc, Gd.ax, Gd.ay, x0, y0, gsx, gsy);
 }
 
+struct G2pItem {
+    double x, y, gx, gy, u, v;
+    int mature;
+};
+
 // Particle velocity from node accelerations plus the attractor (written by
 // the particle's home tile), then the momentum of the tile's nodes
 // (fluidmodel.cpp:359-437).
@@ -542,19 +581,27 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
     __syncthreads();
-    for_bin(B, int(blockIdx.x), [&](int slot, bool home) {
+    auto fetch = [&](int slot) {
+        G2pItem it = {0, 0, 0, 0, 0, 0, 0};
+        if (slot >= 0) {
+            it.x = P.x[slot]; it.y = P.y[slot]; it.gx = P.gx[slot]; it.gy = P.gy[slot];
+            it.u = P.u[slot]; it.v = P.v[slot]; it.mature = P.mature[slot];
+        }
+        return it;
+    };
+    for_bin(B, int(blockIdx.x), fetch, [&](int slot, bool home, const G2pItem &it) {
         int cx = 0, cy = 0;
         bool valid = false;
         double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3], gy[3];
         double mu = 0.0, mv = 0.0;
         if (slot >= 0) {
-            const double x = P.x[slot], y = P.y[slot];
+            const double x = it.x, y = it.y;
             weights(x, y, cx, cy, px, py, gx, gy);
             valid = in_reach(cx, cy, x0, y0);
             if (valid || home) {
                 double cax, cay;  // attractor, 0.03 per step (fluidmodel.cpp:383-410)
-                toward(x, y, P.gx[slot], P.gy[slot], 0.03, cax, cay);
-                double u = P.u[slot], v = P.v[slot];
+                toward(x, y, it.gx, it.gy, 0.03, cax, cay);
+                double u = it.u, v = it.v;
                 for (int a = 0; a < 3; ++a)
                     for (int b = 0; b < 3; ++b) {
                         const double phi = px[a] * py[b];
@@ -569,7 +616,7 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
                 }
                 mu = u;
                 mv = v;
-                if (!P.mature[slot]) {
+                if (!it.mature) {
                     mu *= 0.0;
                     mv *= 0.0;
                 }
