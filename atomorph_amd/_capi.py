@@ -100,7 +100,17 @@ PROTOTYPES = {
     "amh_fluid_get_particles": (C.c_int, [vp, vp, vp]),
     "amh_fluid_step": (C.c_int, [vp, i32, f64, u64, vp]),
     "amh_fluid_render": (C.c_int, [vp, C.POINTER(FluidFrame), f64, vp, vp]),
+    "amh_mlp_create": (C.c_int, [C.POINTER(vp), i32, vp]),
+    "amh_mlp_destroy": (C.c_int, [vp]),
+    "amh_mlp_set_params": (C.c_int, [vp, vp, vp]),
+    "amh_mlp_get_params": (C.c_int, [vp, vp, vp]),
+    "amh_mlp_forward": (C.c_int, [vp, vp, i64, vp, vp]),
+    "amh_mlp_gradients": (C.c_int, [vp, vp, vp, i32, vp, vp, vp]),
+    "amh_mlp_adam_step": (C.c_int, [vp, f32, vp]),
+    "amh_mlp_gemm_test": (C.c_int, [vp, vp, vp, i32, i32, i32, vp]),
 }
+
+MLP_PARAMS = 50946
 
 
 class AmhError(RuntimeError):

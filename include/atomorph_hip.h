@@ -280,6 +280,33 @@ typedef struct amh_fluid_frame {
  * splat, resolve, outward feather, background; out_dev[width*height]. */
 int amh_fluid_render(void *fluid, const amh_fluid_frame *frame, double t, uint32_t *out_dev, void *stream);
 
+/* ---- learned proposal: MLP on the matrix cores -------------------------- */
+/* No counterpart in the reference (its swap descent starts from row-major
+ * order, thread.cpp:740-891).  8 -> 128 -> 128 -> 128 -> 128 -> 2 with GELU;
+ * AMH_MLP_PARAMS FP32 parameters packed as W1[8][128] b1 W2[128][128] b2 W3
+ * b3 W4 b4 W5[128][2] b5 (W row-major [in][out]).  Products use bfloat16
+ * operands (weights and activations) with FP32 accumulation. */
+#define AMH_MLP_PARAMS 50946
+/* Handle for batches of up to batch_max rows (parameters zero). */
+int amh_mlp_create(void **mlp, int32_t batch_max, void *stream);
+int amh_mlp_destroy(void *mlp);
+/* Host parameters in and out (synchronous); set resets the Adam state. */
+int amh_mlp_set_params(void *mlp, const float *host_params, void *stream);
+int amh_mlp_get_params(void *mlp, float *host_params, void *stream);
+/* Inference: out_dev[n][2] for x_dev[n][8] (any n, run in batches). */
+int amh_mlp_forward(void *mlp, const float *x_dev, int64_t n, float *out_dev, void *stream);
+/* Gradient of mean_s sum_o (out - t)^2 over n <= batch_max rows into the
+ * handle (and grad_dev[AMH_MLP_PARAMS] when not NULL); adds the loss to
+ * *loss_dev when not NULL. */
+int amh_mlp_gradients(void *mlp, const float *x_dev, const float *t_dev, int32_t n, float *grad_dev,
+                      float *loss_dev, void *stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradients. */
+int amh_mlp_adam_step(void *mlp, float lr, void *stream);
+/* c_dev[M][N] = a_dev[M][K] . b_dev[K][N] through the MLP's matrix-core
+ * kernel (FP32 row-major in and out, bfloat16 operands). */
+int amh_mlp_gemm_test(const float *a_dev, const float *b_dev, float *c_dev, int32_t M, int32_t N, int32_t K,
+                      void *stream);
+
 #ifdef __cplusplus
 }
 #endif

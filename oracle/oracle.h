@@ -31,6 +31,8 @@
  *   or_fluid_step                     - vs compiled reference FluidModel
  *   or_lap_min_cost                   - vs scipy linear_sum_assignment and
  *                                       brute force permutations (n <= 8)
+ *   or_mlp_forward, or_mlp_grad       - central finite differences of the
+ *                                       loss; closed-form one-unit cases
  */
 #ifndef ATOMORPH_ORACLE_H
 #define ATOMORPH_ORACLE_H
@@ -165,6 +167,23 @@ void or_fluid_step(double *particles, int32_t count, int32_t gsx, int32_t gsy,
  * Hungarian / shortest augmenting path in int64.  assign[i] = column of row
  * i.  Returns the minimum cost. */
 int64_t or_lap_min_cost(const int64_t *cost, int32_t n, int32_t *assign);
+
+/* ---- learned-proposal MLP (no counterpart in the reference) ------------- */
+/* 8 -> 128 -> 128 -> 128 -> 128 -> 2, exact GELU after each hidden layer.
+ * params (OR_MLP_PARAMS floats): W1[8][128] b1[128] W2[128][128] b2[128]
+ * W3 b3 W4 b4 (as W2 b2) W5[128][2] b5[2], each W row-major [in][out].
+ * x: n x 8 inputs, out: n x 2.  bf16_operands != 0 rounds both operands of
+ * every matrix product to bfloat16 first, as the GPU kernels do; all
+ * arithmetic is FP64.  (mlp_oracle.cpp) */
+#define OR_MLP_PARAMS 50946
+void or_mlp_forward(const float *params, const float *x, int32_t n, int bf16_operands, double *out);
+/* Loss mean_s sum_o (out - t)^2 (returned) and its gradient with respect to
+ * every parameter (OR_MLP_PARAMS doubles, packed like params).  With
+ * bf16_operands the backward products round their operands too (dz and the
+ * activations for dW, dz and W for the data gradient); bias gradients are
+ * unrounded column sums. */
+double or_mlp_grad(const float *params, const float *x, const float *t, int32_t n, int bf16_operands,
+                   double *grad);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,11 @@ _ORACLE = {
     "or_distinct_color": (u32, [sz]),
     "or_fluid_step": (None, [vp, i32, i32, i32, i32, f64, C.c_uint]),
     "or_lap_min_cost": (i64, [vp, i32, vp]),
+    "or_mlp_forward": (None, [vp, vp, i32, C.c_int, vp]),
+    "or_mlp_grad": (f64, [vp, vp, vp, i32, C.c_int, vp]),
 }
+
+MLP_PARAMS = 50946
 
 _REF = {
     "ref_rgb_to_hsp": (u32, [u32]),

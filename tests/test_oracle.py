@@ -369,3 +369,76 @@ def test_get_time_closed_form():
     assert o.or_get_time(3, 16, 0, 2) == 3 / 16
     assert o.or_get_time(0, 1, 1, 4) == (1 - 1 / 4) / 2
     assert o.or_get_time(15, 16, 1, 2) == 1.0 * (1 - 1 / 2)
+
+
+# ------------------------------------------------------- learned-proposal MLP
+DIMS = [8, 128, 128, 128, 128, 2]
+
+
+def _mlp_params(rng, scale=0.3):
+    parts = []
+    for din, dout in zip(DIMS[:-1], DIMS[1:]):
+        parts.append(rng.normal(0, scale / np.sqrt(din), din * dout))
+        parts.append(rng.normal(0, 0.1, dout))
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == op.MLP_PARAMS
+    return p
+
+
+def _mlp_loss(p, x, t):
+    out = np.zeros((x.shape[0], 2))
+    o.or_mlp_forward(op.ptr(p), op.ptr(x), x.shape[0], 0, op.ptr(out))
+    return float(np.mean(np.sum((out - t) ** 2, axis=1)))
+
+
+def test_mlp_grad_matches_finite_differences():
+    """or_mlp_grad against central differences of the forward pass, for
+    weights and biases of every layer."""
+    rng = np.random.default_rng(5)
+    p = _mlp_params(rng)
+    x = rng.normal(0, 1, (6, 8)).astype(np.float32)
+    t = rng.normal(0, 1, (6, 2)).astype(np.float32)
+    g = np.zeros(op.MLP_PARAMS)
+    loss = o.or_mlp_grad(op.ptr(p), op.ptr(x), op.ptr(t), 6, 0, op.ptr(g))
+    assert abs(loss - _mlp_loss(p, x, t)) < 1e-12
+    offs, o_ = [], 0
+    for din, dout in zip(DIMS[:-1], DIMS[1:]):
+        offs.append((o_, din * dout, dout))
+        o_ += din * dout + dout
+    idx = []
+    for w0, nw, nb in offs:
+        idx += list(rng.integers(w0, w0 + nw, 8)) + list(rng.integers(w0 + nw, w0 + nw + nb, 3))
+    h = 1e-2
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        # float32 parameters: use the step actually taken
+        step = float(pp[i]) - float(pm[i])
+        fd = (_mlp_loss(pp, x, t) - _mlp_loss(pm, x, t)) / step
+        assert abs(fd - g[i]) <= 2e-4 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_mlp_closed_form_constant_net():
+    """With all weights zero the net outputs b5 and only biases of the last
+    layer and the last weights get gradient: dL/db5 = 2 mean(b5 - t),
+    dL/dW5[k] = gelu(b4[k]) dL/db5."""
+    from math import erf, sqrt
+    rng = np.random.default_rng(6)
+    p = np.zeros(op.MLP_PARAMS, np.float32)
+    b4 = rng.normal(0, 1, 128).astype(np.float32)
+    b5 = np.array([0.3, -0.7], np.float32)
+    p[-(128 * 2 + 2) - 128:-(128 * 2 + 2)] = b4
+    p[-2:] = b5
+    x = rng.normal(0, 1, (4, 8)).astype(np.float32)
+    t = rng.normal(0, 1, (4, 2)).astype(np.float32)
+    out = np.zeros((4, 2))
+    o.or_mlp_forward(op.ptr(p), op.ptr(x), 4, 0, op.ptr(out))
+    assert np.allclose(out, b5[None, :].astype(np.float64), atol=1e-7)
+    g = np.zeros(op.MLP_PARAMS)
+    o.or_mlp_grad(op.ptr(p), op.ptr(x), op.ptr(t), 4, 0, op.ptr(g))
+    db5 = 2 * np.mean(b5[None, :].astype(np.float64) - t, axis=0)
+    assert np.allclose(g[-2:], db5, atol=1e-12)
+    gel = np.array([0.5 * v * (1 + erf(v / sqrt(2))) for v in b4.astype(np.float64)])
+    assert np.allclose(g[-(128 * 2 + 2):-2].reshape(128, 2), np.outer(gel, db5), atol=1e-12)
+    assert np.abs(g[:-(128 * 2 + 2) - 128]).max() == 0.0  # W5 = 0 blocks everything below
