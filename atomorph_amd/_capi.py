@@ -108,6 +108,9 @@ PROTOTYPES = {
     "amh_mlp_gradients": (C.c_int, [vp, vp, vp, i32, vp, vp, vp]),
     "amh_mlp_adam_step": (C.c_int, [vp, f32, vp]),
     "amh_mlp_gemm_test": (C.c_int, [vp, vp, vp, i32, i32, i32, vp]),
+    "amh_warm_workspace_bytes": (sz, [i64, i32, i32]),
+    "amh_learn_features": (C.c_int, [C.POINTER(RenderArgs), i32, vp, vp, vp, sz, vp]),
+    "amh_warm_start": (C.c_int, [vp, C.POINTER(RenderArgs), vp, vp, sz, vp]),
 }
 
 MLP_PARAMS = 50946

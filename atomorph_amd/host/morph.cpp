@@ -135,6 +135,7 @@ struct Options {  // the setters, as the worker sees them (morph.h:97-119)
     int refine_tile = 2;
     size_t match_budget = 0;
     double fluid_sparseness = 1.0;  // grid cells per pixel (the reference's sparseness)
+    bool learned = false;           // learned warm start after the chains are built
 };
 
 struct HostFrame {  // caller-side pixels of one key frame, in insertion order
@@ -219,6 +220,13 @@ struct morph::engine {
     double fluid_sp = 0.0;
     bool fluid_inputs = false;
     DevBuf d_frank, d_fsurf, d_fbcol;
+    // learned warm start: MLP parameters (caller side), the handle and its
+    // workspace (worker side)
+    std::vector<float> mlp_params;
+    size_t mlp_version = 0, mlp_loaded = SIZE_MAX;
+    void *mlp = nullptr;
+    DevBuf d_wws, d_lx, d_lt;
+    bool warm_start();
     bool prepare_fluid();
     bool render_fluid(double t, uint32_t *out_dev);
     void fill_render(amh_render &r) const;
@@ -281,6 +289,7 @@ morph::engine::~engine() {
     if (th.joinable()) th.join();
     if (graph) amh_graph_destroy(graph);
     if (fluid) amh_fluid_destroy(fluid);
+    if (mlp) amh_mlp_destroy(mlp);
     for (auto &v : out_blobs)
         for (blob *b : v) delete b;
     // device buffers free themselves; streams last
@@ -805,6 +814,7 @@ bool morph::engine::step() {
             if (wskip || match_step()) {
                 if (!matched) match_init();
                 wstate = init_chains() ? STATE_ATOM_MORPHING : STATE_DONE;
+                if (wstate == STATE_ATOM_MORPHING && wopt.learned && !warm_start()) wstate = STATE_DONE;
                 counter = 0;
             }
             break;
@@ -835,6 +845,27 @@ bool morph::engine::exact_cost(uint64_t *lo, uint64_t *hi) {
     for (size_t i = 0; i < nch; ++i) s += (unsigned __int128)c[2 * i] | ((unsigned __int128)c[2 * i + 1] << 64);
     *lo = uint64_t(s);
     *hi = uint64_t(s >> 64);
+    return true;
+}
+
+// Learned warm start of the freshly built chains (in place on d_pts).
+bool morph::engine::warm_start() {
+    const size_t nch = cw.size();
+    if (mlp_params.size() != AMH_MLP_PARAMS) return fail("warm start: no MLP parameters (set_learned)");
+    if (!nch || wtot == 0 || h < 2) return true;
+    if (!mlp && amh_mlp_create(&mlp, 65536, cstream)) return fail("mlp create");
+    if (mlp_loaded != mlp_version) {
+        if (amh_mlp_set_params(mlp, mlp_params.data(), cstream)) return fail("mlp params");
+        mlp_loaded = mlp_version;
+    }
+    const size_t bytes = amh_warm_workspace_bytes(wtot, int32_t(nch), h);
+    if (!d_wws.ensure(bytes)) return fail("warm start workspace");
+    amh_render r;
+    std::memset(&r, 0, sizeof(r));
+    r.width = CW; r.height = CH; r.nkeys = h; r.nchains = int32_t(nch); r.wtot = wtot;
+    r.keys_dev = d_shown.as<uint32_t>(); r.pts_dev = d_pts.as<uint64_t>();
+    r.chain_off_dev = d_off.as<int64_t>(); r.chain_w_dev = d_w.as<int32_t>();
+    if (amh_warm_start(mlp, &r, d_pts.as<uint64_t>(), d_wws.p, bytes, cstream)) return fail("warm start");
     return true;
 }
 
@@ -1513,6 +1544,46 @@ void morph::set_refine(unsigned mode, unsigned rounds, int tile, unsigned graph_
     E->opt.graph_passes = graph_passes;
 }
 void morph::set_match_budget(size_t steps) { E->opt.match_budget = steps; }
+void morph::set_learned(const float *params) {
+    if (params) {
+        E->mlp_params.assign(params, params + AMH_MLP_PARAMS);
+        E->mlp_version += 1;
+    }
+    E->opt.learned = params != nullptr;
+}
+
+bool morph::warm_start(float *gpu_ms) {
+    engine &e = *E;
+    if (is_busy() || e.wstate != STATE_ATOM_MORPHING) { e.err = "warm_start: not paused in ATOM_MORPHING"; return false; }
+    void *timer = nullptr;
+    if (gpu_ms && amh_timer_begin(&timer, e.cstream)) return false;
+    bool ok = e.warm_start();
+    if (gpu_ms) ok = amh_timer_end(timer, e.cstream, gpu_ms) == 0 && ok;
+    amh_timer_free(timer);
+    return ok && amh_stream_sync(e.cstream) == 0 && e.snapshot();
+}
+
+size_t morph::learn_samples(std::vector<float> *x, std::vector<float> *t) {
+    engine &e = *E;
+    if (!e.have_snap || e.wtot == 0 || e.h < 2) return 0;
+    const size_t n = size_t(e.wtot) * size_t(e.h);
+    if (!x && !t) return n;
+    const size_t bytes = amh_warm_workspace_bytes(e.wtot, int32_t(e.cw.size()), e.h);
+    if (!e.d_wws.ensure(bytes) || !e.d_lx.ensure(size_t(e.wtot) * 32) || !e.d_lt.ensure(size_t(e.wtot) * 8)) return 0;
+    amh_render r;
+    e.fill_render(r);
+    if (x) x->resize(n * 8);
+    if (t) t->resize(n * 2);
+    for (int y = 0; y < e.h; ++y) {  // row y -> row y + 1 (cyclically)
+        if (amh_learn_features(&r, y, e.d_lx.as<float>(), e.d_lt.as<float>(), e.d_wws.p, bytes, e.rstream)) return 0;
+        const size_t o = size_t(y) * size_t(e.wtot);
+        if ((x && amh_memcpy_d2h(x->data() + o * 8, e.d_lx.p, size_t(e.wtot) * 32, e.rstream)) ||
+            (t && amh_memcpy_d2h(t->data() + o * 2, e.d_lt.p, size_t(e.wtot) * 8, e.rstream)) ||
+            amh_stream_sync(e.rstream))
+            return 0;
+    }
+    return n;
+}
 void morph::set_fluid_sparseness(double s) {
     if (s > 0.0) E->opt.fluid_sparseness = s;
 }

@@ -140,6 +140,17 @@ class morph {
                                      std::vector<size_t> *widths);
     // Replace the chains' refined points (same widths), e.g. a warm start.
     bool        put_chain_points    (const std::vector<uint64_t> &pts);
+    // Learned warm start (no counterpart in the reference): with params
+    // (AMH_MLP_PARAMS floats, see atomorph_hip.h) the chains are reordered by
+    // the MLP's sort-and-match right after they are built; NULL turns it off.
+    void        set_learned         (const float *params);
+    // Applies the warm start to the current chains now (paused in
+    // ATOM_MORPHING; synchronizes).  gpu_ms: device time, may be NULL.
+    bool        warm_start          (float *gpu_ms);
+    // Training samples from the synchronized chains: for every key frame y
+    // and atom, 8 features and the 2 normalised coordinates the atom holds
+    // in key frame y+1 (cyclically).  Returns the count (h * atoms).
+    size_t      learn_samples       (std::vector<float> *x, std::vector<float> *t);
     // Fluid particles of the last rendered fluid frame, 20 doubles each (x y
     // u v gravity_x gravity_y freedom_r R G B A r g b a strength active
     // mature - -, grid coordinates); returns the count (0: no fluid model).

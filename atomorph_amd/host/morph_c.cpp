@@ -157,6 +157,17 @@ int am_morph_put_chains(void *mp, const uint64_t *pts, size_t n) {
     std::vector<uint64_t> p(pts, pts + n);
     return static_cast<am::morph *>(mp)->put_chain_points(p) ? 0 : -1;
 }
+void am_morph_set_learned(void *m, const float *params) { static_cast<am::morph *>(m)->set_learned(params); }
+int am_morph_warm_start(void *m, float *gpu_ms) { return static_cast<am::morph *>(m)->warm_start(gpu_ms) ? 0 : -1; }
+size_t am_morph_learn_samples(void *mp, float *x, float *t) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    if (!x && !t) return m->learn_samples(nullptr, nullptr);
+    std::vector<float> vx, vt;
+    const size_t n = m->learn_samples(x ? &vx : nullptr, t ? &vt : nullptr);
+    if (x && vx.size() == n * 8) std::copy(vx.begin(), vx.end(), x);
+    if (t && vt.size() == n * 2) std::copy(vt.begin(), vt.end(), t);
+    return n;
+}
 size_t am_morph_fluid_particles(void *mp, double *out) {
     am::morph *m = static_cast<am::morph *>(mp);
     std::vector<double> v;

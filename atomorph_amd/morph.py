@@ -47,6 +47,9 @@ _PROTOS = {
     "am_morph_cost": (C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
     "am_morph_counters": (None, [vp, C.POINTER(u64), C.POINTER(u64)]),
     "am_morph_fluid_particles": (C.c_size_t, [vp, vp]),
+    "am_morph_set_learned": (None, [vp, vp]),
+    "am_morph_warm_start": (C.c_int, [vp, C.POINTER(C.c_float)]),
+    "am_morph_learn_samples": (C.c_size_t, [vp, vp, vp]),
     "am_morph_get_time": (C.c_double, [vp, sz, sz]),
     "am_morph_render": (C.c_int, [vp, C.c_double, vp]),
     "am_morph_render_pixels": (C.c_int, [vp, C.c_double, vp, C.c_uint, C.c_uint]),
@@ -165,6 +168,33 @@ class Morph:
         a, b = u64(), u64()
         self.lib.am_morph_counters(self.h, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def set_learned(self, params):
+        """Learned warm start on (params: AMH_MLP_PARAMS float32) or off (None)."""
+        if params is None:
+            self.lib.am_morph_set_learned(self.h, None)
+            self._mlp = None
+            return
+        self._mlp = np.ascontiguousarray(params, np.float32)
+        if self._mlp.size != _capi.MLP_PARAMS:
+            raise ValueError(f"expected {_capi.MLP_PARAMS} parameters, got {self._mlp.size}")
+        self.lib.am_morph_set_learned(self.h, self._mlp.ctypes.data_as(vp))
+
+    def warm_start(self):
+        """Apply the learned warm start to the current chains; device ms."""
+        ms = C.c_float()
+        if self.lib.am_morph_warm_start(self.h, C.byref(ms)) != 0:
+            raise _capi.AmhError(self.error())
+        return ms.value
+
+    def learn_samples(self):
+        """(features (n, 8), targets (n, 2)) float32 from the synchronized chains."""
+        n = self.lib.am_morph_learn_samples(self.h, None, None)
+        x = np.zeros((n, 8), np.float32)
+        t = np.zeros((n, 2), np.float32)
+        if n and self.lib.am_morph_learn_samples(self.h, x.ctypes.data_as(vp), t.ctypes.data_as(vp)) != n:
+            raise _capi.AmhError(self.error())
+        return x, t
 
     def fluid_particles(self):
         """Particles of the last fluid frame, (count, 20) FP64 (see morph.h)."""

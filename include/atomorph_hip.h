@@ -307,6 +307,24 @@ int amh_mlp_adam_step(void *mlp, float lr, void *stream);
 int amh_mlp_gemm_test(const float *a_dev, const float *b_dev, float *c_dev, int32_t M, int32_t N, int32_t K,
                       void *stream);
 
+/* ---- learned warm start: features and sort-and-match --------------------- */
+/* No counterpart in the reference.  Workspace for chains of wtot atoms in
+ * nchains chains over nkeys key frames. */
+size_t amh_warm_workspace_bytes(int64_t wtot, int32_t nchains, int32_t nkeys);
+/* MLP inputs of the atoms of key frame y (x_dev[wtot][8]: x/W, y/H, RGBA/255
+ * of the key-frame pixel, position relative to the chain row's centroid in
+ * units of its RMS radius + 1 px) and, if t_dev != NULL, training targets
+ * (t_dev[wtot][2]: the same slots' positions in key frame y+1, normalised by
+ * that row's centroid and radius).  Uses r's keys_dev, pts_dev, chain
+ * offsets and widths. */
+int amh_learn_features(const amh_render *r, int32_t y, float *x_dev, float *t_dev, void *ws_dev, size_t ws_bytes,
+                       void *stream);
+/* Reorders rows 1 .. nkeys-1 of pts_dev (== r->pts_dev) in place: row y+1 of
+ * every chain is matched to the MLP's predictions from row y by rank in
+ * raster order (LSD radix sort of (chain, y, x) keys at 1/256 px), so each
+ * row stays a permutation of its points.  nchains <= 65535. */
+int amh_warm_start(void *mlp, const amh_render *r, uint64_t *pts_dev, void *ws_dev, size_t ws_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
