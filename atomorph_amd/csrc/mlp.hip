@@ -204,6 +204,116 @@ __global__ void k_adam(float *__restrict__ p, uint16_t *__restrict__ pb, const f
     pb[i] = to_bf16(pi);
 }
 
+// ------------------------------------------------------- fused forward
+// Inference in one kernel with the activations kept in registers: each wave
+// computes 32 rows at a time as Z^T = W^T A^T, so the accumulator of one
+// layer (lane = row l % 32, registers = output features) is already the
+// next layer's B operand once its K order is permuted to match:
+//   MFMA k = 16 ks + 8 (l / 32) + j  <->  feature feat(ks, l / 32, j)
+// and the weights are staged into LDS in that order (A operands, 16 bytes
+// per lane).  Layer 1 pads K 8 -> 16 with zeros, the output layer pads its
+// 2 rows to 32.  LDS: 4 + 3 x 32 + 8 KB of weights, 2 KB of biases.
+__host__ __device__ constexpr int feat(int ks, int h, int j) {
+    return 32 * (ks / 2) + 16 * (ks % 2) + 8 * (j / 4) + 4 * h + (j % 4);
+}
+struct FusedLds {
+    bf16x8 w1[4][64];
+    bf16x8 wh[3][4][8][64];
+    bf16x8 w5[8][64];
+    float b[4][kH];
+    float b5[kOut];
+};
+constexpr size_t kFusedLds = sizeof(FusedLds);
+static_assert(kFusedLds % 16 == 0, "LDS image copied in 16-byte units");
+
+__device__ __forceinline__ bf16x8 pack8(const uint16_t *v) {
+    bf16x8 r;
+    __builtin_memcpy(&r, v, 16);
+    return r;
+}
+
+// The LDS image of the weights in MFMA operand order (built once per
+// parameter change into global memory; the fused kernel copies it).
+__global__ __launch_bounds__(256) void k_fused_stage(const uint16_t *__restrict__ pb, const float *__restrict__ p,
+                                                    FusedLds *__restrict__ S) {
+    const int t = blockIdx.x * 256 + threadIdx.x, nt = gridDim.x * 256;
+    for (int e = t; e < 4 * 64; e += nt) {
+        const int mb = e / 64, l = e % 64, lh = l >> 5, lm = l & 31;
+        uint16_t v[8];
+        for (int j = 0; j < 8; ++j) v[j] = lh == 0 ? pb[w_off(0) + j * kH + mb * 32 + lm] : uint16_t(0);
+        S->w1[mb][l] = pack8(v);
+    }
+    for (int e = t; e < 3 * 4 * 8 * 64; e += nt) {
+        const int li = e / (4 * 8 * 64), mb = (e / (8 * 64)) % 4, ks = (e / 64) % 8, l = e % 64;
+        const int lh = l >> 5, lm = l & 31;
+        uint16_t v[8];
+        for (int j = 0; j < 8; ++j) v[j] = pb[w_off(li + 1) + size_t(feat(ks, lh, j)) * kH + mb * 32 + lm];
+        S->wh[li][mb][ks][l] = pack8(v);
+    }
+    for (int e = t; e < 8 * 64; e += nt) {
+        const int ks = e / 64, l = e % 64, lh = l >> 5, lm = l & 31;
+        uint16_t v[8];
+        for (int j = 0; j < 8; ++j) v[j] = lm < kOut ? pb[w_off(4) + size_t(feat(ks, lh, j)) * kOut + lm] : uint16_t(0);
+        S->w5[ks][l] = pack8(v);
+    }
+    for (int e = t; e < 4 * kH; e += nt) S->b[e / kH][e % kH] = p[b_off(e / kH) + e % kH];
+    if (t < kOut) S->b5[t] = p[b_off(4) + t];
+}
+
+constexpr int kFusedThreads = 1024;  // 16 waves share one copy of the weights in LDS
+
+__global__ __launch_bounds__(kFusedThreads) void k_mlp_fused(const float *__restrict__ x, int64_t n,
+                                                  const FusedLds *__restrict__ image, float *__restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
+    FusedLds &S = *reinterpret_cast<FusedLds *>(lds_raw);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, h = lane >> 5, m = lane & 31;
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(image);
+        uint4 *dst = reinterpret_cast<uint4 *>(lds_raw);
+        for (size_t e = t; e < kFusedLds / 16; e += kFusedThreads) dst[e] = src[e];
+    }
+    __syncthreads();
+
+    const int64_t tiles = (n + 31) / 32;
+    constexpr int kWaves = kFusedThreads / 64;
+    for (int64_t tile = int64_t(blockIdx.x) * kWaves + wave; tile < tiles; tile += int64_t(gridDim.x) * kWaves) {
+        const int64_t row = tile * 32 + m;
+        uint16_t xin[8];
+        for (int j = 0; j < 8; ++j) xin[j] = (h == 0 && row < n) ? to_bf16(x[row * kIn + j]) : uint16_t(0);
+        floatx16 acc[4];
+        for (int mb = 0; mb < 4; ++mb) {
+            for (int r = 0; r < 16; ++r) acc[mb][r] = 0.0f;
+            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.w1[mb][lane], pack8(xin), acc[mb], 0, 0, 0);
+        }
+        bf16x8 B[8];
+        for (int li = 0; li < 4; ++li) {
+            // bias + GELU of layer li's outputs -> B operands of the next layer
+            for (int mb = 0; mb < 4; ++mb) {
+                uint16_t v[16];
+                for (int r = 0; r < 16; ++r) {
+                    const int f = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                    v[r] = to_bf16(gelu(acc[mb][r] + S.b[li][f]));
+                }
+                B[2 * mb] = pack8(v);
+                B[2 * mb + 1] = pack8(v + 8);
+            }
+            if (li == 3) break;
+            for (int mb = 0; mb < 4; ++mb) {
+                for (int r = 0; r < 16; ++r) acc[mb][r] = 0.0f;
+                for (int ks = 0; ks < 8; ++ks)
+                    acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.wh[li][mb][ks][lane], B[ks], acc[mb], 0, 0, 0);
+            }
+        }
+        floatx16 o;
+        for (int r = 0; r < 16; ++r) o[r] = 0.0f;
+        for (int ks = 0; ks < 8; ++ks) o = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.w5[ks][lane], B[ks], o, 0, 0, 0);
+        if (h == 0 && row < n) {  // rows 0 and 1 of the output block: registers 0 and 1
+            out[row * kOut + 0] = o[0] + S.b5[0];
+            out[row * kOut + 1] = o[1] + S.b5[1];
+        }
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- handle
@@ -212,6 +322,8 @@ struct amh_mlp_s {
     void *buf = nullptr;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // [kParams]
     uint16_t *pb = nullptr;                                         // bfloat16 copy
+    FusedLds *image = nullptr;  // weights in the fused forward's LDS order
+    bool image_stale = true;
     float *z[kLayers] = {};    // [batch][dims[l+1]] pre-activations
     float *act[kLayers] = {};  // [batch][dims[l]] inputs of layer l (act[0] unused: the caller's x)
     float *dz[2] = {};         // [batch][128] ping-pong gradients
@@ -220,9 +332,29 @@ struct amh_mlp_s {
 };
 
 namespace {
+int g_cus = 0;
+int g_fused = -1;  // -1 unknown, 0 no, 1 yes
+// The fused forward needs kFusedLds bytes of LDS in one workgroup.
+bool fused_ok() {
+    if (g_fused >= 0) return g_fused == 1;
+    int dev = 0, lds = 0, cus = 0;
+    g_fused = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return false;
+    if (size_t(lds) < kFusedLds) return false;
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_mlp_fused), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            int(kFusedLds)) != hipSuccess)
+        return false;
+    g_cus = cus;
+    g_fused = 1;
+    return true;
+}
+
 size_t mlp_bytes(int32_t batch) {
     const size_t B = size_t(batch);
-    return 4 * kParams * 4 + kParams * 2 + B * kH * 4 * (kLayers + kLayers + 2) + 64 * 256;
+    return 4 * kParams * 4 + kParams * 2 + kFusedLds + B * kH * 4 * (kLayers + kLayers + 2) + 64 * 256;
 }
 
 // forward of rows [0, n) of x (n <= batch) into M->z[4] (n x 2)
@@ -257,6 +389,7 @@ extern "C" int amh_mlp_create(void **mlp, int32_t batch_max, void *stream) {
     M->m = reinterpret_cast<float *>(take(kParams * 4));
     M->v = reinterpret_cast<float *>(take(kParams * 4));
     M->pb = reinterpret_cast<uint16_t *>(take(kParams * 2));
+    M->image = reinterpret_cast<FusedLds *>(take(kFusedLds));
     const size_t B = size_t(batch_max);
     for (int l = 0; l < kLayers; ++l) M->z[l] = reinterpret_cast<float *>(take(B * kH * 4));
     for (int l = 0; l < kLayers; ++l) M->act[l] = reinterpret_cast<float *>(take(B * kH * 4));
@@ -289,6 +422,7 @@ extern "C" int amh_mlp_set_params(void *mlp, const float *host, void *stream) {
     hipLaunchKernelGGL(k_fill, dim3((kParams + 255) / 256), dim3(256), 0, s, M->m, int64_t(kParams), 0.0f);
     hipLaunchKernelGGL(k_fill, dim3((kParams + 255) / 256), dim3(256), 0, s, M->v, int64_t(kParams), 0.0f);
     M->step = 0;
+    M->image_stale = true;
     AMH_LAUNCH_CHECK("amh_mlp_set_params");
     return check_hip(hipStreamSynchronize(s), "set_params");
 }
@@ -307,6 +441,18 @@ extern "C" int amh_mlp_forward(void *mlp, const float *x, int64_t n, float *out,
     if (!M || (n > 0 && (!x || !out)) || n < 0) { set_error("amh_mlp_forward: invalid argument"); return AMH_E_ARG; }
     hipStream_t s = static_cast<hipStream_t>(stream);
     amh::ProfScope ps_("mlp_forward", s);
+    if (n == 0) return AMH_OK;
+    if (fused_ok()) {
+        const int64_t tiles = (n + 31) / 32;
+        const uint32_t grid = uint32_t(std::min<int64_t>(g_cus, (tiles + kFusedThreads / 64 - 1) / (kFusedThreads / 64)));
+        if (M->image_stale) {
+            hipLaunchKernelGGL(k_fused_stage, dim3(64), dim3(256), 0, s, M->pb, M->p, M->image);
+            M->image_stale = false;
+        }
+        hipLaunchKernelGGL(k_mlp_fused, dim3(grid), dim3(kFusedThreads), kFusedLds, s, x, n, M->image, out);
+        AMH_LAUNCH_CHECK("amh_mlp_forward");
+        return AMH_OK;
+    }
     for (int64_t i = 0; i < n; i += M->batch) {
         const int c = int(std::min<int64_t>(M->batch, n - i));
         forward_batch(M, x + i * kIn, c, s);
@@ -364,6 +510,7 @@ extern "C" int amh_mlp_adam_step(void *mlp, float lr, void *stream) {
     if (!M || !(lr > 0.0f)) { set_error("amh_mlp_adam_step: invalid argument"); return AMH_E_ARG; }
     hipStream_t s = static_cast<hipStream_t>(stream);
     M->step += 1;
+    M->image_stale = true;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, float(M->step)), c2 = 1.0f - std::pow(b2, float(M->step));
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, s, M->p, M->pb, M->g, M->m, M->v, kParams,

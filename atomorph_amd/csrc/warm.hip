@@ -4,7 +4,7 @@
 // blobs were built in (thread.cpp:740-891) and leaves the whole assignment
 // to the pair-swap descent.  Here, before the descent, row y+1 of each chain
 // is reordered so that slot i receives the point of key frame y+1 whose
-// rank in raster order (y, then x, at 1/256 px) equals the rank of the
+// rank in raster order (y, then x, at 1/16 px) equals the rank of the
 // MLP's prediction for slot i's row-y atom:
 //   features  per atom of row y: x/W, y/H, RGBA/255 of its key-frame pixel,
 //             (x - cx)/s, (y - cy)/s with (cx, cy) the chain row's centroid
@@ -100,13 +100,13 @@ __global__ __launch_bounds__(kT) void k_features(amh_render R, int y, const RowS
     }
 }
 
-// Raster key of a position in 16.8 fixed point: y (24 bits) then x (24
-// bits), below the chain index (16 bits).
+// Raster key of a position at 1/16 px: y (20 bits) then x (20 bits), below
+// the chain index.
 __device__ __forceinline__ uint64_t raster_key(uint64_t chain, double x, double y) {
-    const double lim = double((1 << 24) - 1);
-    const uint64_t fx = uint64_t(fmin(fmax(rint(x * 256.0), 0.0), lim));
-    const uint64_t fy = uint64_t(fmin(fmax(rint(y * 256.0), 0.0), lim));
-    return (chain << 48) | (fy << 24) | fx;
+    const double lim = double((1 << 20) - 1);
+    const uint64_t fx = uint64_t(fmin(fmax(rint(x * 16.0), 0.0), lim));
+    const uint64_t fy = uint64_t(fmin(fmax(rint(y * 16.0), 0.0), lim));
+    return (chain << 40) | (fy << 20) | fx;
 }
 
 // Sort keys of the predictions for row y+1 and of the points row y+1 holds
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kT) void k_match_keys(amh_render R, int yn, const R
     const double ux = s.cx + s.s * double(pred[a * 2 + 0]), uy = s.cy + s.s * double(pred[a * 2 + 1]);
     const uint64_t q = R.pts_dev[int64_t(yn) * R.wtot + a];
     kp[a] = raster_key(uint64_t(c), ux, uy);
-    kq[a] = (uint64_t(c) << 48) | (uint64_t(pt_fy(q)) << 24) | uint64_t(pt_fx(q));
+    kq[a] = (uint64_t(c) << 40) | (uint64_t(pt_fy(q) >> 4) << 20) | uint64_t(pt_fx(q) >> 4);
     ip[a] = uint32_t(a);
     iq[a] = uint32_t(a);
 }
@@ -302,7 +302,7 @@ extern "C" int amh_warm_start(void *mlp, const amh_render *R, uint64_t *pts_dev,
     hipLaunchKernelGGL(k_row_stats, dim3(uint32_t(R->nchains), uint32_t(R->nkeys)), dim3(kT), 0, s, *R, st);
     int chain_bits = 0;
     while ((1 << chain_bits) < R->nchains) ++chain_bits;
-    const int bits = ((48 + chain_bits + 15) / 16) * 16;  // whole byte passes, an even count
+    const int bits = ((40 + chain_bits + 15) / 16) * 16;  // whole byte passes, an even count
     for (int y = 0; y + 1 < R->nkeys; ++y) {
         hipLaunchKernelGGL(k_features, dim3(ga), dim3(kT), 0, s, *R, y, st, x, static_cast<float *>(nullptr));
         if (int r = amh_mlp_forward(mlp, x, R->wtot, pred, stream)) return r;

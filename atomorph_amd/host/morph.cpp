@@ -226,6 +226,7 @@ struct morph::engine {
     size_t mlp_version = 0, mlp_loaded = SIZE_MAX;
     void *mlp = nullptr;
     DevBuf d_wws, d_lx, d_lt;
+    bool prepare_mlp();
     bool warm_start();
     bool prepare_fluid();
     bool render_fluid(double t, uint32_t *out_dev);
@@ -848,18 +849,26 @@ bool morph::engine::exact_cost(uint64_t *lo, uint64_t *hi) {
     return true;
 }
 
-// Learned warm start of the freshly built chains (in place on d_pts).
-bool morph::engine::warm_start() {
-    const size_t nch = cw.size();
+// The MLP handle with the caller's current parameters, and the warm start
+// workspace for the current chains (allocations and uploads, done before
+// any timed work).
+bool morph::engine::prepare_mlp() {
     if (mlp_params.size() != AMH_MLP_PARAMS) return fail("warm start: no MLP parameters (set_learned)");
-    if (!nch || wtot == 0 || h < 2) return true;
     if (!mlp && amh_mlp_create(&mlp, 65536, cstream)) return fail("mlp create");
     if (mlp_loaded != mlp_version) {
         if (amh_mlp_set_params(mlp, mlp_params.data(), cstream)) return fail("mlp params");
         mlp_loaded = mlp_version;
     }
+    if (!d_wws.ensure(amh_warm_workspace_bytes(wtot, int32_t(cw.size()), h))) return fail("warm start workspace");
+    return true;
+}
+
+// Learned warm start of the freshly built chains (in place on d_pts).
+bool morph::engine::warm_start() {
+    const size_t nch = cw.size();
+    if (!nch || wtot == 0 || h < 2) return true;
+    if (!prepare_mlp()) return false;
     const size_t bytes = amh_warm_workspace_bytes(wtot, int32_t(nch), h);
-    if (!d_wws.ensure(bytes)) return fail("warm start workspace");
     amh_render r;
     std::memset(&r, 0, sizeof(r));
     r.width = CW; r.height = CH; r.nkeys = h; r.nchains = int32_t(nch); r.wtot = wtot;
@@ -1555,6 +1564,7 @@ void morph::set_learned(const float *params) {
 bool morph::warm_start(float *gpu_ms) {
     engine &e = *E;
     if (is_busy() || e.wstate != STATE_ATOM_MORPHING) { e.err = "warm_start: not paused in ATOM_MORPHING"; return false; }
+    if (!e.prepare_mlp() || amh_stream_sync(e.cstream)) return false;
     void *timer = nullptr;
     if (gpu_ms && amh_timer_begin(&timer, e.cstream)) return false;
     bool ok = e.warm_start();

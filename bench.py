@@ -3,14 +3,19 @@
 
 One step = one full morph job on the 1024^2 workload (4 RGBA8 key frames of
 150 noise-thresholded blobs each): blob detection, blob matching, chain build,
-batched optimal-transport refinement (hipGraph replays) and 60 output frames
-rendered into HBM.  The key frames are uploaded before the timed region.
+learned warm start (MLP + sort-and-match), batched optimal-transport
+refinement (hipGraph replays) and 60 output frames rendered into HBM with the
+fluid model on (10 steps per frame on a 256^2 + 20 FP32 grid).  The key
+frames are uploaded before the timed region.
 
 Printed (one JSON line): frames/s of the whole job, plus
   time_to_cost  plain GPU descent (one chain at a time) run to convergence vs
-                the batched persistent descent (and the learned warm start when
-                enabled) timed until they reach the plain descent's final cost;
-  roofline      the dominant kernel's achieved bytes/s against 8 TB/s;
+                the batched persistent descent, and the learned warm start +
+                batched descent, timed until they reach the plain descent's
+                final cost;
+  roofline      the dominant kernel's achieved bytes/s against 8 TB/s, from
+                per-kernel device times of one extra profiled job;
+  mlp           the warm start's matrix-core forward against the BF16 peak;
   cpu_baseline  the compiled reference (oracle/_ref) on this host, one core,
                 on a bounded sample of the same job, scaled to frames/s.
 
@@ -28,9 +33,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FRAMES_OUT = 60
-PROFILED = ["k_blob_stats", "refine_graph", "k_refine_pass", "render_frame", "k_prepare", "k_splat", "k_composite",
-            "fluid_render", "k_fluid_update", "fluid_sort", "k_fluid_splat", "fluid_step", "fluid_bins", "k_p2g", "k_forces",
-            "k_node_div", "k_g2p", "k_advect"]
+PROFILED = ["k_blob_stats", "warm_start", "mlp_forward", "refine_graph", "k_refine_pass", "render_frame", "k_prepare",
+            "k_splat", "k_composite", "fluid_render", "k_fluid_update", "fluid_sort", "k_fluid_splat", "fluid_step",
+            "fluid_bins", "k_p2g", "k_forces", "k_g2p", "k_advect"]
+MLP_FLOP_PER_ROW = 2 * (8 * 128 + 3 * 128 * 128 + 128 * 2)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(*a):
@@ -56,9 +63,10 @@ def to_morphing(m, M, match_steps):
         raise RuntimeError(f"engine did not reach ATOM_MORPHING ({st}): {m.error()}")
 
 
-def time_to_cost(m, M, log_every=20):
-    """Plain descent to convergence, then the batched descent from the same
-    initial chains until it reaches the plain final cost.  Device time of the
+def time_to_cost(m, M, learned=None, log_every=20):
+    """Plain descent to convergence, then the batched descent, and the learned
+    warm start followed by the batched descent, from the same initial chains
+    until they reach the plain final cost.  Device time of the warm start and
     refinement steps only (HIP events on the engine's compute stream)."""
     pts0, widths = m.chains()
     nch = len(widths)
@@ -81,13 +89,21 @@ def time_to_cost(m, M, log_every=20):
     gain = c0 - c_plain
     out = {"c0": c0, "plain_cost": c_plain, "plain_ms": round(t_plain, 3), "plain_sweeps": len(hist) - 1}
 
-    def race(label, setup):
+    def race(setup, params=None):
         m.put_chains(pts0)
         setup()
-        t, reach_lit, reach_gain = 0.0, None, None
-        if c0 <= 1.005 * c_plain:
-            reach_lit = 0.0
+        t, warm_ms = 0.0, 0.0
+        if params is not None:  # learned warm start, timed with the descent
+            m.set_learned(params)
+            warm_ms = m.warm_start()
+            t += warm_ms
+            m.set_learned(None)
+        c = c_start = m.cost()
+        reach_lit = t if c <= 1.005 * c_plain else None
+        reach_gain = t if c <= c_plain + 0.005 * gain else None
         for _ in range(4000):
+            if reach_gain is not None and reach_lit is not None:
+                break
             _, ms = m.run_timed(1)
             t += ms
             c = m.cost()
@@ -95,16 +111,20 @@ def time_to_cost(m, M, log_every=20):
                 reach_lit = t
             if reach_gain is None and c <= c_plain + 0.005 * gain:
                 reach_gain = t
-            if reach_gain is not None and reach_lit is not None:
-                break
             if t > 60000.0:
                 break
-        return {"ms_to_within_0.5pct_of_plain_cost": reach_lit, "ms_to_99.5pct_of_plain_gain": reach_gain,
+        return {"start_cost": c_start, "warm_start_ms": round(warm_ms, 4),
+                "ms_to_within_0.5pct_of_plain_cost": reach_lit, "ms_to_99.5pct_of_plain_gain": reach_gain,
                 "final_cost": c}
 
-    out["batched"] = race("batched", lambda: m.set_refine(M.REFINE_BATCHED, rounds=32, tile=4, graph_passes=16))
-    b = out["batched"]["ms_to_99.5pct_of_plain_gain"]
-    out["ratio_batched_vs_plain"] = (b / t_plain) if (b is not None and t_plain > 0) else None
+    batched = lambda: m.set_refine(M.REFINE_BATCHED, rounds=32, tile=4, graph_passes=4)
+    out["batched"] = race(batched)
+    if learned is not None:
+        out["learned_batched"] = race(batched, learned)
+    for k in ("batched", "learned_batched"):
+        if k in out:
+            b = out[k]["ms_to_within_0.5pct_of_plain_cost"]
+            out[f"ratio_{k}_vs_plain"] = round(b / t_plain, 5) if (b is not None and t_plain > 0) else None
     return out
 
 
@@ -145,7 +165,7 @@ def cpu_baseline(frames, gpu_attempts, match_steps, fluid_steps):
     r.ref_morph_next_state(m)
     r.ref_morph_run(m, 1)
     out = np.zeros((H, W, 4), np.uint8)
-    nr = 3
+    nr = 1 if fluid_steps else 3
     t0 = time.perf_counter()
     for i in range(nr):
         r.ref_morph_render(m, r.ref_morph_get_time(m, i * 20 + 7, FRAMES_OUT), op.ptr(out))
@@ -175,6 +195,7 @@ def main():
                     help="fluid grid scale: 0.25 gives a 256x256 grid (+ 20-node border) at 1024^2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-time-to-cost", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="job without the learned warm start")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the morph engine on a "
@@ -183,10 +204,14 @@ def main():
 
     import numpy as np
     import torch
+    from atomorph_amd import learned as L
     from atomorph_amd import morph as M
     from atomorph_amd import synth
 
     torch.cuda.set_device(0)
+    params = None if args.no_learned else L.load_default()
+    if params is None and not args.no_learned:
+        log("no trained warm-start parameters (atomorph_amd/data/warmstart_mlp.npz): learned path off")
     frames = synth.workload(args.workload)
     H, W = frames[0].shape[:2]
     m = build_morph(M, frames)
@@ -198,11 +223,12 @@ def main():
     if not args.no_time_to_cost:
         m.run(0)
         to_morphing(m, M, args.match_steps)
-        log("time-to-cost: plain descent to convergence, then batched")
-        ttc = time_to_cost(m, M)
+        log("time-to-cost: plain descent to convergence, then batched and learned + batched")
+        ttc = time_to_cost(m, M, learned=params)
         log(f"  {ttc}")
 
     m.set(cycle_length=1 << 16, fluid=args.fluid_steps, fluid_sparseness=args.fluid_sparseness)
+    m.set_learned(params)  # the warm start runs right after the chains are built
     m.set_refine(M.REFINE_BATCHED, rounds=32, tile=4, graph_passes=16)
     phase = {"detect_ms": [], "refine_gpu_ms": [], "render_gpu_ms": [], "job_ms": []}
     attempts_job = accepted_job = 0
@@ -267,22 +293,61 @@ def main():
     lib.amh_profile_reset()
     log("kernels (profiled job): " + json.dumps(kernels))
 
-    # roofline of the refinement pass kernel: per attempt it reads the two
-    # slots' points in rows y-1, y, y+1 (8 B each; rows y-1 = y+1 when h = 2)
-    # and writes 16 B per accepted swap.  The passes run inside hipGraph
-    # replays of 16 launches each.
+    # Algorithmic HBM bytes per launch of the hot kernels (what each must move
+    # at least once), divided by their measured average launch time.
+    #   k_refine_pass  per attempt the two slots' points in rows y-1, y, y+1
+    #                  (8 B each; rows y-1 = y+1 when h = 2), 16 B written per
+    #                  accepted swap; the passes run in hipGraph replays of 16.
+    #   fluid kernels  per active particle A the FP64/int state they read and
+    #                  write, per grid node N the FP32 fields:
+    #                  k_p2g    A (7x8 + 4 + 4) read, N 8x4 written
+    #                  k_forces A (2x8 + 4) read, N 3x4 read + 2x4 written
+    #                  k_g2p    A (6x8 + 4 + 4) read + 2x8 written, N 3x4 + 2x4
+    #                  k_advect A (15x8 + 2x4) read + 8x8 written, N 8x4 read
     per_attempt = 8 * 2 * (2 + (1 if h > 2 else 0))
     refine_bytes = attempts_job * per_attempt + accepted_job * 16
+    rooflines = {}
     g = kernels.get("refine_graph")
     if g:
         launch_s = g["ms"] / 1e3 / (16 * g["launches"])
     else:
         launch_s = float(np.mean(phase["refine_gpu_ms"])) / 1e3 / passes
-    achieved = refine_bytes / passes / launch_s / 1e9
+    rooflines["k_refine_pass"] = (refine_bytes / passes, launch_s, passes)
+    if args.fluid_steps:
+        P = m.fluid_particles()
+        A = int((P[:, 16] > 0).sum()) if P.size else 0
+        N = (int(W * args.fluid_sparseness) + 20) * (int(H * args.fluid_sparseness) + 20)
+        per = {"k_p2g": A * 64 + N * 32, "k_forces": A * 20 + N * 20, "k_g2p": A * 72 + N * 20,
+               "k_advect": A * 192 + N * 32}
+        for k, b in per.items():
+            if k in kernels:
+                rooflines[k] = (b, kernels[k]["avg_us"] / 1e6, kernels[k]["launches"])
+    # the dominant kernel: the largest device time in the profiled job
+    def total_ms(k):
+        if k == "k_refine_pass":
+            return g["ms"] if g else float(np.mean(phase["refine_gpu_ms"]))
+        return kernels[k]["ms"]
+    dom = max(rooflines, key=total_ms)
+    dom_bytes, dom_s, _ = rooflines[dom]
+    achieved = dom_bytes / dom_s / 1e9
+    roof_all = {k: {"bytes_per_launch": int(b), "launch_us": round(t * 1e6, 3), "GBs": round(b / t / 1e9, 1),
+                    "total_ms": round(total_ms(k), 3)} for k, (b, t, _) in rooflines.items()}
     # rendering, for comparison: per atom and frame h points + 2 colours read,
     # per output pixel 4 B written
     render_bytes = FRAMES_OUT * (int(widths.sum()) * (8 * h + 8) + H * W * 4)
     render_gbs = render_bytes / (float(np.mean(phase["render_gpu_ms"])) / 1e3) / 1e9
+
+    # matrix-core use of the warm start's MLP inference (one forward of all
+    # atoms per key segment)
+    mlp_stats = None
+    f = kernels.get("mlp_forward")
+    if f and f["ms"] > 0:
+        flop = MLP_FLOP_PER_ROW * int(widths.sum()) * f["launches"]
+        tf = flop / (f["ms"] / 1e3) / 1e12
+        mlp_stats = {"kernel": "k_gemm (v_mfma_f32_32x32x16_bf16) forward, 5 layers", "bound": "mfma",
+                     "flop_per_launch": int(flop / f["launches"]), "avg_us": f["avg_us"],
+                     "achieved": round(tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 5)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -311,12 +376,13 @@ def main():
                    "atoms": int(widths.sum()), "chains": int(len(widths)), "key_frames": h,
                    "match_steps": args.match_steps, "refine_passes": passes, "refine_attempts": attempts_job},
         "phases_ms": {k: round(float(np.mean(v)), 3) for k, v in phase.items()},
-        "roofline": {"kernel": "k_refine_pass (batched refinement pass)", "bound": "hbm",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": int(refine_bytes / passes), "launch_us": round(launch_s * 1e6, 3)},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(dom_bytes), "launch_us": round(dom_s * 1e6, 3)},
+        "rooflines": roof_all,
         "render_gbs": round(render_gbs, 1),
         "kernels": kernels,
+        "mlp": mlp_stats,
         "time_to_cost": ttc,
         "cpu_baseline": cpu,
     }

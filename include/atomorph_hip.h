@@ -321,7 +321,7 @@ int amh_learn_features(const amh_render *r, int32_t y, float *x_dev, float *t_de
                        void *stream);
 /* Reorders rows 1 .. nkeys-1 of pts_dev (== r->pts_dev) in place: row y+1 of
  * every chain is matched to the MLP's predictions from row y by rank in
- * raster order (LSD radix sort of (chain, y, x) keys at 1/256 px), so each
+ * raster order (LSD radix sort of (chain, y, x) keys at 1/16 px), so each
  * row stays a permutation of its points.  nchains <= 65535. */
 int amh_warm_start(void *mlp, const amh_render *r, uint64_t *pts_dev, void *ws_dev, size_t ws_bytes, void *stream);
 
