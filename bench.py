@@ -344,7 +344,8 @@ def main():
     if f and f["ms"] > 0:
         flop = MLP_FLOP_PER_ROW * int(widths.sum()) * f["launches"]
         tf = flop / (f["ms"] / 1e3) / 1e12
-        mlp_stats = {"kernel": "k_gemm (v_mfma_f32_32x32x16_bf16) forward, 5 layers", "bound": "mfma",
+        mlp_stats = {"kernel": "k_mlp_fused (v_mfma_f32_32x32x16_bf16, 5 layers chained in registers)",
+                     "bound": "mfma",
                      "flop_per_launch": int(flop / f["launches"]), "avg_us": f["avg_us"],
                      "achieved": round(tf, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 5)}
@@ -374,6 +375,7 @@ def main():
                                   f"FP32 grid)" if args.fluid_steps else "fluid off")
                                + f", {FRAMES_OUT} frames rendered",
                    "atoms": int(widths.sum()), "chains": int(len(widths)), "key_frames": h,
+                   "blob_pixels": int(sum(int(np.count_nonzero(f[..., 3])) for f in frames)),
                    "match_steps": args.match_steps, "refine_passes": passes, "refine_attempts": attempts_job},
         "phases_ms": {k: round(float(np.mean(v)), 3) for k, v in phase.items()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -4,7 +4,7 @@
  * The drop-in boundary of the atomorph morph-engine hot path.  The C++ class
  * am::morph (atomorph_amd/host/morph.h, same interface as the reference's
  * morph.h:9-76) calls these entry points; any other host language binds them
- * directly (INTEGRATION.md shows ctypes / cgo / JNI stubs).
+ * directly (ctypes, cgo, JNI: plain pointers and sizes only).
  *
  * Conventions
  *  - Every entry point returns AMH_OK (0) or a negative AMH_E_* code and never
