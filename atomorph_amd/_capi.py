@@ -28,6 +28,11 @@ class RefineStats(C.Structure):
     _fields_ = [("attempts", u64), ("accepted", u64), ("gain", i64), ("reserved", u64)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("tile", i32), ("rounds", i32), ("passes", i32), ("reserved", i32), ("ms", f64),
+                ("gain_at_budget", f64), ("gain", i64)]
+
+
 class ChainBuild(C.Structure):
     _fields_ = [("nkeys", i32), ("nchains", i32), ("wtot", i64), ("frame_width", i32), ("pad_", i32),
                 ("chain_off_dev", vp), ("chain_w_dev", vp), ("chain_blob_dev", vp),
@@ -86,6 +91,8 @@ PROTOTYPES = {
                                           vp, vp]),
     "amh_graph_launch": (C.c_int, [vp, vp]),
     "amh_graph_destroy": (C.c_int, [vp]),
+    "amh_refine_autotune": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i32, f64, u64, C.POINTER(i32), C.POINTER(i32),
+                                      vp, i32, C.POINTER(i32), vp]),
     "amh_rgb_to_hsp": (u32, [u32]),
     "amh_hsp_to_rgb": (u32, [u32]),
     "amh_rgb2hsp": (None, [f64, f64, f64, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64)]),

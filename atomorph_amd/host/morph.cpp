@@ -11,6 +11,7 @@
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -131,8 +132,9 @@ struct Options {  // the setters, as the worker sees them (morph.h:97-119)
     bool keep_background = false, blend_blobs = false, finite = false;
     unsigned fluidsteps = 0, show_blobs = TEXTURE;
     unsigned seed = 0;
-    unsigned refine_mode = REFINE_PLAIN, refine_rounds = 0, graph_passes = 0;  // rounds 0: default
-    int refine_tile = 2;
+    // batched: rounds 0 or tile < 0 = autotuned for the chains
+    unsigned refine_mode = REFINE_PLAIN, refine_rounds = 0, graph_passes = 0;
+    int refine_tile = -1;
     size_t match_budget = 0;
     double fluid_sparseness = 1.0;  // grid cells per pixel (the reference's sparseness)
     bool learned = false;           // learned warm start after the chains are built
@@ -209,6 +211,9 @@ struct morph::engine {
     void *graph = nullptr;
     int graph_tile = -1;
     unsigned graph_np = 0, graph_rounds = 0;
+    // autotuned batched schedule for the current chains (-1: not yet)
+    int tuned_tile = -1, tuned_rounds = 0;
+    DevBuf d_tune;
     // render snapshot (made at synchronize)
     DevBuf d_snap, d_rank, d_brgba, d_lag, d_slope, d_rws, d_out;
     std::vector<int32_t> rank;                    // [h][nchains]
@@ -259,6 +264,7 @@ struct morph::engine {
     void fix_volatiles(size_t c);
     bool init_chains();
     bool morph_step();
+    bool autotune(double budget_ms, std::vector<amh_tune_entry> *report);
     bool exact_cost(uint64_t *lo, uint64_t *hi);
     bool snapshot();
     void run();
@@ -325,6 +331,8 @@ void morph::engine::reset_worker() {
     cw.clear();
     nitems = 0;
     items_tile = -1;
+    tuned_tile = -1;
+    tuned_rounds = 0;
     visits = passes = 0;
     if (graph) {
         amh_graph_destroy(graph);
@@ -740,6 +748,40 @@ bool morph::engine::init_chains() {
     return true;
 }
 
+// Autotuned batched schedules, shared by every engine of the process and
+// keyed by the chain shapes (key frames, then every chain width): a job on
+// the same shapes reuses the tile and rounds measured for the first one.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::vector<int64_t>, std::pair<int, int>> g_tune;
+constexpr double kTuneBudgetMs = 3.0;
+}  // namespace
+
+bool morph::engine::autotune(double budget_ms, std::vector<amh_tune_entry> *report) {
+    const size_t nch = cw.size();
+    if (nch == 0 || h < 2 || wtot == 0) {
+        err = "autotune: no chains";
+        return false;
+    }
+    if (!d_tune.ensure(sizeof(uint64_t) * size_t(wtot) * h)) return fail("device allocation");
+    std::vector<amh_tune_entry> rep(64);
+    int32_t t = -1, r = 0, n = 0;
+    const bool ok = amh_refine_autotune(d_pts.as<uint64_t>(), d_tune.as<uint64_t>(), h, wtot, d_off.as<int64_t>(),
+                                        d_w.as<int32_t>(), cw.data(), int32_t(nch), budget_ms, mix_seed(), &t, &r,
+                                        rep.data(), int32_t(rep.size()), &n, cstream) == AMH_OK;
+    d_tune.release();
+    if (!ok) return fail("autotune");
+    rep.resize(std::min<size_t>(size_t(std::max(n, 0)), rep.size()));
+    if (report) *report = rep;
+    tuned_tile = t;
+    tuned_rounds = r;
+    std::vector<int64_t> key{int64_t(h)};
+    key.insert(key.end(), cw.begin(), cw.end());
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = {t, r};
+    return true;
+}
+
 // ATOM_MORPHING: one refinement step.  Plain: one visit of one chain
 // (thread::morph, thread.cpp:1043-1068: chain counter % W, one random key
 // frame, cycle_length attempts as conflict-free rounds).  Batched: one pass
@@ -748,8 +790,24 @@ bool morph::engine::morph_step() {
     const size_t nch = cw.size();
     if (nch == 0 || h < 2) return false;
     if (wopt.refine_mode == REFINE_BATCHED) {
-        const int tile = wopt.refine_tile < 0 ? 2 : wopt.refine_tile;
-        const int rounds = wopt.refine_rounds > 0 ? int(wopt.refine_rounds) : 8;
+        int tile = wopt.refine_tile, rounds = int(wopt.refine_rounds);
+        if (tile < 0 || rounds <= 0) {  // autotuned: once per chain shape
+            if (tuned_tile < 0) {
+                std::vector<int64_t> key{int64_t(h)};
+                key.insert(key.end(), cw.begin(), cw.end());
+                {
+                    std::lock_guard<std::mutex> lk(g_tune_mu);
+                    auto it = g_tune.find(key);
+                    if (it != g_tune.end()) {
+                        tuned_tile = it->second.first;
+                        tuned_rounds = it->second.second;
+                    }
+                }
+                if (tuned_tile < 0 && !autotune(kTuneBudgetMs, nullptr)) return true;
+            }
+            if (tile < 0) tile = tuned_tile;
+            if (rounds <= 0) rounds = tuned_rounds;
+        }
         if (items_tile != tile) {
             int32_t n = 0;
             if (amh_refine_items(cw.data(), int32_t(nch), tile, nullptr, 0, &n)) return fail("refine items");
@@ -1571,6 +1629,21 @@ bool morph::warm_start(float *gpu_ms) {
     if (gpu_ms) ok = amh_timer_end(timer, e.cstream, gpu_ms) == 0 && ok;
     amh_timer_free(timer);
     return ok && amh_stream_sync(e.cstream) == 0 && e.snapshot();
+}
+
+bool morph::autotune(double budget_ms, int *tile, unsigned *rounds, std::vector<tune_entry> *report) {
+    engine &e = *E;
+    if (is_busy() || e.wstate != STATE_ATOM_MORPHING) { e.err = "autotune: not paused in ATOM_MORPHING"; return false; }
+    std::vector<amh_tune_entry> rep;
+    if (!e.autotune(budget_ms, &rep)) return false;
+    if (tile) *tile = e.tuned_tile;
+    if (rounds) *rounds = unsigned(e.tuned_rounds);
+    if (report) {
+        report->clear();
+        for (const amh_tune_entry &r : rep)
+            report->push_back({r.tile, unsigned(r.rounds), r.passes, r.ms, r.gain_at_budget});
+    }
+    return true;
 }
 
 size_t morph::learn_samples(std::vector<float> *x, std::vector<float> *t) {

@@ -29,6 +29,15 @@
 
 namespace am {
 
+// One candidate measured by morph::autotune.
+struct tune_entry {
+    int      tile;            // batched kernel variant
+    unsigned rounds;          // rounds per pass
+    int      passes;          // passes run to reach the budget
+    double   ms;              // their device time
+    double   gain_at_budget;  // exact cost decrease at the budget
+};
+
 class morph {
   public:
     morph();
@@ -123,9 +132,18 @@ class morph {
     bool        transport_cost      (uint64_t *lo, uint64_t *hi);
     // Refinement schedule: REFINE_PLAIN (default; `rounds` rounds per chain
     // visit, 0 = cycle_length attempts like the reference) or REFINE_BATCHED
-    // (`rounds` rounds per pass, 0 = 8), kernel variant `tile` and
+    // (`rounds` rounds per pass, kernel variant `tile`; rounds 0 or tile < 0
+    // = autotuned on the first pass over new chains, see autotune) with
     // `graph_passes` passes per hipGraph replay (0 = no graph).
     void        set_refine          (unsigned mode, unsigned rounds, int tile, unsigned graph_passes);
+    // Autotunes the batched descent on the current chains now (paused in
+    // ATOM_MORPHING): every kernel variant x rounds per pass races from the
+    // chains' present state for budget_ms of device time on a scratch copy
+    // (the chains are not changed); the largest exact cost decrease wins.
+    // The result is used by batched passes with automatic tile or rounds and
+    // remembered for chains of the same shapes.  report may be NULL.
+    bool        autotune            (double budget_ms, int *tile, unsigned *rounds,
+                                     std::vector<tune_entry> *report);
     // Matching ends by itself after this many steps (0 = only at
     // best energy 0 or next_state(), as in the reference).
     void        set_match_budget    (size_t steps);

@@ -157,6 +157,22 @@ int am_morph_put_chains(void *mp, const uint64_t *pts, size_t n) {
     std::vector<uint64_t> p(pts, pts + n);
     return static_cast<am::morph *>(mp)->put_chain_points(p) ? 0 : -1;
 }
+// report: max_report rows of (tile, rounds, passes, ms, gain_at_budget)
+int am_morph_autotune(void *mp, double budget_ms, int *tile, unsigned *rounds, double *report, size_t max_report,
+                      size_t *nreport) {
+    std::vector<am::tune_entry> rep;
+    if (!static_cast<am::morph *>(mp)->autotune(budget_ms, tile, rounds, &rep)) return -1;
+    for (size_t i = 0; i < rep.size() && i < max_report; ++i) {
+        double *r = report + 5 * i;
+        r[0] = rep[i].tile;
+        r[1] = rep[i].rounds;
+        r[2] = rep[i].passes;
+        r[3] = rep[i].ms;
+        r[4] = rep[i].gain_at_budget;
+    }
+    if (nreport) *nreport = rep.size();
+    return 0;
+}
 void am_morph_set_learned(void *m, const float *params) { static_cast<am::morph *>(m)->set_learned(params); }
 int am_morph_warm_start(void *m, float *gpu_ms) { return static_cast<am::morph *>(m)->warm_start(gpu_ms) ? 0 : -1; }
 size_t am_morph_learn_samples(void *mp, float *x, float *t) {

@@ -49,6 +49,8 @@ _PROTOS = {
     "am_morph_fluid_particles": (C.c_size_t, [vp, vp]),
     "am_morph_set_learned": (None, [vp, vp]),
     "am_morph_warm_start": (C.c_int, [vp, C.POINTER(C.c_float)]),
+    "am_morph_autotune": (C.c_int, [vp, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_uint), vp, sz,
+                                    C.POINTER(sz)]),
     "am_morph_learn_samples": (C.c_size_t, [vp, vp, vp]),
     "am_morph_get_time": (C.c_double, [vp, sz, sz]),
     "am_morph_render": (C.c_int, [vp, C.c_double, vp]),
@@ -186,6 +188,19 @@ class Morph:
         if self.lib.am_morph_warm_start(self.h, C.byref(ms)) != 0:
             raise _capi.AmhError(self.error())
         return ms.value
+
+    def autotune(self, budget_ms=3.0):
+        """Autotune the batched descent on the current chains (which it does
+        not change): returns (tile, rounds, report), report one dict per
+        candidate with its passes, device ms and exact gain at the budget."""
+        t, r, n = C.c_int(), C.c_uint(), sz()
+        rep = np.zeros((64, 5), np.float64)
+        if self.lib.am_morph_autotune(self.h, budget_ms, C.byref(t), C.byref(r), rep.ctypes.data_as(vp), 64,
+                                      C.byref(n)) != 0:
+            raise _capi.AmhError(self.error())
+        rows = [{"tile": int(a), "rounds": int(b), "passes": int(c), "ms": float(d), "gain_at_budget": float(e)}
+                for a, b, c, d, e in rep[:min(n.value, 64)]]
+        return t.value, r.value, rows
 
     def learn_samples(self):
         """(features (n, 8), targets (n, 2)) float32 from the synchronized chains."""

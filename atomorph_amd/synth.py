@@ -10,7 +10,10 @@ Workloads:
                    CPU-runnable parity case;
   noise_blob_frames  thresholded fBm gradient noise (octaves 4, persistence
                    0.5), the N largest 4-connected components kept, one
-                   random colour per blob -- the 512^2 / 1024^2 workloads.
+                   random colour per blob -- the 512^2 / 1024^2 workloads;
+  powerlaw_blob_frame  N separate blobs (rotated filled ellipses) whose areas
+                   follow a power law p(s) ~ s^-alpha on [smin, smax] -- the
+                   2048^2 workload.
 """
 import numpy as np
 
@@ -122,6 +125,66 @@ def noise_blob_frames(size, nframes, nblobs, coverage, seed):
     return [noise_blob_frame(size, nblobs, coverage, seed * 1000003 + f)[0] for f in range(nframes)]
 
 
+def powerlaw_sizes(n, alpha, smin, smax, rng):
+    """n areas from p(s) ~ s^-alpha on [smin, smax] by inverting the CDF."""
+    e = 1.0 - alpha
+    lo, hi = smin ** e, smax ** e
+    return np.array([int(round((lo + rng.uniform() * (hi - lo)) ** (1.0 / e))) for _ in range(n)], np.int64)
+
+
+def _ellipse_mask(area, aspect, theta):
+    """Filled ellipse of about `area` pixels, its largest 4-connected part."""
+    from scipy import ndimage
+    a = np.sqrt(area * aspect / np.pi)
+    b = np.sqrt(area / (aspect * np.pi))
+    r = int(np.ceil(a)) + 1
+    yy, xx = np.mgrid[-r:r + 1, -r:r + 1].astype(np.float64)
+    c, s = np.cos(theta), np.sin(theta)
+    u, v = xx * c + yy * s, -xx * s + yy * c
+    m = (u / a) ** 2 + (v / b) ** 2 <= 1.0
+    if not m.any():
+        m[r, r] = True
+    lab, n = ndimage.label(m, structure=[[0, 1, 0], [1, 1, 1], [0, 1, 0]])
+    if n > 1:
+        m = lab == (np.argmax(np.bincount(lab.ravel())[1:]) + 1)
+    ys, xs = np.nonzero(m)
+    return m[ys.min():ys.max() + 1, xs.min():xs.max() + 1]
+
+
+def powerlaw_blob_frame(size, nblobs, alpha, smin, smax, seed, tries=200):
+    """One frame of `nblobs` blobs that never touch (a free pixel between
+    any two), areas from powerlaw_sizes, placed largest first at uniform
+    random positions, each with a random colour.  Returns (frame, areas)."""
+    from scipy import ndimage
+    rng = SplitMix64(seed)
+    areas = np.sort(powerlaw_sizes(nblobs, alpha, smin, smax, rng))[::-1]
+    taken = np.zeros((size, size), bool)        # blob pixels and their 4-neighbours
+    img = np.zeros((size, size, 4), np.uint8)
+    plus = np.array([[0, 1, 0], [1, 1, 1], [0, 1, 0]], bool)
+    placed = []
+    for area in areas:
+        m = _ellipse_mask(float(area), rng.uniform(1.0, 3.0), rng.uniform(0.0, np.pi))
+        mh, mw = m.shape
+        if mh > size or mw > size:
+            raise ValueError("blob larger than the frame")
+        for _ in range(tries):
+            y0, x0 = rng.randint(0, size - mh), rng.randint(0, size - mw)
+            if not (taken[y0:y0 + mh, x0:x0 + mw] & m).any():
+                break
+        else:
+            raise RuntimeError("no free place for a blob; lower the count or the sizes")
+        grown = ndimage.binary_dilation(np.pad(m, 1), structure=plus)
+        ty0, tx0 = y0 - 1, x0 - 1
+        gy0, gx0 = max(ty0, 0), max(tx0, 0)
+        gy1, gx1 = min(ty0 + mh + 2, size), min(tx0 + mw + 2, size)
+        taken[gy0:gy1, gx0:gx1] |= grown[gy0 - ty0:gy1 - ty0, gx0 - tx0:gx1 - tx0]
+        region = img[y0:y0 + mh, x0:x0 + mw]
+        region[m, 0:3] = [rng.randint(0, 255) for _ in range(3)]
+        region[m, 3] = 255
+        placed.append(int(m.sum()))
+    return img, np.array(placed, np.int64)
+
+
 # The named workloads.
 def workload(name):
     if name == "ellipses64":      # 2 x 64^2, 6 ellipses, seed 1
@@ -130,6 +193,8 @@ def workload(name):
         return noise_blob_frames(512, 2, 40, 0.36, seed=2)
     if name == "blobs1024":       # 4 x 1024^2, 150 blobs per frame
         return noise_blob_frames(1024, 4, 150, 0.18, seed=3)
+    if name == "powerlaw2048":    # 2 x 2048^2, 1200 blobs, areas ~ s^-1.8 on [8, 60000]
+        return [powerlaw_blob_frame(2048, 1200, 1.8, 8, 60000, seed=5 * 1000003 + f)[0] for f in range(2)]
     raise KeyError(name)
 
 
