@@ -76,13 +76,13 @@ def build_host(verbose=False):
     lib_srcs = [s for s in srcs if not os.path.basename(s).startswith("cli")]
     out = os.path.join(LIB, "libatomorph.so")
     _run(["g++"] + CXX_FLAGS + ["-shared", "-I", INCLUDE, "-I", HOST, "-o", out + ".tmp"] + lib_srcs +
-         ["-L", LIB, "-latomorph_hip", "-Wl,-rpath,$ORIGIN"])
+         ["-L", LIB, "-latomorph_hip", "-lz", "-Wl,-rpath,$ORIGIN"])
     os.replace(out + ".tmp", out)
     cli = [s for s in srcs if os.path.basename(s).startswith("cli")]
     if cli:
         exe = os.path.join(LIB, "atomorph_cli")
         _run(["g++"] + CXX_FLAGS + ["-I", INCLUDE, "-I", HOST, "-o", exe] + cli +
-             ["-L", LIB, "-latomorph", "-latomorph_hip", "-Wl,-rpath,$ORIGIN"])
+             ["-L", LIB, "-latomorph", "-latomorph_hip", "-lz", "-Wl,-rpath,$ORIGIN"])
     return out
 
 

@@ -1,0 +1,71 @@
+"""The command-line driver on the GPU: PNG key frames in, PNG frames out,
+the reference's flags and file names; with iteration budgets the frames equal
+the engine's own rendering of the same run through the library API."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from atomorph_amd import _capi as A
+from atomorph_amd import learned as L
+from atomorph_amd import morph as M
+from atomorph_amd import synth
+from tests import gpu_util as G
+from tests import png_util as P
+
+pytestmark = pytest.mark.gpu
+
+CLI = os.path.join(A.LIB_DIR, "atomorph_cli")
+
+
+def _library_run(frames, match_steps, morph_steps, frames_out, learned=None, batched=False):
+    """The driver's counted loop through the library API (CLI defaults:
+    density 2, degeneration 10000, cycle length 100000)."""
+    m = M.Morph()
+    m.set(seed=1, cycle_length=100000, degeneration=10000, density=2, threads=8)
+    if batched:
+        m.set_refine(M.REFINE_BATCHED, rounds=8, tile=2, graph_passes=16)
+    if learned is not None:
+        m.set_learned(learned)
+    for i, f in enumerate(frames):
+        m.add_frame_rgba(i, f)
+    hh, ww = frames[0].shape[:2]
+    m.set_resolution(ww, hh)
+    assert m.run(1) == M.STATE_BLOB_UNIFICATION
+    assert m.run(1) == M.STATE_BLOB_MATCHING
+    if m.run(match_steps + 1) == M.STATE_BLOB_MATCHING:
+        m.next_state()
+    assert m.run(1) == M.STATE_ATOM_MORPHING
+    m.run(morph_steps)
+    m.next_state()
+    assert m.run(1) == M.STATE_DONE
+    return [m.render(m.get_time(i, frames_out)) for i in range(frames_out)]
+
+
+@pytest.mark.parametrize("variant", ["plain", "batched_learned"])
+def test_cli_frames_equal_library_frames(tmp_path, variant):
+    if not os.path.exists(CLI):
+        pytest.fail("atomorph_cli not built: python -m atomorph_amd.build")
+    frames = synth.workload("ellipses64")
+    names = []
+    for i, f in enumerate(frames):
+        names.append(f"key_{i}.png")
+        P.write_rgba8(tmp_path / names[-1], f, filters=(0, 1, 2, 3, 4))
+    out = tmp_path / "out"
+    out.mkdir()
+    cmd = [CLI, "-i", str(tmp_path), "-o", str(out), "-f", "t", "-s", "1", "-F", "8", "--match-steps", "500",
+           "--morph-steps", "6"]
+    learned = None
+    if variant == "batched_learned":
+        learned = L.load_default()
+        cmd += ["--batched", "--tile", "2", "--rounds", "8", "--learned", L.DEFAULT_PARAMS]
+    G.log(f"cli {variant}: {' '.join(cmd[1:])}")
+    r = subprocess.run(cmd + names, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = _library_run(frames, 500, 6, 8, learned=learned, batched=variant == "batched_learned")
+    for i in range(8):
+        got = P.read_rgba8(out / f"t_{i + 1:04d}.png")
+        assert got.shape == (64, 64, 4)
+        assert np.array_equal(got, want[i]), f"frame {i + 1}"
+    assert (want[0][..., 3] > 0).sum() > 100
