@@ -213,15 +213,18 @@ __device__ __forceinline__ float nd_m(const float *f, const float *m, int i, int
 
 // pressure from the bicubic density interpolation, wall forces
 // (fluidmodel.cpp:251-357)
-__device__ __forceinline__ void particle_forces(double x, double y, const Grid &Gd, int gsx, int gsy,
+// node(f, i, j): field f (0 mass, 1 density gradient x, 2 gradient y) of
+// node (i, j), 0 outside the grid.
+template <class Node>
+__device__ __forceinline__ void particle_forces(double x, double y, const Node &node, int gsx, int gsy,
                                                 double &pressure_out, double &fx_out, double &fy_out) {
     const int cx = static_cast<int>(x), cy = static_cast<int>(y);
-    const double d01 = nd(Gd.m, cx, cy, gsx, gsy), d02 = nd(Gd.m, cx, cy + 1, gsx, gsy);
-    const double d11 = nd(Gd.m, cx + 1, cy, gsx, gsy), d12 = nd(Gd.m, cx + 1, cy + 1, gsx, gsy);
-    const double gx01 = nd(Gd.gx, cx, cy, gsx, gsy), gx02 = nd(Gd.gx, cx, cy + 1, gsx, gsy);
-    const double gx11 = nd(Gd.gx, cx + 1, cy, gsx, gsy), gx12 = nd(Gd.gx, cx + 1, cy + 1, gsx, gsy);
-    const double gy01 = nd(Gd.gy, cx, cy, gsx, gsy), gy02 = nd(Gd.gy, cx, cy + 1, gsx, gsy);
-    const double gy11 = nd(Gd.gy, cx + 1, cy, gsx, gsy), gy12 = nd(Gd.gy, cx + 1, cy + 1, gsx, gsy);
+    const double d01 = node(0, cx, cy), d02 = node(0, cx, cy + 1);
+    const double d11 = node(0, cx + 1, cy), d12 = node(0, cx + 1, cy + 1);
+    const double gx01 = node(1, cx, cy), gx02 = node(1, cx, cy + 1);
+    const double gx11 = node(1, cx + 1, cy), gx12 = node(1, cx + 1, cy + 1);
+    const double gy01 = node(2, cx, cy), gy02 = node(2, cx, cy + 1);
+    const double gy11 = node(2, cx + 1, cy), gy12 = node(2, cx + 1, cy + 1);
     const double pdx = d11 - d01, pdy = d02 - d01;
     const double C20 = 3.0 * pdx - gx11 - 2.0 * gx01;
     const double C02 = 3.0 * pdy - gy02 - 2.0 * gy01;
@@ -431,6 +434,32 @@ __device__ __forceinline__ bool in_reach(int cx, int cy, int x0, int y0) {
     return cx >= x0 - 2 && cx < x0 + kTile && cy >= y0 - 2 && cy < y0 + kTile;
 }
 
+// Node values a tile block reads, staged in LDS once per block: nodes
+// x0-2 .. x0+kTile+1 by y0-2 .. y0+kTile+1 cover the stencil of every
+// particle in reach of the tile (origins x0-2 .. x0+kTile-1, 3 nodes) and
+// the bicubic density corners of particle_forces (int(x) <= origin + 1).
+// Nodes outside the grid hold 0, as nd() reads them.
+constexpr int kHalo = kTile + 4;
+
+__device__ __forceinline__ int halo_index(int i, int j, int x0, int y0) {
+    return (i - (x0 - 2)) * kHalo + (j - (y0 - 2));
+}
+
+// stage(f, k) = value of field f at grid index k; F fields into sm.
+template <int F, class Stage>
+__device__ __forceinline__ void stage_halo(float (&sm)[F][kHalo * kHalo], int x0, int y0, int gsx, int gsy,
+                                           const Stage &stage) {
+    for (int e = threadIdx.x; e < kHalo * kHalo; e += kT) {
+        const int i = x0 - 2 + e / kHalo, j = y0 - 2 + e % kHalo;
+        if (node_ok(i, j, gsx, gsy)) {
+            const int64_t k = int64_t(i) * gsy + j;
+            for (int f = 0; f < F; ++f) sm[f][e] = stage(f, k);
+        } else {
+            for (int f = 0; f < F; ++f) sm[f][e] = 0.0f;
+        }
+    }
+}
+
 struct P2gItem {
     double x, y, s, r, g, b, a;
     int mature;
@@ -534,10 +563,13 @@ __device__ __forceinline__ void store_tile2(const float (&acc)[2][kTile * kTile]
 // Node accelerations: pressure and wall forces (fluidmodel.cpp:251-357).
 __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float acc[2][kTile * kTile];
+    __shared__ float nodes[3][kHalo * kHalo];  // m, gx, gy
     const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
+    stage_halo(nodes, x0, y0, gsx, gsy, [&](int f, int64_t k) { return f == 0 ? Gd.m[k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
     __syncthreads();
+    const auto node = [&](int f, int i, int j) { return nodes[f][halo_index(i, j, x0, y0)]; };
     auto fetch = [&](int slot) {
         double2 it = {0.0, 0.0};
         if (slot >= 0) it = double2{P.x[slot], P.y[slot]};
@@ -552,7 +584,7 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
             const double x = it.x, y = it.y;
             weights(x, y, cx, cy, px, py, gx, gy);
             valid = in_reach(cx, cy, x0, y0);
-            if (valid) particle_forces(x, y, Gd, gsx, gsy, pressure, fx, fy);
+            if (valid) particle_forces(x, y, node, gsx, gsy, pressure, fx, fy);
         }
         const Seg S = segments(valid, cx, cy);
         for (int a = 0; a < 3; ++a)
@@ -577,9 +609,11 @@ struct G2pItem {
 // (fluidmodel.cpp:359-437).
 __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float acc[2][kTile * kTile];
+    __shared__ float accel[2][kHalo * kHalo];  // node accelerations per mass
     const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
+    stage_halo(accel, x0, y0, gsx, gsy, [&](int f, int64_t k) { return per_mass(f == 0 ? Gd.ax[k] : Gd.ay[k], Gd.m[k]); });
     __syncthreads();
     auto fetch = [&](int slot) {
         G2pItem it = {0, 0, 0, 0, 0, 0, 0};
@@ -605,8 +639,15 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
                 for (int a = 0; a < 3; ++a)
                     for (int b = 0; b < 3; ++b) {
                         const double phi = px[a] * py[b];
-                        const double ax = nd_m(Gd.ax, Gd.m, cx + a, cy + b, gsx, gsy);
-                        const double ay = nd_m(Gd.ay, Gd.m, cx + a, cy + b, gsx, gsy);
+                        double ax, ay;
+                        if (valid) {  // in reach: the node is staged
+                            const int e = halo_index(cx + a, cy + b, x0, y0);
+                            ax = accel[0][e];
+                            ay = accel[1][e];
+                        } else {
+                            ax = nd_m(Gd.ax, Gd.m, cx + a, cy + b, gsx, gsy);
+                            ay = nd_m(Gd.ay, Gd.m, cx + a, cy + b, gsx, gsy);
+                        }
                         u += phi * (ax + cax);
                         v += phi * (ay + cay);
                     }
@@ -640,70 +681,103 @@ __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
     return double(mix64(seed ^ (uint64_t(i) << 3) ^ uint64_t(k)) >> 11) * (1.0 / 9007199254740992.0) * 0.01;
 }
 
-// atom: the particle held in each slot when P is the cell-ordered copy
-// (NULL: slot i holds particle i); the wall jitter is keyed by particle.
-__global__ void k_advect(Particles P, int n, Grid Gd, int gsx, int gsy, int steps_left, double freedom_radius,
-                         uint64_t seed, const int32_t *__restrict__ atom) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n || !P.active[i]) return;
-    const int id = atom ? atom[i] : i;
-    int cx, cy;
-    double px[3], py[3], gx[3], gy[3];
-    double x = P.x[i], y = P.y[i];
-    weights(x, y, cx, cy, px, py, gx, gy);
-    double gu = 0, gv = 0, nR = 0, nG = 0, nB = 0, nA = 0, wsum = 0;
-    for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-            if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
-            const int64_t k = int64_t(cx + a) * gsy + cy + b;
-            const double phi = px[a] * py[b];
-            const float mk = Gd.m[k];
-            gu += phi * per_mass(Gd.u[k], mk);
-            gv += phi * per_mass(Gd.v[k], mk);
-            const double w = Gd.w[k];
-            if (w > 0.0) {
-                // node colour = sum(s c)/sum(s); weighted by sum(s)
-                wsum += w;
-                nR += Gd.r[k];
-                nG += Gd.g[k];
-                nB += Gd.b[k];
-                nA += Gd.a[k];
+// Advection of every active particle by the block of its home tile (each
+// particle has exactly one home entry), with the node velocities and
+// colours of the tile's halo staged in LDS.  atom: the particle held in each
+// slot when P is the cell-ordered copy (NULL: slot i holds particle i); the
+// wall jitter is keyed by particle.
+__global__ __launch_bounds__(kT) void k_advect(Particles P, Grid Gd, int gsx, int gsy, Bins B, int steps_left,
+                                               double freedom_radius, uint64_t seed,
+                                               const int32_t *__restrict__ atom) {
+    __shared__ float nodes[7][kHalo * kHalo];  // u/m, v/m, w, r, g, b, a
+    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
+    const int x0 = tx * kTile, y0 = ty * kTile;
+    stage_halo(nodes, x0, y0, gsx, gsy, [&](int f, int64_t k) {
+        switch (f) {
+            case 0: return per_mass(Gd.u[k], Gd.m[k]);
+            case 1: return per_mass(Gd.v[k], Gd.m[k]);
+            case 2: return Gd.w[k];
+            case 3: return Gd.r[k];
+            case 4: return Gd.g[k];
+            case 5: return Gd.b[k];
+            default: return Gd.a[k];
+        }
+    });
+    __syncthreads();
+    auto fetch = [&](int slot) {
+        double2 it = {0.0, 0.0};
+        if (slot >= 0) it = double2{P.x[slot], P.y[slot]};
+        return it;
+    };
+    for_bin(B, int(blockIdx.x), fetch, [&](int i, bool home, const double2 &it) {
+        if (i < 0 || !home) return;
+        const int id = atom ? atom[i] : i;
+        int cx, cy;
+        double px[3], py[3], gx[3], gy[3];
+        double x = it.x, y = it.y;
+        weights(x, y, cx, cy, px, py, gx, gy);
+        const bool staged = in_reach(cx, cy, x0, y0);
+        double gu = 0, gv = 0, nR = 0, nG = 0, nB = 0, nA = 0, wsum = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) {
+                if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
+                const double phi = px[a] * py[b];
+                float uu, vv, w, r, g, bb, aa;
+                if (staged) {
+                    const int e = halo_index(cx + a, cy + b, x0, y0);
+                    uu = nodes[0][e]; vv = nodes[1][e]; w = nodes[2][e];
+                    r = nodes[3][e]; g = nodes[4][e]; bb = nodes[5][e]; aa = nodes[6][e];
+                } else {
+                    const int64_t k = int64_t(cx + a) * gsy + cy + b;
+                    uu = per_mass(Gd.u[k], Gd.m[k]); vv = per_mass(Gd.v[k], Gd.m[k]); w = Gd.w[k];
+                    r = Gd.r[k]; g = Gd.g[k]; bb = Gd.b[k]; aa = Gd.a[k];
+                }
+                gu += phi * uu;
+                gv += phi * vv;
+                if (double(w) > 0.0) {
+                    // node colour = sum(s c)/sum(s); weighted by sum(s)
+                    wsum += w;
+                    nR += r;
+                    nG += g;
+                    nB += bb;
+                    nA += aa;
+                }
+            }
+        if (wsum > 0.0) {  // colour diffusion (fluidmodel.cpp:470-495)
+            nR /= wsum; nG /= wsum; nB /= wsum; nA /= wsum;
+            if (!P.mature[i]) {
+                P.r[i] = nR; P.g[i] = nG; P.b[i] = nB; P.a[i] = nA;
+            } else {
+                const double wr = fabs(nR - P.R[i]), wg = fabs(nG - P.G[i]), wb = fabs(nB - P.B[i]),
+                             wa = fabs(nA - P.A[i]);
+                P.r[i] = (1.0 - wr) * P.r[i] + wr * nR;
+                P.g[i] = (1.0 - wg) * P.g[i] + wg * nG;
+                P.b[i] = (1.0 - wb) * P.b[i] + wb * nB;
+                P.a[i] = (1.0 - wa) * P.a[i] + wa * nA;
             }
         }
-    if (wsum > 0.0) {  // colour diffusion (fluidmodel.cpp:470-495)
-        nR /= wsum; nG /= wsum; nB /= wsum; nA /= wsum;
-        if (!P.mature[i]) {
-            P.r[i] = nR; P.g[i] = nG; P.b[i] = nB; P.a[i] = nA;
-        } else {
-            const double wr = fabs(nR - P.R[i]), wg = fabs(nG - P.G[i]), wb = fabs(nB - P.B[i]),
-                         wa = fabs(nA - P.A[i]);
-            P.r[i] = (1.0 - wr) * P.r[i] + wr * nR;
-            P.g[i] = (1.0 - wg) * P.g[i] + wg * nG;
-            P.b[i] = (1.0 - wb) * P.b[i] + wb * nB;
-            P.a[i] = (1.0 - wa) * P.a[i] + wa * nA;
+        x += gu;
+        y += gv;
+        {   // freedom radius (fluidmodel.cpp:501-539)
+            const double A = fabs(y - P.gy[i]), Bd = fabs(x - P.gx[i]), Cd = sqrt(A * A + Bd * Bd);
+            const double r = freedom_radius * P.fr[i];
+            if (Cd > r) {
+                double mx, my;
+                toward(x, y, P.gx[i], P.gy[i], Cd - r, mx, my);
+                const double w = 1.0 / (steps_left + 1);
+                x += mx * w;
+                y += my * w;
+            }
         }
-    }
-    x += gu;
-    y += gv;
-    {   // freedom radius (fluidmodel.cpp:501-539)
-        const double A = fabs(y - P.gy[i]), B = fabs(x - P.gx[i]), Cd = sqrt(A * A + B * B);
-        const double r = freedom_radius * P.fr[i];
-        if (Cd > r) {
-            double mx, my;
-            toward(x, y, P.gx[i], P.gy[i], Cd - r, mx, my);
-            const double w = 1.0 / (steps_left + 1);
-            x += mx * w;
-            y += my * w;
-        }
-    }
-    double u = P.u2[i], v = P.v2[i];
-    u += gu - u;
-    v += gv - v;
-    if (x < 1.0) { x = 1.0 + jitter(seed, id, 0); u = 0.0; }
-    else if (x > gsx - 2) { x = gsx - 2 - jitter(seed, id, 1); u = 0.0; }
-    if (y < 1.0) { y = 1.0 + jitter(seed, id, 2); v = 0.0; }
-    else if (y > gsy - 2) { y = gsy - 2 - jitter(seed, id, 3); v = 0.0; }
-    P.x[i] = x; P.y[i] = y; P.u[i] = u; P.v[i] = v;
+        double u = P.u2[i], v = P.v2[i];
+        u += gu - u;
+        v += gv - v;
+        if (x < 1.0) { x = 1.0 + jitter(seed, id, 0); u = 0.0; }
+        else if (x > gsx - 2) { x = gsx - 2 - jitter(seed, id, 1); u = 0.0; }
+        if (y < 1.0) { y = 1.0 + jitter(seed, id, 2); v = 0.0; }
+        else if (y > gsy - 2) { y = gsy - 2 - jitter(seed, id, 3); v = 0.0; }
+        P.x[i] = x; P.y[i] = y; P.u[i] = u; P.v[i] = v;
+    });
 }
 
 __global__ void k_fill_i32(int32_t *p, int64_t n) {
@@ -990,8 +1064,8 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     }
     {
         amh::ProfScope ps_("k_advect", s);
-        hipLaunchKernelGGL(k_advect, dim3(gp), dim3(kT), 0, s, Q, f->count, f->G, f->gsx, f->gsy, int(steps_left),
-                           freedom_radius, seed, atom);
+        hipLaunchKernelGGL(k_advect, dim3(uint32_t(tiles)), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B,
+                           int(steps_left), freedom_radius, seed, atom);
     }
     AMH_LAUNCH_CHECK("amh_fluid_step");
     return AMH_OK;
@@ -1275,7 +1349,100 @@ __global__ __launch_bounds__(kT) void k_fluid_compose(amh_render R, Frame F, con
     out[p] = c;
 }
 
+// ------------------------------------------------------ activation ranks
+// Which atoms of a chain carry the round((1-t)|A| + t|B|) active particles
+// of key segment y -> y+1: slots ordered by class (HAS_FLUID in both rows,
+// only in row y, only in row y+1, in neither), then by a pseudo-random
+// permutation of the slots keyed by (seed, chain, y).  One block per
+// (y, chain): pass 1 counts the classes, pass 2 walks the slots in
+// permutation order and hands out ranks with a block-wide ballot scan.
+constexpr int kRankT = 256;
+constexpr int kRankWaves = kRankT / 64;
+
+__device__ __forceinline__ int slot_class(const uint64_t *__restrict__ pts, int64_t wtot, int y, int yn, int64_t a) {
+    const bool f1 = pt_flags(pts[int64_t(y) * wtot + a]) & HAS_FLUID;
+    const bool f2 = pt_flags(pts[int64_t(yn) * wtot + a]) & HAS_FLUID;
+    return f1 && f2 ? 0 : f1 ? 1 : f2 ? 2 : 3;
+}
+
+__global__ __launch_bounds__(kRankT) void k_fluid_ranks(const uint64_t *__restrict__ pts, int h, int64_t wtot,
+                                                        const int64_t *__restrict__ off,
+                                                        const int32_t *__restrict__ cw, int nchains, uint64_t seed,
+                                                        int32_t *__restrict__ rank) {
+    const int c = int(blockIdx.x % uint32_t(nchains)), y = int(blockIdx.x / uint32_t(nchains));
+    const int yn = (y + 1) % h, w = cw[c];
+    const int64_t o = off[c];
+    if (w <= 0) return;
+    const uint64_t key = mix64(seed ^ (uint64_t(uint32_t(c)) << 32) ^ uint64_t(uint32_t(y)));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ int base[4];
+    __shared__ int wcnt[kRankWaves][4];
+    if (threadIdx.x < 4) base[threadIdx.x] = 0;
+    __syncthreads();
+    // pass 1: slots per class
+    int cnt[4] = {0, 0, 0, 0};
+    for (int p = threadIdx.x; p < w; p += kRankT) cnt[slot_class(pts, wtot, y, yn, o + feistel_perm(uint32_t(p), uint32_t(w), key))]++;
+    for (int k = 0; k < 4; ++k) {
+        int v = cnt[k];
+        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+        if (lane == 0) atomicAdd(&base[k], v);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive scan of the class totals
+        int run = 0;
+        for (int k = 0; k < 4; ++k) {
+            const int t = base[k];
+            base[k] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+    // pass 2: ranks in permutation order
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int start = 0; start < w; start += kRankT) {
+        const int p = start + threadIdx.x;
+        const bool valid = p < w;
+        const uint32_t x = valid ? feistel_perm(uint32_t(p), uint32_t(w), key) : 0u;
+        const int cls = valid ? slot_class(pts, wtot, y, yn, o + x) : 4;
+        int mine = 0;
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t m = __ballot(cls == k);
+            if (cls == k) mine = __popcll(m & below);
+            if (lane == 0) wcnt[wave][k] = __popcll(m);
+        }
+        __syncthreads();
+        if (valid) {
+            int r = base[cls] + mine;
+            for (int v = 0; v < wave; ++v) r += wcnt[v][cls];
+            rank[int64_t(y) * wtot + o + x] = r;
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            int add = 0;
+            for (int v = 0; v < kRankWaves; ++v) add += wcnt[v][threadIdx.x];
+            base[threadIdx.x] += add;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+extern "C" int amh_fluid_ranks(const uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
+                               const int32_t *chain_w_dev, int32_t nchains, uint64_t seed, int32_t *rank_dev,
+                               void *stream) {
+    if (!pts_dev || !chain_off_dev || !chain_w_dev || !rank_dev || nkeys <= 0 || nchains <= 0 || wtot < 0 ||
+        int64_t(nkeys) * nchains > INT32_MAX) {
+        set_error("amh_fluid_ranks: invalid argument");
+        return AMH_E_ARG;
+    }
+    if (wtot == 0) return AMH_OK;
+    hipLaunchKernelGGL(k_fluid_ranks, dim3(uint32_t(nkeys) * uint32_t(nchains)), dim3(kRankT), 0,
+                       static_cast<hipStream_t>(stream), pts_dev, int(nkeys), wtot, chain_off_dev, chain_w_dev,
+                       int(nchains), seed, rank_dev);
+    AMH_LAUNCH_CHECK("amh_fluid_ranks");
+    return AMH_OK;
+}
 
 extern "C" int amh_fluid_reset(void *fluid, double sparseness, void *stream) {
     amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);

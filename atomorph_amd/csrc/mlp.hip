@@ -260,7 +260,18 @@ __global__ __launch_bounds__(256) void k_fused_stage(const uint16_t *__restrict_
     if (t < kOut) S->b5[t] = p[b_off(4) + t];
 }
 
-constexpr int kFusedThreads = 1024;  // 16 waves share one copy of the weights in LDS
+constexpr int kFusedThreads = 512;  // 8 waves (2 per SIMD, 256 registers each) share one copy of the weights
+
+// Bias + GELU of one 32-feature output block (accumulator layout: feature
+// 8*(r/4) + 4*h + r%4 of block mb) -> the two B operands it feeds.
+__device__ __forceinline__ void gelu_block(const floatx16 &acc, const float *__restrict__ bias, int mb, int h,
+                                           bf16x8 &lo, bf16x8 &hi) {
+    uint16_t v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = to_bf16(gelu(acc[r] + bias[mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)]));
+    lo = pack8(v);
+    hi = pack8(v + 8);
+}
 
 __global__ __launch_bounds__(kFusedThreads) void k_mlp_fused(const float *__restrict__ x, int64_t n,
                                                   const FusedLds *__restrict__ image, float *__restrict__ out) {
@@ -279,33 +290,49 @@ __global__ __launch_bounds__(kFusedThreads) void k_mlp_fused(const float *__rest
     for (int64_t tile = int64_t(blockIdx.x) * kWaves + wave; tile < tiles; tile += int64_t(gridDim.x) * kWaves) {
         const int64_t row = tile * 32 + m;
         uint16_t xin[8];
+#pragma unroll
         for (int j = 0; j < 8; ++j) xin[j] = (h == 0 && row < n) ? to_bf16(x[row * kIn + j]) : uint16_t(0);
-        floatx16 acc[4];
-        for (int mb = 0; mb < 4; ++mb) {
-            for (int r = 0; r < 16; ++r) acc[mb][r] = 0.0f;
-            acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.w1[mb][lane], pack8(xin), acc[mb], 0, 0, 0);
+        const bf16x8 xa = pack8(xin);
+        // Layer 1 (K = 8 in one MFMA), two output blocks at a time: each
+        // pair is finished (bias, GELU) before the next starts, so only two
+        // accumulators are live.
+        bf16x8 B[8], Bn[8];
+#pragma unroll
+        for (int mp = 0; mp < 2; ++mp) {
+            floatx16 a0, a1;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) a0[r] = a1[r] = 0.0f;
+            a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.w1[2 * mp][lane], xa, a0, 0, 0, 0);
+            a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.w1[2 * mp + 1][lane], xa, a1, 0, 0, 0);
+            gelu_block(a0, S.b[0], 2 * mp, h, B[4 * mp], B[4 * mp + 1]);
+            gelu_block(a1, S.b[0], 2 * mp + 1, h, B[4 * mp + 2], B[4 * mp + 3]);
+            __builtin_amdgcn_sched_barrier(0);  // keep the pairs apart: bounded register use
         }
-        bf16x8 B[8];
-        for (int li = 0; li < 4; ++li) {
-            // bias + GELU of layer li's outputs -> B operands of the next layer
-            for (int mb = 0; mb < 4; ++mb) {
-                uint16_t v[16];
-                for (int r = 0; r < 16; ++r) {
-                    const int f = mb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-                    v[r] = to_bf16(gelu(acc[mb][r] + S.b[li][f]));
+        // Hidden layers 2-4: B (this layer's input) -> Bn, then swap.
+#pragma unroll 1
+        for (int li = 0; li < 3; ++li) {
+#pragma unroll
+            for (int mp = 0; mp < 2; ++mp) {
+                floatx16 a0, a1;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) a0[r] = a1[r] = 0.0f;
+#pragma unroll
+                for (int ks = 0; ks < 8; ++ks) {
+                    a0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.wh[li][2 * mp][ks][lane], B[ks], a0, 0, 0, 0);
+                    a1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.wh[li][2 * mp + 1][ks][lane], B[ks], a1, 0, 0, 0);
                 }
-                B[2 * mb] = pack8(v);
-                B[2 * mb + 1] = pack8(v + 8);
+                __builtin_amdgcn_sched_barrier(0);
+                gelu_block(a0, S.b[li + 1], 2 * mp, h, Bn[4 * mp], Bn[4 * mp + 1]);
+                gelu_block(a1, S.b[li + 1], 2 * mp + 1, h, Bn[4 * mp + 2], Bn[4 * mp + 3]);
+                __builtin_amdgcn_sched_barrier(0);
             }
-            if (li == 3) break;
-            for (int mb = 0; mb < 4; ++mb) {
-                for (int r = 0; r < 16; ++r) acc[mb][r] = 0.0f;
-                for (int ks = 0; ks < 8; ++ks)
-                    acc[mb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.wh[li][mb][ks][lane], B[ks], acc[mb], 0, 0, 0);
-            }
+#pragma unroll
+            for (int ks = 0; ks < 8; ++ks) B[ks] = Bn[ks];
         }
         floatx16 o;
+#pragma unroll
         for (int r = 0; r < 16; ++r) o[r] = 0.0f;
+#pragma unroll
         for (int ks = 0; ks < 8; ++ks) o = __builtin_amdgcn_mfma_f32_32x32x16_bf16(S.w5[ks][lane], B[ks], o, 0, 0, 0);
         if (h == 0 && row < n) {  // rows 0 and 1 of the output block: registers 0 and 1
             out[row * kOut + 0] = o[0] + S.b5[0];

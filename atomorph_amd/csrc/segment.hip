@@ -207,58 +207,106 @@ __global__ __launch_bounds__(256) void k_root_ids(const int *__restrict__ labels
     }
 }
 
-__global__ __launch_bounds__(256) void k_blob_stats(const int *__restrict__ labels, const uint32_t *__restrict__ stored,
-                                                    int W, int64_t n, const int *__restrict__ root_id,
-                                                    int *__restrict__ bop, amh_blob_stat *__restrict__ st) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool fg = i < n && labels[i] >= 0;
-    int b = -1;
-    if (i < n) {
-        if (fg) b = root_id[labels[i]];
-        bop[i] = b;
+// Per-blob sums in two levels: each block covers kStatChunk consecutive
+// pixels (a few rows), folds wave-uniform runs with shuffles and everything
+// else with LDS atomics into a small open-addressed table of the blobs it
+// meets, then flushes one set of global atomics per (block, blob).  Device
+// atomics on this GPU are performed past the L2, so the global traffic is
+// kept to the table flushes.
+constexpr int kStatThreads = 256;
+constexpr int kStatChunk = 4096;
+constexpr int kStatSlots = 256;
+
+struct StatTable {
+    int key[kStatSlots];
+    unsigned long long sum[kStatSlots][7];  // count, x, y, c0..c3
+    int box[kStatSlots][4];                 // x1 y1 x2 y2
+};
+
+// Slot of blob b in the table (inserting it), or -1 when the table is full.
+__device__ __forceinline__ int stat_slot(StatTable &T, int b) {
+    int h = int((uint32_t(b) * 2654435761u) >> 24);  // 8 bits: [0, 256)
+    for (int probe = 0; probe < kStatSlots; ++probe) {
+        const int k = atomicCAS(&T.key[h], -1, b);
+        if (k == -1 || k == b) return h;
+        h = (h + 1) & (kStatSlots - 1);
     }
-    const int x = fg ? int(i % W) : 0, y = fg ? int(i / W) : 0;
-    const uint32_t c = fg ? stored[i] : 0u;
-    const int b0 = __shfl(b, 0, 64);
-    const bool uniform = __all(b == b0) && b0 >= 0;
-    if (uniform) {
-        // whole wave in one blob: reduce first
-        int64_t s[7] = {1, x, y, int64_t(c & 255u), int64_t((c >> 8) & 255u), int64_t((c >> 16) & 255u),
-                        int64_t(c >> 24)};
-        int mnx = x, mny = y, mxx = x, mxy = y;
-        for (int off = 32; off > 0; off >>= 1) {
-            for (int k = 0; k < 7; ++k) s[k] += __shfl_down(s[k], off, 64);
-            mnx = min(mnx, __shfl_down(mnx, off, 64));
-            mny = min(mny, __shfl_down(mny, off, 64));
-            mxx = max(mxx, __shfl_down(mxx, off, 64));
-            mxy = max(mxy, __shfl_down(mxy, off, 64));
-        }
-        if ((threadIdx.x & 63) == 0) {
-            amh_blob_stat *p = st + b0;
-            atomicAdd((unsigned long long *)&p->count, (unsigned long long)s[0]);
-            atomicAdd((unsigned long long *)&p->sum_x, (unsigned long long)s[1]);
-            atomicAdd((unsigned long long *)&p->sum_y, (unsigned long long)s[2]);
-            for (int k = 0; k < 4; ++k) atomicAdd((unsigned long long *)&p->sum_c[k], (unsigned long long)s[3 + k]);
-            atomicMin(&p->x1, mnx);
-            atomicMin(&p->y1, mny);
-            atomicMax(&p->x2, mxx);
-            atomicMax(&p->y2, mxy);
-        }
-    } else if (fg) {
-        amh_blob_stat *p = st + b;
-        atomicAdd((unsigned long long *)&p->count, 1ull);
-        atomicAdd((unsigned long long *)&p->sum_x, (unsigned long long)x);
-        atomicAdd((unsigned long long *)&p->sum_y, (unsigned long long)y);
-        atomicAdd((unsigned long long *)&p->sum_c[0], (unsigned long long)(c & 255u));
-        atomicAdd((unsigned long long *)&p->sum_c[1], (unsigned long long)((c >> 8) & 255u));
-        atomicAdd((unsigned long long *)&p->sum_c[2], (unsigned long long)((c >> 16) & 255u));
-        atomicAdd((unsigned long long *)&p->sum_c[3], (unsigned long long)(c >> 24));
-        atomicMin(&p->x1, x);
-        atomicMin(&p->y1, y);
-        atomicMax(&p->x2, x);
-        atomicMax(&p->y2, y);
+    return -1;
+}
+
+__device__ __forceinline__ void stat_add_global(amh_blob_stat *p, const unsigned long long *v, int x1, int y1,
+                                                int x2, int y2) {
+    atomicAdd((unsigned long long *)&p->count, v[0]);
+    atomicAdd((unsigned long long *)&p->sum_x, v[1]);
+    atomicAdd((unsigned long long *)&p->sum_y, v[2]);
+    for (int k = 0; k < 4; ++k) atomicAdd((unsigned long long *)&p->sum_c[k], v[3 + k]);
+    atomicMin(&p->x1, x1);
+    atomicMin(&p->y1, y1);
+    atomicMax(&p->x2, x2);
+    atomicMax(&p->y2, y2);
+}
+
+__device__ __forceinline__ void stat_add(StatTable &T, amh_blob_stat *st, int b, const unsigned long long *v, int x1,
+                                         int y1, int x2, int y2) {
+    const int h = stat_slot(T, b);
+    if (h < 0) {  // more blobs in one chunk than slots: straight to memory
+        stat_add_global(st + b, v, x1, y1, x2, y2);
+        return;
     }
-    if (fg && labels[i] == int(i)) st[b].first = i;
+    for (int k = 0; k < 7; ++k) atomicAdd(&T.sum[h][k], v[k]);
+    atomicMin(&T.box[h][0], x1);
+    atomicMin(&T.box[h][1], y1);
+    atomicMax(&T.box[h][2], x2);
+    atomicMax(&T.box[h][3], y2);
+}
+
+__global__ __launch_bounds__(kStatThreads) void k_blob_stats(const int *__restrict__ labels,
+                                                             const uint32_t *__restrict__ stored, int W, int64_t n,
+                                                             const int *__restrict__ root_id, int *__restrict__ bop,
+                                                             amh_blob_stat *__restrict__ st) {
+    __shared__ StatTable T;
+    for (int h = threadIdx.x; h < kStatSlots; h += kStatThreads) {
+        T.key[h] = -1;
+        for (int k = 0; k < 7; ++k) T.sum[h][k] = 0ull;
+        T.box[h][0] = T.box[h][1] = 0x7FFFFFFF;
+        T.box[h][2] = T.box[h][3] = -1;
+    }
+    __syncthreads();
+    const int64_t base = int64_t(blockIdx.x) * kStatChunk;
+    for (int l = threadIdx.x; l < kStatChunk; l += kStatThreads) {
+        const int64_t i = base + l;
+        const bool fg = i < n && labels[i] >= 0;
+        int b = -1;
+        if (i < n) {
+            if (fg) b = root_id[labels[i]];
+            bop[i] = b;
+        }
+        const int x = fg ? int(i % W) : 0, y = fg ? int(i / W) : 0;
+        const uint32_t c = fg ? stored[i] : 0u;
+        const int b0 = __shfl(b, 0, 64);
+        const bool uniform = __all(b == b0) && b0 >= 0;
+        unsigned long long v[7] = {1ull, (unsigned long long)x, (unsigned long long)y, c & 255u, (c >> 8) & 255u,
+                                   (c >> 16) & 255u, c >> 24};
+        if (uniform) {  // whole wave in one blob: reduce first
+            int mnx = x, mny = y, mxx = x, mxy = y;
+            for (int off = 32; off > 0; off >>= 1) {
+                for (int k = 0; k < 7; ++k) v[k] += __shfl_down(v[k], off, 64);
+                mnx = min(mnx, __shfl_down(mnx, off, 64));
+                mny = min(mny, __shfl_down(mny, off, 64));
+                mxx = max(mxx, __shfl_down(mxx, off, 64));
+                mxy = max(mxy, __shfl_down(mxy, off, 64));
+            }
+            if ((threadIdx.x & 63) == 0) stat_add(T, st, b0, v, mnx, mny, mxx, mxy);
+        } else if (fg) {
+            stat_add(T, st, b, v, x, y, x, y);
+        }
+        if (fg && labels[i] == int(i)) st[b].first = i;
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < kStatSlots; h += kStatThreads) {
+        const int b = T.key[h];
+        if (b >= 0) stat_add_global(st + b, T.sum[h], T.box[h][0], T.box[h][1], T.box[h][2], T.box[h][3]);
+    }
 }
 
 __global__ void k_stats_init(amh_blob_stat *st, int nb) {
@@ -503,8 +551,8 @@ extern "C" int amh_blob_stats(const int32_t *labels, const uint32_t *stored, int
         hipLaunchKernelGGL(k_stats_init, dim3((nblobs + 255) / 256), dim3(256), 0, s, st, int(nblobs));
     {
         amh::ProfScope ps_("k_blob_stats", s);
-        hipLaunchKernelGGL(k_blob_stats, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, s, labels, stored, int(W), n,
-                           static_cast<const int *>(ws), bop, st);
+        hipLaunchKernelGGL(k_blob_stats, dim3(uint32_t((n + kStatChunk - 1) / kStatChunk)), dim3(kStatThreads), 0, s,
+                           labels, stored, int(W), n, static_cast<const int *>(ws), bop, st);
     }
     AMH_LAUNCH_CHECK("amh_blob_stats");
     return AMH_OK;

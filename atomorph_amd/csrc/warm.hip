@@ -100,39 +100,43 @@ __global__ __launch_bounds__(kT) void k_features(amh_render R, int y, const RowS
     }
 }
 
-// Raster key of a position at 1/16 px: y (20 bits) then x (20 bits), below
-// the chain index.
-__device__ __forceinline__ uint64_t raster_key(uint64_t chain, double x, double y) {
-    const double lim = double((1 << 20) - 1);
+// Raster key of a position at 1/16 px: set bit, chain, y, x, with cb chain
+// bits and cx bits per coordinate (positions clamped to the frame's range).
+__device__ __forceinline__ uint64_t raster_key(uint64_t set, uint64_t chain, double x, double y, int cb, int cx) {
+    const double lim = double((1ull << cx) - 1);
     const uint64_t fx = uint64_t(fmin(fmax(rint(x * 16.0), 0.0), lim));
     const uint64_t fy = uint64_t(fmin(fmax(rint(y * 16.0), 0.0), lim));
-    return (chain << 40) | (fy << 20) | fx;
+    return (set << (cb + 2 * cx)) | (chain << (2 * cx)) | (fy << cx) | fx;
 }
 
-// Sort keys of the predictions for row y+1 and of the points row y+1 holds
-// now; values = slots.
+// One sort for both lists: keys [0, n) are the predictions for row y+1
+// (set 0), keys [n, 2n) the points row y+1 holds now (set 1); values are
+// the slots.  Sorted, set 0 and set 1 each list chain by chain (each chain
+// has its width in both), so rank r of chain c sits at the same offset in
+// both halves.
 __global__ __launch_bounds__(kT) void k_match_keys(amh_render R, int yn, const RowStat *__restrict__ st,
-                                                  const float *__restrict__ pred, uint64_t *__restrict__ kp,
-                                                  uint32_t *__restrict__ ip, uint64_t *__restrict__ kq,
-                                                  uint32_t *__restrict__ iq) {
+                                                  const float *__restrict__ pred, int cb, int cx,
+                                                  uint64_t *__restrict__ key, uint32_t *__restrict__ val) {
     const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
     if (a >= R.wtot) return;
     const int c = find_chain(R.chain_off_dev, R.nchains, a);
     const RowStat s = st[int64_t(yn) * R.nchains + c];
     const double ux = s.cx + s.s * double(pred[a * 2 + 0]), uy = s.cy + s.s * double(pred[a * 2 + 1]);
     const uint64_t q = R.pts_dev[int64_t(yn) * R.wtot + a];
-    kp[a] = raster_key(uint64_t(c), ux, uy);
-    kq[a] = (uint64_t(c) << 40) | (uint64_t(pt_fy(q) >> 4) << 20) | uint64_t(pt_fx(q) >> 4);
-    ip[a] = uint32_t(a);
-    iq[a] = uint32_t(a);
+    key[a] = raster_key(0, uint64_t(c), ux, uy, cb, cx);
+    const uint64_t lim = (1ull << cx) - 1;
+    key[R.wtot + a] = (1ull << (cb + 2 * cx)) | (uint64_t(c) << (2 * cx)) |
+                      (min(uint64_t(pt_fy(q) >> 4), lim) << cx) | min(uint64_t(pt_fx(q) >> 4), lim);
+    val[a] = uint32_t(a);
+    val[R.wtot + a] = uint32_t(a);
 }
 
-// Rank r of both sorted lists pairs up inside one chain: the slot with the
+// Rank r of both sorted halves pairs up inside one chain: the slot with the
 // r-th prediction takes the r-th point.
-__global__ void k_match_apply(int64_t n, const uint32_t *__restrict__ ip, const uint32_t *__restrict__ iq,
-                              const uint64_t *__restrict__ row, uint64_t *__restrict__ out) {
+__global__ void k_match_apply(int64_t n, const uint32_t *__restrict__ sorted, const uint64_t *__restrict__ row,
+                              uint64_t *__restrict__ out) {
     const int64_t r = int64_t(blockIdx.x) * kT + threadIdx.x;
-    if (r < n) out[ip[r]] = row[iq[r]];
+    if (r < n) out[sorted[r]] = row[sorted[n + r]];
 }
 
 // ------------------------------------------------------- radix sort
@@ -196,7 +200,7 @@ __global__ __launch_bounds__(kT) void k_rs_scatter(const uint64_t *__restrict__ 
 }
 
 struct WarmLayout {
-    size_t stats, x, pred, kp, ip, kq, iq, k2, i2, hist, hoff, sums, offs, row, total;
+    size_t stats, x, pred, kk, kv, k2, i2, hist, hoff, sums, offs, row, total;
     int nblk;
 };
 
@@ -205,18 +209,16 @@ WarmLayout warm_layout(int64_t wtot, int32_t nchains, int32_t nkeys) {
     size_t o = 0;
     auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
     const size_t n = size_t(wtot);
-    L.nblk = int((wtot + kRsTile - 1) / kRsTile);
+    L.nblk = int((2 * wtot + kRsTile - 1) / kRsTile);
     const size_t hn = size_t(256) * size_t(std::max(L.nblk, 1));
     const size_t sb = size_t(scan::blocks(int64_t(hn)));
     L.stats = o; o = al(o + sizeof(RowStat) * size_t(nkeys) * size_t(std::max(nchains, 1)));
     L.x = o; o = al(o + n * 8 * 4);
     L.pred = o; o = al(o + n * 2 * 4);
-    L.kp = o; o = al(o + n * 8);
-    L.ip = o; o = al(o + n * 4);
-    L.kq = o; o = al(o + n * 8);
-    L.iq = o; o = al(o + n * 4);
-    L.k2 = o; o = al(o + n * 8);
-    L.i2 = o; o = al(o + n * 4);
+    L.kk = o; o = al(o + 2 * n * 8);
+    L.kv = o; o = al(o + 2 * n * 4);
+    L.k2 = o; o = al(o + 2 * n * 8);
+    L.i2 = o; o = al(o + 2 * n * 4);
     L.hist = o; o = al(o + hn * 4);
     L.hoff = o; o = al(o + (hn + 1) * 4);
     L.sums = o; o = al(o + sb * 4);
@@ -226,10 +228,10 @@ WarmLayout warm_layout(int64_t wtot, int32_t nchains, int32_t nkeys) {
     return L;
 }
 
-// Sorts (k, v) by the low `bits` bits of the keys; ping-pongs through
-// (k2, v2) and leaves the result in (k, v) (an even number of passes).
-void radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uint32_t *v2, int64_t n, int bits, char *base,
-                const WarmLayout &L, hipStream_t s) {
+// Sorts (k, v) by the low `bits` bits of the keys, ping-ponging through
+// (k2, v2); returns the values array that holds the result.
+uint32_t *radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uint32_t *v2, int64_t n, int bits, char *base,
+                     const WarmLayout &L, hipStream_t s) {
     int *hist = reinterpret_cast<int *>(base + L.hist);
     int *hoff = reinterpret_cast<int *>(base + L.hoff);
     int *sums = reinterpret_cast<int *>(base + L.sums);
@@ -243,6 +245,7 @@ void radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uint32_t *v2, int64_t n,
         std::swap(k, k2);
         std::swap(v, v2);
     }
+    return v;
 }
 
 }  // namespace
@@ -278,7 +281,7 @@ extern "C" int amh_learn_features(const amh_render *R, int32_t y, float *x_dev, 
 extern "C" int amh_warm_start(void *mlp, const amh_render *R, uint64_t *pts_dev, void *ws, size_t ws_bytes,
                               void *stream) {
     if (!mlp || !R || !pts_dev || R->pts_dev != pts_dev || !ws || R->wtot <= 0 || R->nchains <= 0 ||
-        R->nchains > 65535 || R->wtot > int64_t(0x7FFFFFFF) || !R->keys_dev || !R->chain_off_dev ||
+        R->nchains > 65535 || R->wtot > int64_t(0x3FFFFFFF) || !R->keys_dev || !R->chain_off_dev ||
         !R->chain_w_dev) {
         set_error("amh_warm_start: invalid argument");
         return AMH_E_ARG;
@@ -293,26 +296,25 @@ extern "C" int amh_warm_start(void *mlp, const amh_render *R, uint64_t *pts_dev,
     char *b = static_cast<char *>(ws);
     RowStat *st = reinterpret_cast<RowStat *>(b + L.stats);
     float *x = reinterpret_cast<float *>(b + L.x), *pred = reinterpret_cast<float *>(b + L.pred);
-    uint64_t *kp = reinterpret_cast<uint64_t *>(b + L.kp), *kq = reinterpret_cast<uint64_t *>(b + L.kq);
-    uint64_t *k2 = reinterpret_cast<uint64_t *>(b + L.k2);
-    uint32_t *ip = reinterpret_cast<uint32_t *>(b + L.ip), *iq = reinterpret_cast<uint32_t *>(b + L.iq);
-    uint32_t *i2 = reinterpret_cast<uint32_t *>(b + L.i2);
+    uint64_t *kk = reinterpret_cast<uint64_t *>(b + L.kk), *k2 = reinterpret_cast<uint64_t *>(b + L.k2);
+    uint32_t *kv = reinterpret_cast<uint32_t *>(b + L.kv), *i2 = reinterpret_cast<uint32_t *>(b + L.i2);
     uint64_t *row = reinterpret_cast<uint64_t *>(b + L.row);
     const uint32_t ga = uint32_t((R->wtot + kT - 1) / kT);
     hipLaunchKernelGGL(k_row_stats, dim3(uint32_t(R->nchains), uint32_t(R->nkeys)), dim3(kT), 0, s, *R, st);
-    int chain_bits = 0;
-    while ((1 << chain_bits) < R->nchains) ++chain_bits;
-    const int bits = ((40 + chain_bits + 15) / 16) * 16;  // whole byte passes, an even count
+    // key bits: set (1), chain, and y, x at 1/16 px over the frame
+    int cb = 0, cx = 0;
+    while ((1 << cb) < R->nchains) ++cb;
+    while ((int64_t(1) << cx) < int64_t(std::max(std::max(R->width, R->height), 1)) * 16) ++cx;
+    const int bits = 1 + cb + 2 * cx;
     for (int y = 0; y + 1 < R->nkeys; ++y) {
         hipLaunchKernelGGL(k_features, dim3(ga), dim3(kT), 0, s, *R, y, st, x, static_cast<float *>(nullptr));
         if (int r = amh_mlp_forward(mlp, x, R->wtot, pred, stream)) return r;
-        hipLaunchKernelGGL(k_match_keys, dim3(ga), dim3(kT), 0, s, *R, y + 1, st, pred, kp, ip, kq, iq);
-        radix_sort(kp, ip, k2, i2, R->wtot, bits, b, L, s);
-        radix_sort(kq, iq, k2, i2, R->wtot, bits, b, L, s);
+        hipLaunchKernelGGL(k_match_keys, dim3(ga), dim3(kT), 0, s, *R, y + 1, st, pred, cb, cx, kk, kv);
+        const uint32_t *sorted = radix_sort(kk, kv, k2, i2, 2 * R->wtot, bits, b, L, s);
         uint64_t *dst = pts_dev + int64_t(y + 1) * R->wtot;
         if (int r = check_hip(hipMemcpyAsync(row, dst, size_t(R->wtot) * 8, hipMemcpyDeviceToDevice, s), "warm"))
             return r;
-        hipLaunchKernelGGL(k_match_apply, dim3(ga), dim3(kT), 0, s, R->wtot, ip, iq, row, dst);
+        hipLaunchKernelGGL(k_match_apply, dim3(ga), dim3(kT), 0, s, R->wtot, sorted, row, dst);
     }
     AMH_LAUNCH_CHECK("amh_warm_start");
     return AMH_OK;

@@ -150,6 +150,7 @@ struct BlobInfo {  // one blob of a key frame after detection or padding
     size_t size = 0;
     double x = 0, y = 0, r = 0, g = 0, b = 0, a = 0;
     int first = -1;    // smallest pixel index (canvas)
+    pixel px{};        // the blob as one pixel (blob_distance), set by match_init
 };
 
 inline double mono_seconds() {
@@ -378,30 +379,11 @@ bool morph::engine::prepare_fluid() {
         fluid_sp = sp;
     }
     if (fluid_inputs) return true;
-    std::vector<uint64_t> pts(size_t(wtot) * h);
-    if (amh_memcpy_d2h(pts.data(), d_snap.p, 8 * pts.size(), rstream) || amh_stream_sync(rstream))
-        return fail("fluid chains");
-    std::vector<int32_t> rk(size_t(h) * wtot);
+    // per key segment and chain: blob size and its AVERAGE / DISTINCT colour
     std::vector<int32_t> surf(size_t(h) * nch);
     std::vector<double> bcol(size_t(h) * nch * 4, 0.0);
-    std::vector<std::pair<uint64_t, int32_t>> keyx;
     for (int y = 0; y < h; ++y) {
-        const int yn = (y + 1) % h;
         for (size_t c = 0; c < nch; ++c) {
-            const int64_t o = off[c];
-            keyx.clear();
-            for (int32_t x = 0; x < cw[c]; ++x) {
-                const uint64_t p1 = pts[size_t(y) * wtot + o + x], p2 = pts[size_t(yn) * wtot + o + x];
-                const bool f1 = (p1 >> 48) & HAS_FLUID, f2 = (p2 >> 48) & HAS_FLUID;
-                const uint64_t cls = f1 && f2 ? 0 : f1 ? 1 : f2 ? 2 : 3;
-                uint64_t hsh = (uint64_t(c) << 40) ^ (uint64_t(y) << 32) ^ uint64_t(x);
-                hsh += 0x9E3779B97F4A7C15ull;
-                hsh = (hsh ^ (hsh >> 30)) * 0xBF58476D1CE4E5B9ull;
-                hsh = (hsh ^ (hsh >> 27)) * 0x94D049BB133111EBull;
-                keyx.emplace_back((cls << 62) | ((hsh ^ (hsh >> 31)) >> 2), x);
-            }
-            std::sort(keyx.begin(), keyx.end());
-            for (size_t i = 0; i < keyx.size(); ++i) rk[size_t(y) * wtot + o + keyx[i].second] = int32_t(i);
             const BlobInfo &bi = fb[y][best_map[c * h + y]];
             surf[size_t(y) * nch + c] = int32_t(bi.size);
             double *q = &bcol[(size_t(y) * nch + c) * 4];
@@ -414,10 +396,14 @@ bool morph::engine::prepare_fluid() {
             }
         }
     }
-    if (!d_frank.ensure(4 * rk.size()) || !d_fsurf.ensure(4 * surf.size()) || !d_fbcol.ensure(8 * bcol.size()))
+    if (!d_frank.ensure(4 * size_t(wtot) * h) || !d_fsurf.ensure(4 * surf.size()) ||
+        !d_fbcol.ensure(8 * bcol.size()))
         return fail("device allocation");
-    if (amh_memcpy_h2d(d_frank.p, rk.data(), 4 * rk.size(), rstream) ||
-        amh_memcpy_h2d(d_fsurf.p, surf.data(), 4 * surf.size(), rstream) ||
+    // activation order of the atoms, on the device from the render snapshot
+    if (amh_fluid_ranks(d_snap.as<uint64_t>(), h, wtot, d_off.as<int64_t>(), d_w.as<int32_t>(), int32_t(nch),
+                        mix_seed(), d_frank.as<int32_t>(), rstream))
+        return fail("fluid ranks");
+    if (amh_memcpy_h2d(d_fsurf.p, surf.data(), 4 * surf.size(), rstream) ||
         amh_memcpy_h2d(d_fbcol.p, bcol.data(), 8 * bcol.size(), rstream) || amh_stream_sync(rstream))
         return fail("fluid inputs");
     fluid_inputs = true;
@@ -552,11 +538,9 @@ double morph::engine::blob_distance(const BlobInfo &b1, const BlobInfo &b2) cons
     const size_t sz1 = b1.size, sz2 = b2.size, szs = sz1 + sz2;
     double pix = 0.0, col = 0.0, siz = 0.0;
     if (szs > 0) siz = std::fabs(double(sz1) - double(sz2)) / double(szs);
-    if (sz1 > 0 && sz2 > 0) {
-        pixel p1 = create_pixel(to_u16(b1.x), to_u16(b1.y), create_color(b1.r, b1.g, b1.b, b1.a));
-        pixel p2 = create_pixel(to_u16(b2.x), to_u16(b2.y), create_color(b2.r, b2.g, b2.b, b2.a));
-        pix = std::sqrt(double(pixel_distance(p1, p2)) / bbox_d);
-        col = color_distance(p1.c, p2.c);
+    if (sz1 > 0 && sz2 > 0) {  // the pixels are made once per blob by match_init
+        pix = std::sqrt(double(pixel_distance(b1.px, b2.px)) / bbox_d);
+        col = color_distance(b1.px.c, b2.px.c);
     }
     return w_xy * pix + w_rgba * col + w_size * siz;
 }
@@ -589,6 +573,8 @@ bool morph::engine::match_init() {
             fb[y].push_back(e);
         }
         std::shuffle(fb[y].begin(), fb[y].end(), rng);
+        for (BlobInfo &bi : fb[y])  // create_pixel as thread::blob_distance does per call
+            bi.px = create_pixel(to_u16(bi.x), to_u16(bi.y), create_color(bi.r, bi.g, bi.b, bi.a));
     }
     bmap.assign(bw * h, 0);
     for (size_t c = 0; c < bw; ++c)
@@ -943,13 +929,17 @@ bool morph::engine::snapshot() {
     if (!nch || wtot == 0) return true;
     if (amh_memcpy_d2d(d_snap.p, d_pts.p, 8 * size_t(wtot) * h, cstream)) return fail("snapshot");
     brgba.assign(size_t(h) * bw, 0u);
-    for (int y = 0; y < h; ++y)
+    // group of blob b in frame y = the chain whose column holds it
+    std::vector<size_t> group(bw);
+    for (int y = 0; y < h; ++y) {
+        std::fill(group.begin(), group.end(), size_t(0));
+        for (size_t c = 0; c < nch; ++c) {
+            const int32_t b = rank[size_t(y) * nch + c];
+            if (b >= 0 && size_t(b) < bw) group[size_t(b)] = c;
+        }
         for (size_t b = 0; b < bw; ++b) {
             const BlobInfo &bi = fb[y][b];
-            // group of blob b in frame y = the chain whose column holds it
-            size_t grp = 0;
-            for (size_t c = 0; c < nch; ++c)
-                if (rank[size_t(y) * nch + c] == int32_t(b)) grp = c;
+            const size_t grp = group[b];
             color cc;
             if (wopt.show_blobs == DISTINCT) {  // morph.cpp:1323-1328
                 std::mt19937 gen(static_cast<std::mt19937::result_type>(grp));
@@ -961,6 +951,7 @@ bool morph::engine::snapshot() {
             }
             brgba[size_t(y) * bw + b] = color_word(cc);
         }
+    }
     const size_t nb = std::max<size_t>(bw, 1);
     if (!d_rank.ensure(4 * rank.size()) || !d_brgba.ensure(4 * size_t(h) * nb) || !d_lag.ensure(2048) ||
         !d_slope.ensure(2048))

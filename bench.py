@@ -162,6 +162,14 @@ def cpu_baseline(frames, gpu_attempts, match_steps, fluid_steps):
     r.ref_morph_run(m, n)           # one 2^16-attempt visit per chain
     t_visits = time.perf_counter() - t0
     rate = n * (1 << 16) / t_visits
+    # the reference's racing-thread mode (thread.cpp:1051-1062): every chain
+    # visit runs `threads` workers of 2^16 attempts each on the same chain
+    nthr = max(1, min(16, os.cpu_count() or 1))
+    r.ref_morph_set_option(m, b"threads", float(nthr))
+    t0 = time.perf_counter()
+    r.ref_morph_run(m, n)
+    rate_thr = n * nthr * (1 << 16) / (time.perf_counter() - t0)
+    r.ref_morph_set_option(m, b"threads", 0.0)
     r.ref_morph_next_state(m)
     r.ref_morph_run(m, 1)
     out = np.zeros((H, W, 4), np.uint8)
@@ -172,6 +180,7 @@ def cpu_baseline(frames, gpu_attempts, match_steps, fluid_steps):
     t_frame = (time.perf_counter() - t0) / nr
     r.ref_morph_free(m)
     total = t_detect + t_match_init + gpu_attempts / rate + FRAMES_OUT * t_frame
+    total_thr = t_detect + t_match_init + gpu_attempts / rate_thr + FRAMES_OUT * t_frame
     return {"value": round(FRAMES_OUT / total, 5), "unit": "frames/s", "cores": 1, "kind": "reference",
             "sample": (f"compiled reference on this host, threads=0: full blob detection ({t_detect:.1f} s), "
                        f"{match_steps} matching steps + chain init ({t_match_init:.1f} s), {n} chain visits of 2^16 "
@@ -179,7 +188,11 @@ def cpu_baseline(frames, gpu_attempts, match_steps, fluid_steps):
                        f"{nr} rendered frames ({t_frame:.2f} s each, scaled to {FRAMES_OUT}); job estimate "
                        f"{total:.1f} s"),
             "phases_s": {"detect": round(t_detect, 3), "match_init": round(t_match_init, 3),
-                         "refine": round(gpu_attempts / rate, 3), "render": round(FRAMES_OUT * t_frame, 3)}}
+                         "refine": round(gpu_attempts / rate, 3), "render": round(FRAMES_OUT * t_frame, 3)},
+            "threaded": {"cores": nthr, "value": round(FRAMES_OUT / total_thr, 5),
+                         "refine_attempts_per_s": round(rate_thr, 1),
+                         "note": "refinement in the reference's racing-thread mode; detection, matching and "
+                                 "rendering are single-threaded in the reference"}}
 
 
 def main():
@@ -332,6 +345,17 @@ def main():
     achieved = dom_bytes / dom_s / 1e9
     roof_all = {k: {"bytes_per_launch": int(b), "launch_us": round(t * 1e6, 3), "GBs": round(b / t / 1e9, 1),
                     "total_ms": round(total_ms(k), 3)} for k, (b, t, _) in rooflines.items()}
+    # measured HBM bytes per launch of the dominant kernel: two rocprofv3
+    # counter passes (FETCH_SIZE, WRITE_SIZE) on this workload, each scaled
+    # by what the counter reads for a plain copy of known size; kept as data
+    # beside the profile summaries (null when absent for this workload)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{args.workload}_hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tdata = json.load(f)
+        if tdata.get("workload") == args.workload and dom in tdata.get("bytes_per_launch", {}):
+            traffic = int(tdata["bytes_per_launch"][dom])
     # rendering, for comparison: per atom and frame h points + 2 colours read,
     # per output pixel 4 B written
     render_bytes = FRAMES_OUT * (int(widths.sum()) * (8 * h + 8) + H * W * 4)
@@ -379,7 +403,7 @@ def main():
                    "match_steps": args.match_steps, "refine_passes": passes, "refine_attempts": attempts_job},
         "phases_ms": {k: round(float(np.mean(v)), 3) for k, v in phase.items()},
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_launch": int(dom_bytes), "launch_us": round(dom_s * 1e6, 3)},
         "rooflines": roof_all,
         "render_gbs": round(render_gbs, 1),

@@ -289,6 +289,15 @@ int amh_fluid_get_particles(void *fluid, double *host, void *stream);
 /* One FluidModel::step (fluidmodel.cpp:165-580); the wall bounce jitter
  * (rand() in the reference) comes from `seed`. */
 int amh_fluid_step(void *fluid, int32_t steps_left, double freedom_radius, uint64_t seed, void *stream);
+/* Activation order of the fluid particles (no counterpart in the reference,
+ * whose std::map bookkeeping births and kills particles, morph.cpp:915-1025):
+ * rank_dev[y * wtot + off[c] + x] = place of slot x of chain c among the
+ * chain's slots in key segment y -> y+1, ordered by class (HAS_FLUID in
+ * rows y and y+1: both, only y, only y+1, neither), then by a pseudo-random
+ * permutation of the slots keyed by (seed, c, y).  Every chain's ranks in a
+ * segment are a permutation of [0, width). */
+int amh_fluid_ranks(const uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
+                    const int32_t *chain_w_dev, int32_t nchains, uint64_t seed, int32_t *rank_dev, void *stream);
 typedef struct amh_fluid_frame {
     const amh_render *r;          /* chains, key frames, options, Perlin     */
     const int32_t *rank_dev;      /* [nkeys][wtot]: activation order of each

@@ -206,3 +206,48 @@ def test_engine_fluid_render():
     imgs = [m.render(i / 16) for i in range(16)]
     assert all((im[..., 3] > 0).sum() > 100 for im in imgs)
     assert not (imgs[5] == atoms).all()
+
+
+def test_fluid_activation_ranks():
+    """amh_fluid_ranks: per key segment and chain the ranks are a
+    permutation of the slots, classes first (HAS_FLUID in both rows, only the
+    source, only the destination, neither), and within a class the order of
+    the keyed permutation -- equal to a host replay of the definition."""
+    T = G.torch()
+    lib = G.lib()
+    rng = np.random.default_rng(21)
+    widths = np.array([1, 2, 3, 64, 255, 256, 257, 700, 1500], np.int64)
+    h = 3
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]]).astype(np.int64)
+    wtot = int(widths.sum())
+    flags = rng.integers(0, 4, (h, wtot))
+    pts = op.pack_points(rng.integers(0, 64, (h, wtot)), rng.integers(0, 64, (h, wtot)), 0, 0, flags)
+    seed = 0x1234ABCD
+    # every device buffer stays referenced until the kernel has run: a
+    # temporary freed early hands its memory to the next allocation
+    dev = G.to_dev(pts.ravel())
+    off_d = G.to_dev(off)
+    w_d = G.to_dev(widths.astype(np.int32))
+    rank = G.zeros(h * wtot, T.int32)
+    assert off_d.data_ptr() != w_d.data_ptr()
+    G.log(f"fluid ranks: {len(widths)} chains, wtot {wtot}, {h} key frames")
+    G.check(lib.amh_fluid_ranks(G.dptr(dev), h, wtot, G.dptr(off_d), G.dptr(w_d), len(widths), seed, G.dptr(rank),
+                                G.stream()), "amh_fluid_ranks")
+    got = G.to_host(rank, np.int32).reshape(h, wtot)
+    orc = op.oracle()
+    for y in range(h):
+        yn = (y + 1) % h
+        for c, (o, w) in enumerate(zip(off, widths)):
+            f1 = (flags[y, o:o + w] & 2) > 0
+            f2 = (flags[yn, o:o + w] & 2) > 0
+            cls = np.where(f1 & f2, 0, np.where(f1, 1, np.where(f2, 2, 3)))
+            key = orc.or_hash64((seed ^ (c << 32) ^ y) & ((1 << 64) - 1))
+            base = np.concatenate([[0], np.cumsum(np.bincount(cls, minlength=4))[:-1]])
+            want = np.zeros(w, np.int64)
+            for p in range(w):
+                x = orc.or_perm_index(p, int(w), key)
+                want[x] = base[cls[x]]
+                base[cls[x]] += 1
+            seg = got[y, o:o + w]
+            assert np.array_equal(np.sort(seg), np.arange(w)), (y, c)
+            assert np.array_equal(seg, want), (y, c)
