@@ -12,7 +12,7 @@
 // density, density gradient, strength-weighted colour), pressure and wall
 // forces, node acceleration, G2P with the trajectory attractor and momentum
 // scatter, node velocity, advection with colour diffusion, freedom clamp and
-// walls.  Particle state is FP64; the grid accumulates in FP32 atomics.
+// walls.  Particle state is FP64; the grid is FP32, summed per tile in LDS.
 #include "colour.h"
 #include "common.h"
 #include "../../include/atomorph_hip.h"
@@ -311,11 +311,43 @@ __device__ __forceinline__ int entry_tile(int j, int tx0, int tx1, int ty0, int 
     return (bx <= tx1 && by <= ty1) ? bx * nty + by : -1;
 }
 
-__global__ __launch_bounds__(kT) void k_bin_count(Particles P, int n, int gsx, int gsy, Bins B, int par) {
+// What a particle is binned by: tiles(i, tx0, tx1, ty0, ty1) gives the
+// block tx0..tx1 x ty0..ty1 of tiles it enters ((tx0, ty0) its home) and
+// returns false for a particle that is not binned.
+struct StencilTiles {  // the step: tiles holding the 3x3 stencil nodes
+    Particles P;
+    int gsx, gsy;
+    __device__ __forceinline__ bool operator()(int i, int &tx0, int &tx1, int &ty0, int &ty1) const {
+        if (!P.active[i]) return false;
+        stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
+        return true;
+    }
+};
+
+constexpr int kPxTile = 32;  // pixels per side of a splat tile
+
+constexpr uint32_t kNoTap = 0xFFFFFFFFu;  // splat: particle draws nothing (X, Y <= 65534)
+
+struct SplatTiles {  // the splat: pixel tiles holding the 2x2 taps at (X, Y)
+    const uint32_t *xy;  // X | Y << 16, kNoTap = nothing to draw
+    int W, H;
+    __device__ __forceinline__ bool operator()(int i, int &tx0, int &tx1, int &ty0, int &ty1) const {
+        const uint32_t v = xy[i];
+        if (v == kNoTap) return false;
+        const int X = int(v & 0xFFFFu), Y = int(v >> 16);
+        tx0 = X / kPxTile;
+        tx1 = min(X + 1, W - 1) / kPxTile;
+        ty0 = Y / kPxTile;
+        ty1 = min(Y + 1, H - 1) / kPxTile;
+        return true;
+    }
+};
+
+template <class Tiles>
+__global__ __launch_bounds__(kT) void k_bin_count(Tiles tiles_of, int n, Bins B, int par) {
     const int i = blockIdx.x * kT + threadIdx.x;
-    const bool act = i < n && P.active[i];
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-    if (act) stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
+    const bool act = i < n && tiles_of(i, tx0, tx1, ty0, ty1);
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     // Group the wave's lanes by bin, per entry: lane lead[j] leads the lanes
@@ -353,8 +385,8 @@ __global__ __launch_bounds__(kT) void k_bin_count(Particles P, int n, int gsx, i
 // kLds every block scans the counts into LDS (block 0 also stores them in
 // B.start); otherwise B.start holds the scan already.  The grid covers
 // max(n, tiles) threads.
-template <bool kLds>
-__global__ __launch_bounds__(kT) void k_bin_place(Particles P, int n, int gsx, int gsy, Bins B, int tiles, int par) {
+template <bool kLds, class Tiles>
+__global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B, int tiles, int par) {
     __shared__ int sstart[kLds ? kLdsTiles + 1 : 1];
     __shared__ int wsum[kT / 64];
     const int32_t *start = B.start;
@@ -386,9 +418,8 @@ __global__ __launch_bounds__(kT) void k_bin_place(Particles P, int n, int gsx, i
     }
     for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) B.count[par ^ 1][k] = 0;
     const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n || !P.active[i]) return;
     int tx0, tx1, ty0, ty1;
-    stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
+    if (i >= n || !tiles_of(i, tx0, tx1, ty0, ty1)) return;
     for (int j = 0; j < 4; ++j) {
         const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
         if (t >= 0) B.list[start[t] + B.rank[int64_t(j) * n + i]] = i | (j == 0 ? kHome : 0);
@@ -867,8 +898,12 @@ struct amh_fluid_s {
     int32_t *scan_sums = nullptr, *scan_offs = nullptr;
     int32_t *cell_rank = nullptr;  // [nn] sort key of each cell (Morton order)
     Bins bins;                     // per-step binning by node tile
+    Bins pbins;                    // the splat's binning by pixel tile (rank, list shared with bins)
+    int32_t pbin_par = 0;
+    uint32_t *sxy = nullptr;       // [n] splat: top-left tap X | Y << 16, kNoTap = not drawn
+    double *staps = nullptr;       // [n][4] splat tap weights
+    uint32_t *scol = nullptr;      // [n] splat colours
     // render buffers
-    double *acc = nullptr;     // [npx][5] sum w, sum w*c
     int32_t *mask = nullptr;   // [npx] blob membership / feather state
     uint32_t *img = nullptr;   // [npx] colours during feather
     int32_t W = 0, H = 0;
@@ -881,10 +916,11 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     const size_t n = size_t(count), nn = size_t(gsx) * gsy, npx = size_t(W) * H;
     const size_t nb = size_t(scan::blocks(int64_t(nn) + 1));
     const size_t tiles = size_t((gsx + kTile - 1) / kTile) * size_t((gsy + kTile - 1) / kTile);
+    const size_t ptiles = size_t((W + kPxTile - 1) / kPxTile) * size_t((H + kPxTile - 1) / kPxTile);
     (void)nkeys;
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + 2) * 4 * 2 +
-           (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + nn * 4 * kGridFields + npx * 8 * 5 +
-           npx * 4 * 2 + 64 * 256;
+           (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) +
+           nn * 4 * kGridFields + npx * 4 * 2 + 64 * 256;
 }
 
 void carve_particles(Particles &Q, size_t n, const std::function<char *(size_t)> &take) {
@@ -930,11 +966,21 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->bins.start = reinterpret_cast<int32_t *>(take((tiles + 1) * 4));
     f->bins.rank = reinterpret_cast<int32_t *>(take(n * 4 * 4));
     f->bins.list = reinterpret_cast<int32_t *>(take(n * 4 * 4));
+    f->pbins.ntx = (width + kPxTile - 1) / kPxTile;
+    f->pbins.nty = (height + kPxTile - 1) / kPxTile;
+    const size_t ptiles = size_t(f->pbins.ntx) * size_t(f->pbins.nty);
+    f->pbins.count[0] = reinterpret_cast<int32_t *>(take(ptiles * 4));
+    f->pbins.count[1] = reinterpret_cast<int32_t *>(take(ptiles * 4));
+    f->pbins.start = reinterpret_cast<int32_t *>(take((ptiles + 1) * 4));
+    f->pbins.rank = f->bins.rank;  // the splat bins after the last step of a frame
+    f->pbins.list = f->bins.list;
+    f->sxy = reinterpret_cast<uint32_t *>(take(n * 4));
+    f->staps = reinterpret_cast<double *>(take(n * 32));
+    f->scol = reinterpret_cast<uint32_t *>(take(n * 4));
     float **g[kGridFields] = {&f->G.m, &f->G.gx, &f->G.gy, &f->G.u, &f->G.v, &f->G.ax,
                               &f->G.ay, &f->G.r, &f->G.g, &f->G.b, &f->G.a, &f->G.w};
     float *grid = reinterpret_cast<float *>(take(nn * 4 * kGridFields));
     for (int k = 0; k < kGridFields; ++k) *g[k] = grid + size_t(k) * nn;
-    f->acc = reinterpret_cast<double *>(take(npx * 8 * 5));
     f->mask = reinterpret_cast<int32_t *>(take(npx * 4));
     f->img = reinterpret_cast<uint32_t *>(take(npx * 4));
     // cells ranked in Morton (Z) order of (x, y): a run of consecutive ranks
@@ -1028,27 +1074,32 @@ extern "C" int amh_fluid_get_particles(void *fluid, double *host, void *stream) 
 namespace {
 // One FluidModel::step (fluidmodel.cpp:165-580) over the particles in Q
 // (particle order, or the cell-ordered copy with its atom map).
+// Counting sort of the particles' entries into the bins of B (tiles bins,
+// counts of parity par; the other parity is cleared for the next use).
+template <class Tiles>
+void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s) {
+    const uint32_t gp = uint32_t((n + kT - 1) / kT);
+    hipLaunchKernelGGL(k_bin_count<Tiles>, dim3(gp), dim3(kT), 0, s, tiles_of, n, B, par);
+    const uint32_t gb = uint32_t((std::max(n, tiles) + kT - 1) / kT);
+    if (tiles <= kLdsTiles) {
+        hipLaunchKernelGGL((k_bin_place<true, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
+    } else {
+        hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count[par], tiles, B.start);
+        hipLaunchKernelGGL((k_bin_place<false, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
+    }
+}
+
 int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32_t steps_left,
                   double freedom_radius, uint64_t seed, hipStream_t s) {
     amh::ProfScope step_("fluid_step", s);
     if (f->count == 0) return AMH_OK;
     const Bins &B = f->bins;
     const int tiles = B.ntx * B.nty;
-    const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
     {
         amh::ProfScope ps_("fluid_bins", s);
         const int par = f->bin_par;
         f->bin_par ^= 1;
-        hipLaunchKernelGGL(k_bin_count, dim3(gp), dim3(kT), 0, s, Q, f->count, f->gsx, f->gsy, B, par);
-        const uint32_t gb = uint32_t((std::max(f->count, tiles) + kT - 1) / kT);
-        if (tiles <= kLdsTiles) {
-            hipLaunchKernelGGL(k_bin_place<true>, dim3(gb), dim3(kT), 0, s, Q, f->count, f->gsx, f->gsy, B, tiles,
-                               par);
-        } else {
-            hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count[par], tiles, B.start);
-            hipLaunchKernelGGL(k_bin_place<false>, dim3(gb), dim3(kT), 0, s, Q, f->count, f->gsx, f->gsy, B, tiles,
-                               par);
-        }
+        bin_particles(StencilTiles{Q, f->gsx, f->gsy}, f->count, B, tiles, par, s);
     }
     {
         amh::ProfScope ps_("k_p2g", s);
@@ -1186,99 +1237,109 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, Frame F, Part
     P.active[a] = 1;
 }
 
-// draw_fluid, splat (morph.cpp:1080-1167).  Bilinear taps into per-pixel
-// FP64 sums (weight, weight * channel); like the grid scatters, a block sums
-// its taps in an LDS copy of the pixels it touches when they fit.
-constexpr int kSplatPx = 1024;  // 5 FP64 sums: 40 KB of LDS
-
-__global__ __launch_bounds__(kT) void k_fluid_splat(Particles P, int n, double t, int W, int H, double sp, int bx1,
-                                                   int by1, int bx2, int by2, double *acc) {
-    __shared__ double tile[5][kSplatPx];
-    __shared__ int red[4 * (kT / 64)];
+// draw_fluid, splat (morph.cpp:1080-1167), in two passes without atomics
+// on global memory.  k_splat_prep: per particle the colour update, the
+// pixel of its top-left tap and the four bilinear tap weights (0 = not
+// drawn); then the particles are binned by 32 x 32 pixel tile; then
+// k_splat_tile gathers a tile's taps in LDS (FP64 sums: weight, weight *
+// channel) and resolves its pixels (morph.cpp:1169-1209) straight into the
+// image.
+__global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, double t, int W, int H, double sp, int bx1,
+                                                  int by1, int bx2, int by2, uint32_t *__restrict__ xy,
+                                                  double *__restrict__ taps, uint32_t *__restrict__ cols) {
     const int i = blockIdx.x * kT + threadIdx.x;
-    bool valid = i < n && P.active[i];
-    int X = 0, Y = 0;
-    double wt[4] = {0.0, 0.0, 0.0, 0.0};  // taps (X,Y) (X+1,Y) (X,Y+1) (X+1,Y+1)
-    uint32_t col = 0;
-    if (valid) {
-        const double s = P.strength[i];
-        const double w = (1.0 / (100.0 * pow((t - (1.0 * (1.0 - s))), 2.0) + 1.0)) * pow(s, 2.0);
-        const double r = (w * P.R[i]) + (1.0 - w) * P.r[i], g = (w * P.G[i]) + (1.0 - w) * P.g[i];
-        const double b = (w * P.B[i]) + (1.0 - w) * P.b[i], al = (w * P.A[i]) + (1.0 - w) * P.a[i];
-        P.r[i] = r; P.g[i] = g; P.b[i] = b; P.a[i] = al;
-        double px, py;
-        const double x = P.x[i], y = P.y[i];
-        if (x <= 10.0) px = 0.0;
-        else if (x >= W * sp + 10.0) px = (W - 1);
-        else px = (x - 10.0) / sp;
-        if (y <= 10.0) py = 0.0;
-        else if (y >= H * sp + 10.0) py = (H - 1);
-        else py = (y - 10.0) / sp;
-        X = min(int(px), W - 1);
-        Y = min(int(py), H - 1);
-        P.mature[i] = 1;
-        double ix, iy;
-        const double xf = modf(px, &ix), yf = modf(py, &iy);
-        if ((ix >= W || iy >= H) && (ix > bx2 || ix < bx1 || iy > by2 || iy < by1)) valid = false;
-        const double w11 = ((1.0 - xf) * (1.0 - yf)), w21 = (xf * (1.0 - yf));
-        const double w12 = ((1.0 - xf) * yf), w22 = (xf * yf);
-        col = rgba_pack(narrow8(round(r * 255.0)), narrow8(round(g * 255.0)), narrow8(round(b * 255.0)),
-                        narrow8(round(al * 255.0)));
-        // a tap is drawn when its weight is positive, its guard holds and it
-        // lies inside the image
-        if (w11 > 0.0) wt[0] = w11;
-        if ((X < bx2 || X + 1 < W) && w21 > 0.0 && X + 1 < W) wt[1] = w21;
-        if ((Y < by2 || Y + 1 < H) && w12 > 0.0 && Y + 1 < H) wt[2] = w12;
-        if (w22 > 0.0 && ((Y < by2 && X < bx2) || (Y + 1 < H && X + 1 < W)) && X + 1 < W && Y + 1 < H) wt[3] = w22;
-    }
-    // X, Y are in [0, W) x [0, H) here: the box is in pixels, row-major
-    const Box B = block_box<kSplatPx, 2>(valid, X, Y, red);
-    if (!B.fits) {  // block-uniform
-        if (!valid) return;
-        for (int q = 0; q < 4; ++q) {
-            if (wt[q] <= 0.0) continue;
-            double *d = acc + (int64_t(Y + (q >> 1)) * W + X + (q & 1)) * 5;
-            atomicAdd(&d[0], wt[q]);
-            for (int k = 0; k < 4; ++k) atomicAdd(&d[1 + k], double(ch(col, k)) * wt[q]);
-        }
+    if (i >= n) return;
+    if (!P.active[i]) {
+        xy[i] = kNoTap;
         return;
     }
-    const int area = B.w * B.h;  // B.w columns (x) by B.h rows (y)
-    for (int idx = threadIdx.x; idx < area; idx += kT)
-        for (int f = 0; f < 5; ++f) tile[f][idx] = 0.0;
-    __syncthreads();
-    if (valid) {
-        for (int q = 0; q < 4; ++q) {
-            if (wt[q] <= 0.0) continue;
-            const int idx = (Y + (q >> 1) - B.y0) * B.w + (X + (q & 1) - B.x0);
-            atomicAdd(&tile[0][idx], wt[q]);
-            for (int k = 0; k < 4; ++k) atomicAdd(&tile[1 + k][idx], double(ch(col, k)) * wt[q]);
-        }
+    const double s = P.strength[i];
+    const double w = (1.0 / (100.0 * pow((t - (1.0 * (1.0 - s))), 2.0) + 1.0)) * pow(s, 2.0);
+    const double r = (w * P.R[i]) + (1.0 - w) * P.r[i], g = (w * P.G[i]) + (1.0 - w) * P.g[i];
+    const double b = (w * P.B[i]) + (1.0 - w) * P.b[i], al = (w * P.A[i]) + (1.0 - w) * P.a[i];
+    P.r[i] = r; P.g[i] = g; P.b[i] = b; P.a[i] = al;
+    double px, py;
+    const double x = P.x[i], y = P.y[i];
+    if (x <= 10.0) px = 0.0;
+    else if (x >= W * sp + 10.0) px = (W - 1);
+    else px = (x - 10.0) / sp;
+    if (y <= 10.0) py = 0.0;
+    else if (y >= H * sp + 10.0) py = (H - 1);
+    else py = (y - 10.0) / sp;
+    const int X = min(int(px), W - 1), Y = min(int(py), H - 1);
+    P.mature[i] = 1;
+    double ix, iy;
+    const double xf = modf(px, &ix), yf = modf(py, &iy);
+    if ((ix >= W || iy >= H) && (ix > bx2 || ix < bx1 || iy > by2 || iy < by1)) {
+        xy[i] = kNoTap;
+        return;
     }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < area; idx += kT) {
-        if (tile[0][idx] == 0.0) continue;
-        const int qx = B.x0 + idx % B.w, qy = B.y0 + idx / B.w;
-        double *d = acc + (int64_t(qy) * W + qx) * 5;
-        for (int f = 0; f < 5; ++f) atomicAdd(&d[f], tile[f][idx]);
-    }
+    const double w11 = ((1.0 - xf) * (1.0 - yf)), w21 = (xf * (1.0 - yf));
+    const double w12 = ((1.0 - xf) * yf), w22 = (xf * yf);
+    // a tap is drawn when its weight is positive, its guard holds and it lies
+    // inside the image; taps (X,Y) (X+1,Y) (X,Y+1) (X+1,Y+1)
+    double wt[4] = {0.0, 0.0, 0.0, 0.0};
+    if (w11 > 0.0) wt[0] = w11;
+    if ((X < bx2 || X + 1 < W) && w21 > 0.0 && X + 1 < W) wt[1] = w21;
+    if ((Y < by2 || Y + 1 < H) && w12 > 0.0 && Y + 1 < H) wt[2] = w12;
+    if (w22 > 0.0 && ((Y < by2 && X < bx2) || (Y + 1 < H && X + 1 < W)) && X + 1 < W && Y + 1 < H) wt[3] = w22;
+    xy[i] = uint32_t(X) | (uint32_t(Y) << 16);
+    for (int q = 0; q < 4; ++q) taps[int64_t(i) * 4 + q] = wt[q];
+    cols[i] = rgba_pack(narrow8(round(r * 255.0)), narrow8(round(g * 255.0)), narrow8(round(b * 255.0)),
+                        narrow8(round(al * 255.0)));
 }
 
-// resolve (morph.cpp:1169-1209); clears the accumulators
-__global__ void k_fluid_resolve(double *acc, int64_t npx, double t, int keep_bg, uint32_t *img, int32_t *mask) {
-    const int64_t p = int64_t(blockIdx.x) * kT + threadIdx.x;
-    if (p >= npx) return;
-    double *q = acc + p * 5;
-    const double ws = q[0];
-    if (ws > 0.0) {
-        const double r = round(q[1] / ws), g = round(q[2] / ws), b = round(q[3] / ws);
-        double a = round(q[4] / ws);
-        if (keep_bg) a *= 1.0 - pow(t, 24.0);
-        img[p] = rgba_pack(narrow8(r), narrow8(g), narrow8(b), narrow8(a));
-        mask[p] = 1;
-        for (int k = 0; k < 5; ++k) q[k] = 0.0;
-    } else {
-        mask[p] = 0;
+struct SplatItem {
+    uint32_t xy;
+    uint32_t col;
+    double w[4];
+};
+
+__global__ __launch_bounds__(kT) void k_splat_tile(Bins B, const uint32_t *__restrict__ xy,
+                                                  const double *__restrict__ taps, const uint32_t *__restrict__ cols,
+                                                  int W, int H, double t, int keep_bg, uint32_t *__restrict__ img,
+                                                  int32_t *__restrict__ mask) {
+    __shared__ double sums[5][kPxTile * kPxTile];  // 40 KB
+    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
+    const int x0 = tx * kPxTile, y0 = ty * kPxTile;
+    for (int k = threadIdx.x; k < 5 * kPxTile * kPxTile; k += kT) (&sums[0][0])[k] = 0.0;
+    __syncthreads();
+    auto fetch = [&](int slot) {
+        SplatItem it = {kNoTap, 0u, {0.0, 0.0, 0.0, 0.0}};
+        if (slot >= 0) {
+            it.xy = xy[slot];
+            it.col = cols[slot];
+            for (int q = 0; q < 4; ++q) it.w[q] = taps[int64_t(slot) * 4 + q];
+        }
+        return it;
+    };
+    for_bin(B, int(blockIdx.x), fetch, [&](int slot, bool, const SplatItem &it) {
+        if (slot < 0 || it.xy == kNoTap) return;
+        const int X = int(it.xy & 0xFFFFu), Y = int(it.xy >> 16);
+        for (int q = 0; q < 4; ++q) {
+            const double wq = it.w[q];
+            const int lx = X + (q & 1) - x0, ly = Y + (q >> 1) - y0;
+            if (wq <= 0.0 || lx < 0 || lx >= kPxTile || ly < 0 || ly >= kPxTile) continue;
+            const int idx = ly * kPxTile + lx;
+            atomicAdd(&sums[0][idx], wq);
+            for (int k = 0; k < 4; ++k) atomicAdd(&sums[1 + k][idx], double(ch(it.col, k)) * wq);
+        }
+    });
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < kPxTile * kPxTile; idx += kT) {
+        const int px = x0 + idx % kPxTile, py = y0 + idx / kPxTile;
+        if (px >= W || py >= H) continue;
+        const int64_t p = int64_t(py) * W + px;
+        const double ws = sums[0][idx];
+        if (ws > 0.0) {
+            const double r = round(sums[1][idx] / ws), g = round(sums[2][idx] / ws), b = round(sums[3][idx] / ws);
+            double a = round(sums[4][idx] / ws);
+            if (keep_bg) a *= 1.0 - pow(t, 24.0);
+            img[p] = rgba_pack(narrow8(r), narrow8(g), narrow8(b), narrow8(a));
+            mask[p] = 1;
+        } else {
+            mask[p] = 0;
+        }
     }
 }
 
@@ -1494,13 +1555,20 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
         }
         {
             amh::ProfScope ps_("k_fluid_splat", s);
-            hipLaunchKernelGGL(k_fluid_splat, dim3(gp), dim3(kT), 0, s, f->S, f->count, F.tl, f->W, f->H,
-                               fa->sparseness, R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->acc);
+            hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, F.tl, f->W, f->H,
+                               fa->sparseness, R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps,
+                               f->scol);
+            const int ptiles = f->pbins.ntx * f->pbins.nty;
+            const int par = f->pbin_par;
+            f->pbin_par ^= 1;
+            bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s);
+            hipLaunchKernelGGL(k_splat_tile, dim3(uint32_t(ptiles)), dim3(kT), 0, s, f->pbins, f->sxy, f->staps,
+                               f->scol, f->W, f->H, F.tl, R.keep_background, f->img, f->mask);
         }
         hipLaunchKernelGGL(k_sort_scatter, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->P);
+    } else {  // nothing drawn: every pixel empty
+        hipLaunchKernelGGL(k_fill_i32, dim3(gpx), dim3(kT), 0, s, f->mask, npx);
     }
-    hipLaunchKernelGGL(k_fluid_resolve, dim3(gpx), dim3(kT), 0, s, f->acc, npx, F.tl, R.keep_background, f->img,
-                       f->mask);
     for (int layer = 0; layer < R.feather; ++layer) {
         const double alpha = 1.0 - (double(layer + 1) / double(R.feather + 1));
         hipLaunchKernelGGL(k_fluid_feather, dim3(gpx), dim3(kT), 0, s, f->W, f->H, alpha, f->img, f->mask);

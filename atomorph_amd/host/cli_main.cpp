@@ -16,6 +16,9 @@
  *   --learned FILE    learned warm start with the MLP parameters in FILE
  *                     (.npz written by python -m atomorph_amd.learned)
  *   --fluid-sparseness X   fluid grid cells per pixel (reference: 1)
+ *   --load-chains FILE     resume the refinement from a checkpoint (with
+ *                     --match-steps, once the chains are built)
+ *   --save-chains FILE     checkpoint the refined chains before rendering
  */
 #include <getopt.h>
 
@@ -54,6 +57,7 @@ struct Options {  // demo/options.h:27-62 (defaults kept)
     int batched = 0, tile = -1, rounds = 0;
     std::string learned;
     double fluid_sparseness = 0.0;
+    std::string load_chains, save_chains;
     std::vector<std::string> files;
 };
 
@@ -112,10 +116,12 @@ void usage(FILE *s) {
                  "      --tile              INT   Batched kernel variant (default: autotuned).\n"
                  "      --rounds            INT   Batched rounds per pass (default: autotuned).\n"
                  "      --learned           FILE  Learned warm start with these MLP parameters (.npz).\n"
-                 "      --fluid-sparseness  REAL  Fluid grid cells per pixel (default 1).\n");
+                 "      --fluid-sparseness  REAL  Fluid grid cells per pixel (default 1).\n"
+                 "      --load-chains       FILE  Resume the refinement from a chain checkpoint.\n"
+                 "      --save-chains       FILE  Write the refined chains to a checkpoint.\n");
 }
 
-enum { kMatchSteps = 1000, kMorphSteps, kBatched, kLearned, kSparseness, kTile, kRounds };
+enum { kMatchSteps = 1000, kMorphSteps, kBatched, kLearned, kSparseness, kTile, kRounds, kLoadChains, kSaveChains };
 
 bool parse(int argc, char **argv) {  // demo/options.h:126-224
     opt.name = argv[0];
@@ -168,6 +174,8 @@ bool parse(int argc, char **argv) {  // demo/options.h:126-224
         {"fluid-sparseness", required_argument, nullptr, kSparseness},
         {"tile", required_argument, nullptr, kTile},
         {"rounds", required_argument, nullptr, kRounds},
+        {"load-chains", required_argument, nullptr, kLoadChains},
+        {"save-chains", required_argument, nullptr, kSaveChains},
         {nullptr, 0, nullptr, 0}};
     for (;;) {
         int index = 0;
@@ -209,6 +217,8 @@ bool parse(int argc, char **argv) {  // demo/options.h:126-224
             case kSparseness: opt.fluid_sparseness = std::atof(optarg); break;
             case kTile: opt.tile = std::atoi(optarg); break;
             case kRounds: opt.rounds = std::max(0, std::atoi(optarg)); break;
+            case kLoadChains: opt.load_chains = optarg; break;
+            case kSaveChains: opt.save_chains = optarg; break;
             case '?': break;  // getopt_long printed the message
             default: return false;
         }
@@ -363,9 +373,14 @@ bool counted_loop(am::morph *m) {
     if (m->get_state() == am::STATE_BLOB_MATCHING) m->next_state();
     if (!step(m, 1)) return false;  // chain build (and the learned warm start)
     if (m->get_state() != am::STATE_ATOM_MORPHING) return false;
+    if (!opt.load_chains.empty()) {
+        if (!m->load_chains(opt.load_chains.c_str())) return false;
+        if (opt.verbose) std::printf("Resumed chains from %s.\n", opt.load_chains.c_str());
+    }
     if (opt.verbose) std::printf("Matching atoms: %zu steps, energy %.2f.\n", morph_steps, m->get_energy());
     if (morph_steps && !step(m, morph_steps)) return false;
     if (opt.verbose) std::printf("Matching atoms done, energy %.2f.\n", m->get_energy());
+    if (!opt.save_chains.empty() && !m->save_chains(opt.save_chains.c_str())) return false;
     m->next_state();
     return step(m, 1) && m->get_state() == am::STATE_DONE;
 }

@@ -1686,6 +1686,100 @@ bool morph::put_chain_points(const std::vector<uint64_t> &pts) {
     return e.snapshot();
 }
 
+// Chain checkpoint file (no counterpart in the reference, which keeps no
+// state on disk), little-endian:
+//   "AMCHAIN1"  u32 version = 1, u32 key frames h, u32 chains, u32 0,
+//   u64 atoms, u32 width[chains], u64 point[h][atoms] (key-frame-major),
+//   u64 FNV-1a 64 of every byte before it.
+namespace {
+constexpr char kChainMagic[8] = {'A', 'M', 'C', 'H', 'A', 'I', 'N', '1'};
+uint64_t fnv1a(const uint8_t *p, size_t n, uint64_t h = 0xcbf29ce484222325ull) {
+    for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 0x100000001b3ull;
+    return h;
+}
+template <class T> void put_le(std::vector<uint8_t> &v, T x) {
+    for (size_t k = 0; k < sizeof(T); ++k) v.push_back(uint8_t(uint64_t(x) >> (8 * k)));
+}
+template <class T> T get_le(const uint8_t *p) {
+    uint64_t x = 0;
+    for (size_t k = 0; k < sizeof(T); ++k) x |= uint64_t(p[k]) << (8 * k);
+    return T(x);
+}
+}  // namespace
+
+bool morph::save_chains(const char *path) {
+    engine &e = *E;
+    std::vector<uint64_t> pts;
+    size_t h = 0;
+    std::vector<size_t> widths;
+    if (!path || !e.have_snap || get_chain_points(&pts, &h, &widths) == 0 || pts.empty()) {
+        e.err = "save_chains: no synchronized chains";
+        return false;
+    }
+    std::vector<uint8_t> out(kChainMagic, kChainMagic + 8);
+    put_le<uint32_t>(out, 1);
+    put_le<uint32_t>(out, uint32_t(h));
+    put_le<uint32_t>(out, uint32_t(widths.size()));
+    put_le<uint32_t>(out, 0);
+    put_le<uint64_t>(out, uint64_t(e.wtot));
+    for (size_t w : widths) put_le<uint32_t>(out, uint32_t(w));
+    const size_t head = out.size();
+    out.resize(head + pts.size() * 8);
+    for (size_t i = 0; i < pts.size(); ++i)
+        for (int k = 0; k < 8; ++k) out[head + i * 8 + k] = uint8_t(pts[i] >> (8 * k));
+    put_le<uint64_t>(out, fnv1a(out.data(), out.size()));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) {
+        e.err = std::string("save_chains: cannot create ") + path;
+        return false;
+    }
+    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    if (std::fclose(f) != 0 || !ok) {
+        e.err = std::string("save_chains: write failed: ") + path;
+        return false;
+    }
+    return true;
+}
+
+bool morph::load_chains(const char *path) {
+    engine &e = *E;
+    if (!path || is_busy() || e.wstate != STATE_ATOM_MORPHING || e.wtot == 0) {
+        e.err = "load_chains: not paused in ATOM_MORPHING";
+        return false;
+    }
+    FILE *f = std::fopen(path, "rb");
+    if (!f) {
+        e.err = std::string("load_chains: cannot open ") + path;
+        return false;
+    }
+    std::vector<uint8_t> in;
+    uint8_t buf[65536];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) in.insert(in.end(), buf, buf + n);
+    std::fclose(f);
+    auto bad = [&](const char *why) {
+        e.err = std::string("load_chains: ") + why;
+        return false;
+    };
+    if (in.size() < 40 || std::memcmp(in.data(), kChainMagic, 8) != 0) return bad("not a chain file");
+    if (fnv1a(in.data(), in.size() - 8) != get_le<uint64_t>(in.data() + in.size() - 8)) return bad("checksum mismatch");
+    const uint32_t version = get_le<uint32_t>(in.data() + 8), h = get_le<uint32_t>(in.data() + 12);
+    const uint32_t nch = get_le<uint32_t>(in.data() + 16);
+    const uint64_t atoms = get_le<uint64_t>(in.data() + 24);
+    if (version != 1) return bad("unknown version");
+    if (h != uint32_t(e.h) || nch != e.cw.size() || atoms != uint64_t(e.wtot))
+        return bad("the file's chains do not have the engine's shape");
+    const size_t head = 32 + size_t(nch) * 4;
+    if (in.size() != head + size_t(h) * atoms * 8 + 8) return bad("truncated file");
+    for (uint32_t c = 0; c < nch; ++c)
+        if (get_le<uint32_t>(in.data() + 32 + 4 * c) != uint32_t(e.cw[c]))
+            return bad("the file's chain widths differ from the engine's");
+    std::vector<uint64_t> pts(size_t(h) * atoms);
+    for (size_t i = 0; i < pts.size(); ++i) pts[i] = get_le<uint64_t>(in.data() + head + 8 * i);
+    if (!put_chain_points(pts)) return bad("upload failed");
+    return true;
+}
+
 size_t morph::get_fluid_particles(std::vector<double> *out) {
     engine &e = *E;
     if (!e.fluid || e.wtot == 0) return 0;

@@ -158,6 +158,13 @@ class morph {
                                      std::vector<size_t> *widths);
     // Replace the chains' refined points (same widths), e.g. a warm start.
     bool        put_chain_points    (const std::vector<uint64_t> &pts);
+    // Chain checkpoint (no counterpart in the reference): save_chains writes
+    // the synchronized chains to a file; load_chains (paused in
+    // ATOM_MORPHING, on chains of the same shapes: same key frames, seed
+    // and matching) puts them back, so a refinement can resume.  The file
+    // carries the shapes and a checksum; a mismatch is an error.
+    bool        save_chains         (const char *path);
+    bool        load_chains         (const char *path);
     // Learned warm start (no counterpart in the reference): with params
     // (AMH_MLP_PARAMS floats, see atomorph_hip.h) the chains are reordered by
     // the MLP's sort-and-match right after they are built; NULL turns it off.

@@ -173,6 +173,8 @@ int am_morph_autotune(void *mp, double budget_ms, int *tile, unsigned *rounds, d
     if (nreport) *nreport = rep.size();
     return 0;
 }
+int am_morph_save_chains(void *m, const char *path) { return static_cast<am::morph *>(m)->save_chains(path) ? 0 : -1; }
+int am_morph_load_chains(void *m, const char *path) { return static_cast<am::morph *>(m)->load_chains(path) ? 0 : -1; }
 void am_morph_set_learned(void *m, const float *params) { static_cast<am::morph *>(m)->set_learned(params); }
 int am_morph_warm_start(void *m, float *gpu_ms) { return static_cast<am::morph *>(m)->warm_start(gpu_ms) ? 0 : -1; }
 size_t am_morph_learn_samples(void *mp, float *x, float *t) {

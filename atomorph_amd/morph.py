@@ -49,6 +49,8 @@ _PROTOS = {
     "am_morph_fluid_particles": (C.c_size_t, [vp, vp]),
     "am_morph_set_learned": (None, [vp, vp]),
     "am_morph_warm_start": (C.c_int, [vp, C.POINTER(C.c_float)]),
+    "am_morph_save_chains": (C.c_int, [vp, C.c_char_p]),
+    "am_morph_load_chains": (C.c_int, [vp, C.c_char_p]),
     "am_morph_autotune": (C.c_int, [vp, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_uint), vp, sz,
                                     C.POINTER(sz)]),
     "am_morph_learn_samples": (C.c_size_t, [vp, vp, vp]),
@@ -201,6 +203,16 @@ class Morph:
         rows = [{"tile": int(a), "rounds": int(b), "passes": int(c), "ms": float(d), "gain_at_budget": float(e)}
                 for a, b, c, d, e in rep[:min(n.value, 64)]]
         return t.value, r.value, rows
+
+    def save_chains(self, path):
+        """Checkpoint the synchronized chains to a file."""
+        if self.lib.am_morph_save_chains(self.h, str(path).encode()) != 0:
+            raise _capi.AmhError(self.error())
+
+    def load_chains(self, path):
+        """Resume from a checkpoint (paused in ATOM_MORPHING, same chain shapes)."""
+        if self.lib.am_morph_load_chains(self.h, str(path).encode()) != 0:
+            raise _capi.AmhError(self.error())
 
     def learn_samples(self):
         """(features (n, 8), targets (n, 2)) float32 from the synchronized chains."""

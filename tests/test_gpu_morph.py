@@ -5,6 +5,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+from atomorph_amd import _capi as A
 from atomorph_amd import morph as M
 from atomorph_amd import synth
 from oracle import oracle_py as op
@@ -168,3 +169,38 @@ def test_blobs512_pipeline_runs():
     assert c1 < c0
     img = m.render(0.25)
     assert img.shape == (512, 512, 4) and (img[..., 3] > 0).sum() > 50000
+
+
+def test_chain_checkpoint_round_trip(tmp_path):
+    """save_chains / load_chains: a second engine on the same key frames and
+    seed resumes from the file with the same points, cost and frames; a
+    corrupted file and chains of another shape are refused."""
+    frames = synth.workload("ellipses64")
+    a = _pipeline(frames)
+    a.set_refine(M.REFINE_BATCHED, rounds=8, tile=2, graph_passes=0)
+    G.log("checkpoint: refine and save")
+    a.run(4)
+    path = tmp_path / "chains.amc"
+    a.save_chains(path)
+    pts_a, widths = a.chains()
+    b = _pipeline(frames)
+    pts_b, _ = b.chains()
+    assert not np.array_equal(pts_b, pts_a)
+    G.log("checkpoint: load")
+    b.load_chains(path)
+    pts_b, widths_b = b.chains()
+    assert np.array_equal(pts_b, pts_a) and np.array_equal(widths_b, widths)
+    assert b.cost() == a.cost()
+    assert (b.render(0.3) == a.render(0.3)).all()
+    b.set_refine(M.REFINE_BATCHED, rounds=8, tile=2, graph_passes=0)
+    b.run(2)
+    assert b.cost() <= a.cost()
+    data = bytearray(path.read_bytes())
+    data[len(data) // 2] ^= 0x10
+    bad = tmp_path / "bad.amc"
+    bad.write_bytes(bytes(data))
+    with pytest.raises(A.AmhError, match="checksum"):
+        b.load_chains(bad)
+    other = _pipeline(synth.ellipse_frames(seed=2, size=64, count=6, nframes=2))
+    with pytest.raises(A.AmhError, match="shape|widths"):
+        other.load_chains(path)
