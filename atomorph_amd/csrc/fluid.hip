@@ -343,11 +343,11 @@ struct SplatTiles {  // the splat: pixel tiles holding the 2x2 taps at (X, Y)
     }
 };
 
-template <class Tiles>
-__global__ __launch_bounds__(kT) void k_bin_count(Tiles tiles_of, int n, Bins B, int par) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-    const bool act = i < n && tiles_of(i, tx0, tx1, ty0, ty1);
+// Counts slot i's entries (tiles tx0..tx1 x ty0..ty1 when act) into
+// B.count[par] and stores each entry's place in its bin in B.rank.  Every
+// lane of the wave calls it (it groups the wave's lanes with ballots).
+__device__ __forceinline__ void count_entries(bool act, int tx0, int tx1, int ty0, int ty1, int i, int n,
+                                              const Bins &B, int par) {
     const int lane = threadIdx.x & 63;
     const uint64_t below = (1ull << lane) - 1ull;
     // Group the wave's lanes by bin, per entry: lane lead[j] leads the lanes
@@ -378,6 +378,14 @@ __global__ __launch_bounds__(kT) void k_bin_count(Tiles tiles_of, int n, Bins B,
         const int b = __shfl(base[j], lead[j] < 0 ? lane : lead[j], 64);
         if (i < n) B.rank[int64_t(j) * n + i] = lead[j] < 0 ? 0 : b + pre[j];
     }
+}
+
+template <class Tiles>
+__global__ __launch_bounds__(kT) void k_bin_count(Tiles tiles_of, int n, Bins B, int par) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    const bool act = i < n && tiles_of(i, tx0, tx1, ty0, ty1);
+    count_entries(act, tx0, tx1, ty0, ty1, i, n, B, par);
 }
 
 // Particles into their bins (the tiles recomputed from the unchanged
@@ -712,104 +720,90 @@ __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
     return double(mix64(seed ^ (uint64_t(i) << 3) ^ uint64_t(k)) >> 11) * (1.0 / 9007199254740992.0) * 0.01;
 }
 
-// Advection of every active particle by the block of its home tile (each
-// particle has exactly one home entry), with the node velocities and
-// colours of the tile's halo staged in LDS.  atom: the particle held in each
-// slot when P is the cell-ordered copy (NULL: slot i holds particle i); the
-// wall jitter is keyed by particle.
-__global__ __launch_bounds__(kT) void k_advect(Particles P, Grid Gd, int gsx, int gsy, Bins B, int steps_left,
-                                               double freedom_radius, uint64_t seed,
-                                               const int32_t *__restrict__ atom) {
-    __shared__ float nodes[7][kHalo * kHalo];  // u/m, v/m, w, r, g, b, a
-    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
-    const int x0 = tx * kTile, y0 = ty * kTile;
-    stage_halo(nodes, x0, y0, gsx, gsy, [&](int f, int64_t k) {
-        switch (f) {
-            case 0: return per_mass(Gd.u[k], Gd.m[k]);
-            case 1: return per_mass(Gd.v[k], Gd.m[k]);
-            case 2: return Gd.w[k];
-            case 3: return Gd.r[k];
-            case 4: return Gd.g[k];
-            case 5: return Gd.b[k];
-            default: return Gd.a[k];
-        }
-    });
-    __syncthreads();
-    auto fetch = [&](int slot) {
-        double2 it = {0.0, 0.0};
-        if (slot >= 0) it = double2{P.x[slot], P.y[slot]};
-        return it;
-    };
-    for_bin(B, int(blockIdx.x), fetch, [&](int i, bool home, const double2 &it) {
-        if (i < 0 || !home) return;
-        const int id = atom ? atom[i] : i;
-        int cx, cy;
-        double px[3], py[3], gx[3], gy[3];
-        double x = it.x, y = it.y;
-        weights(x, y, cx, cy, px, py, gx, gy);
-        const bool staged = in_reach(cx, cy, x0, y0);
-        double gu = 0, gv = 0, nR = 0, nG = 0, nB = 0, nA = 0, wsum = 0;
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) {
-                if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
-                const double phi = px[a] * py[b];
-                float uu, vv, w, r, g, bb, aa;
-                if (staged) {
-                    const int e = halo_index(cx + a, cy + b, x0, y0);
-                    uu = nodes[0][e]; vv = nodes[1][e]; w = nodes[2][e];
-                    r = nodes[3][e]; g = nodes[4][e]; bb = nodes[5][e]; aa = nodes[6][e];
-                } else {
-                    const int64_t k = int64_t(cx + a) * gsy + cy + b;
-                    uu = per_mass(Gd.u[k], Gd.m[k]); vv = per_mass(Gd.v[k], Gd.m[k]); w = Gd.w[k];
-                    r = Gd.r[k]; g = Gd.g[k]; bb = Gd.b[k]; aa = Gd.a[k];
-                }
-                gu += phi * uu;
-                gv += phi * vv;
-                if (double(w) > 0.0) {
-                    // node colour = sum(s c)/sum(s); weighted by sum(s)
-                    wsum += w;
-                    nR += r;
-                    nG += g;
-                    nB += bb;
-                    nA += aa;
-                }
-            }
-        if (wsum > 0.0) {  // colour diffusion (fluidmodel.cpp:470-495)
-            nR /= wsum; nG /= wsum; nB /= wsum; nA /= wsum;
-            if (!P.mature[i]) {
-                P.r[i] = nR; P.g[i] = nG; P.b[i] = nB; P.a[i] = nA;
-            } else {
-                const double wr = fabs(nR - P.R[i]), wg = fabs(nG - P.G[i]), wb = fabs(nB - P.B[i]),
-                             wa = fabs(nA - P.A[i]);
-                P.r[i] = (1.0 - wr) * P.r[i] + wr * nR;
-                P.g[i] = (1.0 - wg) * P.g[i] + wg * nG;
-                P.b[i] = (1.0 - wb) * P.b[i] + wb * nB;
-                P.a[i] = (1.0 - wa) * P.a[i] + wa * nA;
+// One particle's advection (fluidmodel.cpp:443-564); returns its new
+// position.  atom: the particle held in each slot when P is the cell-ordered
+// copy (NULL: slot i holds particle i); the wall jitter is keyed by particle.
+__device__ __forceinline__ double2 advect_one(Particles P, int i, const Grid &Gd, int gsx, int gsy, int steps_left,
+                                              double freedom_radius, uint64_t seed, const int32_t *__restrict__ atom) {
+    const int id = atom ? atom[i] : i;
+    int cx, cy;
+    double px[3], py[3], gx[3], gy[3];
+    double x = P.x[i], y = P.y[i];
+    weights(x, y, cx, cy, px, py, gx, gy);
+    double gu = 0, gv = 0, nR = 0, nG = 0, nB = 0, nA = 0, wsum = 0;
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
+            const int64_t k = int64_t(cx + a) * gsy + cy + b;
+            const double phi = px[a] * py[b];
+            const float mk = Gd.m[k];
+            gu += phi * per_mass(Gd.u[k], mk);
+            gv += phi * per_mass(Gd.v[k], mk);
+            const double w = Gd.w[k];
+            if (w > 0.0) {
+                // node colour = sum(s c)/sum(s); weighted by sum(s)
+                wsum += w;
+                nR += Gd.r[k];
+                nG += Gd.g[k];
+                nB += Gd.b[k];
+                nA += Gd.a[k];
             }
         }
-        x += gu;
-        y += gv;
-        {   // freedom radius (fluidmodel.cpp:501-539)
-            const double A = fabs(y - P.gy[i]), Bd = fabs(x - P.gx[i]), Cd = sqrt(A * A + Bd * Bd);
-            const double r = freedom_radius * P.fr[i];
-            if (Cd > r) {
-                double mx, my;
-                toward(x, y, P.gx[i], P.gy[i], Cd - r, mx, my);
-                const double w = 1.0 / (steps_left + 1);
-                x += mx * w;
-                y += my * w;
-            }
+    if (wsum > 0.0) {  // colour diffusion (fluidmodel.cpp:470-495)
+        nR /= wsum; nG /= wsum; nB /= wsum; nA /= wsum;
+        if (!P.mature[i]) {
+            P.r[i] = nR; P.g[i] = nG; P.b[i] = nB; P.a[i] = nA;
+        } else {
+            const double wr = fabs(nR - P.R[i]), wg = fabs(nG - P.G[i]), wb = fabs(nB - P.B[i]),
+                         wa = fabs(nA - P.A[i]);
+            P.r[i] = (1.0 - wr) * P.r[i] + wr * nR;
+            P.g[i] = (1.0 - wg) * P.g[i] + wg * nG;
+            P.b[i] = (1.0 - wb) * P.b[i] + wb * nB;
+            P.a[i] = (1.0 - wa) * P.a[i] + wa * nA;
         }
-        double u = P.u2[i], v = P.v2[i];
-        u += gu - u;
-        v += gv - v;
-        if (x < 1.0) { x = 1.0 + jitter(seed, id, 0); u = 0.0; }
-        else if (x > gsx - 2) { x = gsx - 2 - jitter(seed, id, 1); u = 0.0; }
-        if (y < 1.0) { y = 1.0 + jitter(seed, id, 2); v = 0.0; }
-        else if (y > gsy - 2) { y = gsy - 2 - jitter(seed, id, 3); v = 0.0; }
-        P.x[i] = x; P.y[i] = y; P.u[i] = u; P.v[i] = v;
-    });
+    }
+    x += gu;
+    y += gv;
+    {   // freedom radius (fluidmodel.cpp:501-539)
+        const double A = fabs(y - P.gy[i]), B = fabs(x - P.gx[i]), Cd = sqrt(A * A + B * B);
+        const double r = freedom_radius * P.fr[i];
+        if (Cd > r) {
+            double mx, my;
+            toward(x, y, P.gx[i], P.gy[i], Cd - r, mx, my);
+            const double w = 1.0 / (steps_left + 1);
+            x += mx * w;
+            y += my * w;
+        }
+    }
+    double u = P.u2[i], v = P.v2[i];
+    u += gu - u;
+    v += gv - v;
+    if (x < 1.0) { x = 1.0 + jitter(seed, id, 0); u = 0.0; }
+    else if (x > gsx - 2) { x = gsx - 2 - jitter(seed, id, 1); u = 0.0; }
+    if (y < 1.0) { y = 1.0 + jitter(seed, id, 2); v = 0.0; }
+    else if (y > gsy - 2) { y = gsy - 2 - jitter(seed, id, 3); v = 0.0; }
+    P.x[i] = x; P.y[i] = y; P.u[i] = u; P.v[i] = v;
+    return double2{x, y};
 }
+
+// Advection, one thread per particle slot.  (A per-tile form over the home
+// entries with the node halo in LDS measured slower: the particle fields it
+// gathers through the bin list cost more than the node reads it saves.)
+// With count_par >= 0 the moved particles are also counted into the next
+// step's bins (B.count[count_par]), which saves that step's counting pass.
+__global__ __launch_bounds__(kT) void k_advect(Particles P, int n, Grid Gd, int gsx, int gsy, int steps_left,
+                                               double freedom_radius, uint64_t seed,
+                                               const int32_t *__restrict__ atom, Bins B, int count_par) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    const bool act = i < n && P.active[i];
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    if (act) {
+        const double2 q = advect_one(P, i, Gd, gsx, gsy, steps_left, freedom_radius, seed, atom);
+        stencil_tiles(q.x, q.y, gsx, gsy, tx0, tx1, ty0, ty1);
+    }
+    if (count_par >= 0) count_entries(act, tx0, tx1, ty0, ty1, i, n, B, count_par);
+}
+
 
 __global__ void k_fill_i32(int32_t *p, int64_t n) {
     const int64_t k = int64_t(blockIdx.x) * kT + threadIdx.x;
@@ -1076,10 +1070,12 @@ namespace {
 // (particle order, or the cell-ordered copy with its atom map).
 // Counting sort of the particles' entries into the bins of B (tiles bins,
 // counts of parity par; the other parity is cleared for the next use).
+// counted: the counts and ranks are in place already.
 template <class Tiles>
-void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s) {
+void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s,
+                   bool counted = false) {
     const uint32_t gp = uint32_t((n + kT - 1) / kT);
-    hipLaunchKernelGGL(k_bin_count<Tiles>, dim3(gp), dim3(kT), 0, s, tiles_of, n, B, par);
+    if (!counted) hipLaunchKernelGGL(k_bin_count<Tiles>, dim3(gp), dim3(kT), 0, s, tiles_of, n, B, par);
     const uint32_t gb = uint32_t((std::max(n, tiles) + kT - 1) / kT);
     if (tiles <= kLdsTiles) {
         hipLaunchKernelGGL((k_bin_place<true, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
@@ -1089,8 +1085,13 @@ void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int p
     }
 }
 
+// One model step on the particles Q.  counted: the previous step's
+// advection counted this step's bins; count_next: this step's advection
+// counts the next step's (when another step follows with unchanged
+// particles).
 int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32_t steps_left,
-                  double freedom_radius, uint64_t seed, hipStream_t s) {
+                  double freedom_radius, uint64_t seed, hipStream_t s, bool counted = false,
+                  bool count_next = false) {
     amh::ProfScope step_("fluid_step", s);
     if (f->count == 0) return AMH_OK;
     const Bins &B = f->bins;
@@ -1099,7 +1100,7 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
         amh::ProfScope ps_("fluid_bins", s);
         const int par = f->bin_par;
         f->bin_par ^= 1;
-        bin_particles(StencilTiles{Q, f->gsx, f->gsy}, f->count, B, tiles, par, s);
+        bin_particles(StencilTiles{Q, f->gsx, f->gsy}, f->count, B, tiles, par, s, counted);
     }
     {
         amh::ProfScope ps_("k_p2g", s);
@@ -1115,8 +1116,9 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     }
     {
         amh::ProfScope ps_("k_advect", s);
-        hipLaunchKernelGGL(k_advect, dim3(uint32_t(tiles)), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B,
-                           int(steps_left), freedom_radius, seed, atom);
+        hipLaunchKernelGGL(k_advect, dim3(uint32_t((f->count + kT - 1) / kT)), dim3(kT), 0, s, Q, f->count, f->G,
+                           f->gsx, f->gsy, int(steps_left), freedom_radius, seed, atom, B,
+                           count_next ? f->bin_par : -1);
     }
     AMH_LAUNCH_CHECK("amh_fluid_step");
     return AMH_OK;
@@ -1550,7 +1552,7 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
         if (int r = sort_particles(f, s)) return r;
         for (int k = 0; k < fa->fluidsteps; ++k) {
             const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), radius,
-                                        mix64(f->frame * 131 + k), s);
+                                        mix64(f->frame * 131 + k), s, k > 0, k + 1 < fa->fluidsteps);
             if (r) return r;
         }
         {

@@ -450,6 +450,10 @@ extern "C" int amh_mlp_set_params(void *mlp, const float *host, void *stream) {
     hipLaunchKernelGGL(k_fill, dim3((kParams + 255) / 256), dim3(256), 0, s, M->v, int64_t(kParams), 0.0f);
     M->step = 0;
     M->image_stale = true;
+    if (fused_ok()) {  // the fused forward's weight image, now rather than at the first forward
+        hipLaunchKernelGGL(k_fused_stage, dim3(64), dim3(256), 0, s, M->pb, M->p, M->image);
+        M->image_stale = false;
+    }
     AMH_LAUNCH_CHECK("amh_mlp_set_params");
     return check_hip(hipStreamSynchronize(s), "set_params");
 }
