@@ -822,13 +822,19 @@ __device__ __forceinline__ int cell_of(double x, double y, int gsx, int gsy) {
     return cx * gsy + cy;
 }
 
+// Inactive particles go after every cell, spread over this many buckets by
+// wave: one bucket for all of them made every wave's atomic hit the same
+// counter, and those serialise.
+constexpr int kIdleBuckets = 256;
+
 __global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, int gsy,
                                                   const int32_t *__restrict__ cell_rank, int32_t *__restrict__ cell,
                                                   int32_t *__restrict__ pos, int32_t *__restrict__ cnt) {
     const int i = blockIdx.x * kT + threadIdx.x;
     const bool in = i < n;
     const int nn = gsx * gsy;
-    const int key = !in ? -1 : P.active[i] ? cell_rank[cell_of(P.x[i], P.y[i], gsx, gsy)] : nn;
+    const int idle = nn + int((uint32_t(i) >> 6) & uint32_t(kIdleBuckets - 1));
+    const int key = !in ? -1 : P.active[i] ? cell_rank[cell_of(P.x[i], P.y[i], gsx, gsy)] : idle;
     // group the wave's lanes by key: one atomic per distinct key, all in flight together
     const int lane = threadIdx.x & 63;
     int lead = lane, pre = 0, total = 0;
@@ -851,12 +857,24 @@ __global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, 
     pos[i] = b + pre;
 }
 
-__global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, const int32_t *__restrict__ cell,
-                                                   const int32_t *__restrict__ pos,
-                                                   const int32_t *__restrict__ start, Particles S) {
+// Slot s of particle i in the cell order: S.atom[s] = i, and slot[i] = s
+// (slot may alias pos: each thread reads its own entry first).
+__global__ __launch_bounds__(kT) void k_sort_slots(int n, const int32_t *__restrict__ cell, const int32_t *pos,
+                                                  const int32_t *__restrict__ start, int32_t *slot, Particles S) {
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
     const int s = start[cell[i]] + pos[i];
+    S.atom[s] = i;
+    slot[i] = s;
+}
+
+// S = P in cell order, one thread per slot: the 18 fields are written
+// coalesced and read through the slot's particle (a scattered store per
+// field moved whole cache lines for 8 bytes).
+__global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, Particles S) {
+    const int s = blockIdx.x * kT + threadIdx.x;
+    if (s >= n) return;
+    const int i = S.atom[s];
     S.x[s] = P.x[i]; S.y[s] = P.y[i]; S.u[s] = P.u[i]; S.v[s] = P.v[i];
     S.gx[s] = P.gx[i]; S.gy[s] = P.gy[i]; S.fr[s] = P.fr[i];
     S.R[s] = P.R[i]; S.G[s] = P.G[i]; S.B[s] = P.B[i]; S.A[s] = P.A[i];
@@ -864,14 +882,16 @@ __global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, const in
     S.strength[s] = P.strength[i];
     S.active[s] = P.active[i];
     S.mature[s] = P.mature[i];
-    S.atom[s] = i;
 }
 
-// What the steps and the splat change goes back to particle order.
-__global__ __launch_bounds__(kT) void k_sort_scatter(Particles S, int n, Particles P) {
-    const int s = blockIdx.x * kT + threadIdx.x;
-    if (s >= n || !S.active[s]) return;
-    const int i = S.atom[s];
+// What the steps and the splat change goes back to particle order, one
+// thread per particle (coalesced stores).
+__global__ __launch_bounds__(kT) void k_sort_scatter(Particles S, int n, const int32_t *__restrict__ slot,
+                                                    Particles P) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    if (i >= n) return;
+    const int s = slot[i];
+    if (!S.active[s]) return;
     P.x[i] = S.x[s]; P.y[i] = S.y[s]; P.u[i] = S.u[s]; P.v[i] = S.v[s];
     P.r[i] = S.r[s]; P.g[i] = S.g[s]; P.b[i] = S.b[s]; P.a[i] = S.a[s];
     P.mature[i] = S.mature[s];
@@ -887,7 +907,7 @@ struct amh_fluid_s {
     Particles S;               // the frame's cell-ordered copy
     Grid G;
     // cell ordering: per particle its cell and rank in it; per cell counts
-    // and starts ([nn + 1]: the last bucket holds the inactive particles)
+    // and starts ([nn + kIdleBuckets + 1]: the inactive particles last)
     int32_t *cell = nullptr, *pos = nullptr, *cnt = nullptr, *start = nullptr;
     int32_t *scan_sums = nullptr, *scan_offs = nullptr;
     int32_t *cell_rank = nullptr;  // [nn] sort key of each cell (Morton order)
@@ -908,11 +928,11 @@ struct amh_fluid_s {
 namespace {
 size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     const size_t n = size_t(count), nn = size_t(gsx) * gsy, npx = size_t(W) * H;
-    const size_t nb = size_t(scan::blocks(int64_t(nn) + 1));
+    const size_t nb = size_t(scan::blocks(int64_t(nn) + kIdleBuckets));
     const size_t tiles = size_t((gsx + kTile - 1) / kTile) * size_t((gsy + kTile - 1) / kTile);
     const size_t ptiles = size_t((W + kPxTile - 1) / kPxTile) * size_t((H + kPxTile - 1) / kPxTile);
     (void)nkeys;
-    return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + 2) * 4 * 2 +
+    return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
            (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) +
            nn * 4 * kGridFields + npx * 4 * 2 + 64 * 256;
 }
@@ -946,9 +966,9 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     carve_particles(f->S, n, take);
     f->cell = reinterpret_cast<int32_t *>(take(n * 4));
     f->pos = reinterpret_cast<int32_t *>(take(n * 4));
-    f->cnt = reinterpret_cast<int32_t *>(take((nn + 2) * 4));
-    f->start = reinterpret_cast<int32_t *>(take((nn + 2) * 4));
-    const size_t nb = size_t(scan::blocks(int64_t(nn) + 1));
+    f->cnt = reinterpret_cast<int32_t *>(take((nn + kIdleBuckets + 1) * 4));
+    f->start = reinterpret_cast<int32_t *>(take((nn + kIdleBuckets + 1) * 4));
+    const size_t nb = size_t(scan::blocks(int64_t(nn) + kIdleBuckets));
     f->scan_sums = reinterpret_cast<int32_t *>(take(nb * 4));
     f->scan_offs = reinterpret_cast<int32_t *>(take((nb + 1) * 4));
     f->cell_rank = reinterpret_cast<int32_t *>(take(nn * 4));
@@ -1129,11 +1149,14 @@ int sort_particles(amh_fluid_s *f, hipStream_t s) {
     amh::ProfScope ps_("fluid_sort", s);
     const int64_t nn = int64_t(f->gsx) * f->gsy;
     const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
-    hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((nn + 1 + kT - 1) / kT)), dim3(kT), 0, s, f->cnt, nn + 1);
+    const int64_t nk = nn + kIdleBuckets;
+    hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((nk + kT - 1) / kT)), dim3(kT), 0, s, f->cnt, nk);
     hipLaunchKernelGGL(k_sort_count, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->gsx, f->gsy, f->cell_rank,
                        f->cell, f->pos, f->cnt);
-    scan::exclusive(f->cnt, nn + 1, f->scan_sums, f->scan_offs, f->start, s);
-    hipLaunchKernelGGL(k_sort_gather, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->cell, f->pos, f->start, f->S);
+    scan::exclusive(f->cnt, nk, f->scan_sums, f->scan_offs, f->start, s);
+    // f->pos becomes each particle's slot (k_sort_scatter reads it after the steps)
+    hipLaunchKernelGGL(k_sort_slots, dim3(gp), dim3(kT), 0, s, f->count, f->cell, f->pos, f->start, f->pos, f->S);
+    hipLaunchKernelGGL(k_sort_gather, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->S);
     AMH_LAUNCH_CHECK("fluid sort");
     return AMH_OK;
 }
@@ -1567,7 +1590,7 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
             hipLaunchKernelGGL(k_splat_tile, dim3(uint32_t(ptiles)), dim3(kT), 0, s, f->pbins, f->sxy, f->staps,
                                f->scol, f->W, f->H, F.tl, R.keep_background, f->img, f->mask);
         }
-        hipLaunchKernelGGL(k_sort_scatter, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->P);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->pos, f->P);
     } else {  // nothing drawn: every pixel empty
         hipLaunchKernelGGL(k_fill_i32, dim3(gpx), dim3(kT), 0, s, f->mask, npx);
     }

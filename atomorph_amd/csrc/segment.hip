@@ -340,33 +340,45 @@ __global__ __launch_bounds__(256) void k_group_hist(const int *__restrict__ bop,
     }
 }
 
-// Column scan: for blob b, start[blk][b] = first[b] + sum over earlier blocks.
-__global__ void k_group_colscan(int *__restrict__ hist, int nblk, int nblobs, const int64_t *__restrict__ first) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nblobs) return;
-    int64_t run = first[b];
-    for (int k = 0; k < nblk; ++k) {
+// Column scan, one workgroup per blob b: hist[blk][b] becomes the number of
+// b's pixels in earlier blocks, and count[b] the blob's total.
+__global__ __launch_bounds__(256) void k_group_colscan(int *__restrict__ hist, int nblk, int nblobs,
+                                                       int64_t *__restrict__ count) {
+    __shared__ int wsum[4];
+    __shared__ int carry;
+    const int b = int(blockIdx.x), lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nblk; base += 256) {
+        const int k = base + int(threadIdx.x);
         int *p = hist + int64_t(k) * nblobs + b;
-        const int c = *p;
-        *p = int(run);
-        run += c;
+        const int v = k < nblk ? *p : 0;
+        int incl = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int t = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += t;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int before = carry;
+        for (int w = 0; w < wave; ++w) before += wsum[w];
+        if (k < nblk) *p = before + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 255) carry = before + incl;
+        __syncthreads();
     }
+    if (threadIdx.x == 0) count[b] = carry;
 }
 
-// Blob starts: exclusive scan of counts from the statistics.
-__global__ __launch_bounds__(1024) void k_first_from_hist(const int *__restrict__ hist, int nblk, int nblobs,
-                                                          int64_t *__restrict__ first) {
-    // counts per blob = sum over blocks (single workgroup, serial over
-    // blocks per blob then a scan over blobs)
+// Blob starts: exclusive scan of the counts in place, first[nblobs] = total.
+__global__ __launch_bounds__(1024) void k_first_scan(int nblobs, int64_t *__restrict__ first) {
     __shared__ int64_t carry;
     __shared__ int64_t buf[1024];
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < nblobs; base += 1024) {
         const int b = base + threadIdx.x;
-        int64_t cnt = 0;
-        if (b < nblobs)
-            for (int k = 0; k < nblk; ++k) cnt += hist[int64_t(k) * nblobs + b];
+        const int64_t cnt = b < nblobs ? first[b] : 0;
         buf[threadIdx.x] = cnt;
         __syncthreads();
         for (int off = 1; off < 1024; off <<= 1) {
@@ -384,7 +396,8 @@ __global__ __launch_bounds__(1024) void k_first_from_hist(const int *__restrict_
 }
 
 __global__ __launch_bounds__(64) void k_group_scatter(const int *__restrict__ bop, int64_t n, int nblobs,
-                                                      int *__restrict__ hist, int *__restrict__ list) {
+                                                      int *__restrict__ hist, const int64_t *__restrict__ first,
+                                                      int *__restrict__ list) {
     // one wave per block: walk the block's pixels in order, 64 at a time;
     // lanes with the same blob are ranked by a match over the wave.
     const int64_t base = int64_t(blockIdx.x) * kScanBlock;
@@ -406,7 +419,7 @@ __global__ __launch_bounds__(64) void k_group_scatter(const int *__restrict__ bo
         }
         const int start = b >= 0 ? h[b] : 0;
         __syncthreads();
-        if (b >= 0) list[start + rank] = int(i);
+        if (b >= 0) list[first[b] + start + rank] = int(i);
         if (b >= 0 && lastlane == lane) h[b] = start + total;
         __syncthreads();
     }
@@ -578,9 +591,9 @@ extern "C" int amh_group_pixels(const int32_t *bop, int32_t npixels, int32_t nbl
     if (int r = check_hip(hipMemsetAsync(hist, 0, size_t(nblk) * nblobs * sizeof(int), s), "amh_group_pixels memset"))
         return r;
     hipLaunchKernelGGL(k_group_hist, dim3(nblk), dim3(256), 0, s, bop, n, int(nblobs), hist);
-    hipLaunchKernelGGL(k_first_from_hist, dim3(1), dim3(1024), 0, s, hist, nblk, int(nblobs), first);
-    hipLaunchKernelGGL(k_group_colscan, dim3((nblobs + 255) / 256), dim3(256), 0, s, hist, nblk, int(nblobs), first);
-    hipLaunchKernelGGL(k_group_scatter, dim3(nblk), dim3(64), 0, s, bop, n, int(nblobs), hist, list);
+    hipLaunchKernelGGL(k_group_colscan, dim3(uint32_t(nblobs)), dim3(256), 0, s, hist, nblk, int(nblobs), first);
+    hipLaunchKernelGGL(k_first_scan, dim3(1), dim3(1024), 0, s, int(nblobs), first);
+    hipLaunchKernelGGL(k_group_scatter, dim3(nblk), dim3(64), 0, s, bop, n, int(nblobs), hist, first, list);
     AMH_LAUNCH_CHECK("amh_group_pixels");
     return AMH_OK;
 }
