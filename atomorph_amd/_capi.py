@@ -92,8 +92,8 @@ PROTOTYPES = {
     "amh_graph_launch": (C.c_int, [vp, vp]),
     "amh_graph_destroy": (C.c_int, [vp]),
     "amh_fluid_ranks": (C.c_int, [vp, i32, i64, vp, vp, i32, u64, vp, vp]),
-    "amh_refine_autotune": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i32, f64, u64, C.POINTER(i32), C.POINTER(i32),
-                                      vp, i32, C.POINTER(i32), vp]),
+    "amh_refine_autotune": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i32, f64, u64, i32, C.POINTER(i32),
+                                      C.POINTER(i32), vp, i32, C.POINTER(i32), vp]),
     "amh_rgb_to_hsp": (u32, [u32]),
     "amh_hsp_to_rgb": (u32, [u32]),
     "amh_rgb2hsp": (None, [f64, f64, f64, C.POINTER(f64), C.POINTER(f64), C.POINTER(f64)]),

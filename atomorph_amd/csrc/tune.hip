@@ -42,11 +42,11 @@ struct DevMem {
 
 extern "C" int amh_refine_autotune(const uint64_t *pts_dev, uint64_t *work_dev, int32_t nkeys, int64_t wtot,
                                    const int64_t *chain_off_dev, const int32_t *chain_w_dev, const int32_t *chain_w,
-                                   int32_t nchains, double budget_ms, uint64_t seed, int32_t *tile_out,
-                                   int32_t *rounds_out, amh_tune_entry *report, int32_t max_report,
-                                   int32_t *nreport, void *stream) {
+                                   int32_t nchains, double budget_ms, uint64_t seed, int32_t rounds_fixed,
+                                   int32_t *tile_out, int32_t *rounds_out, amh_tune_entry *report,
+                                   int32_t max_report, int32_t *nreport, void *stream) {
     if (!pts_dev || !work_dev || !chain_off_dev || !chain_w_dev || !chain_w || !tile_out || !rounds_out ||
-        nkeys < 2 || wtot <= 0 || nchains <= 0 || pts_dev == work_dev || max_report < 0 ||
+        nkeys < 2 || wtot <= 0 || nchains <= 0 || pts_dev == work_dev || max_report < 0 || rounds_fixed < 0 ||
         (max_report > 0 && !report)) {
         set_error("amh_refine_autotune: invalid argument");
         return AMH_E_ARG;
@@ -89,6 +89,8 @@ extern "C" int amh_refine_autotune(const uint64_t *pts_dev, uint64_t *work_dev, 
                               "amh_refine_autotune items"))
             return r;
         for (int32_t rounds : kRoundChoices) {
+            if (rounds_fixed > 0 && rounds != kRoundChoices[0]) continue;  // one candidate per tile
+            if (rounds_fixed > 0) rounds = rounds_fixed;
             // same start for every candidate
             if (int r = check_hip(hipMemcpyAsync(work_dev, pts_dev, pts_bytes, hipMemcpyDeviceToDevice, s),
                                   "amh_refine_autotune copy"))

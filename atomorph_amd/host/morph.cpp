@@ -213,7 +213,7 @@ struct morph::engine {
     int graph_tile = -1;
     unsigned graph_np = 0, graph_rounds = 0;
     // autotuned batched schedule for the current chains (-1: not yet)
-    int tuned_tile = -1, tuned_rounds = 0;
+    int tuned_tile = -1, tuned_rounds = 0, tuned_fixed = 0;
     DevBuf d_tune;
     // render snapshot (made at synchronize)
     DevBuf d_snap, d_rank, d_brgba, d_lag, d_slope, d_rws, d_out;
@@ -265,7 +265,7 @@ struct morph::engine {
     void fix_volatiles(size_t c);
     bool init_chains();
     bool morph_step();
-    bool autotune(double budget_ms, std::vector<amh_tune_entry> *report);
+    bool autotune(double budget_ms, int rounds_fixed, std::vector<amh_tune_entry> *report);
     bool exact_cost(uint64_t *lo, uint64_t *hi);
     bool snapshot();
     void run();
@@ -743,7 +743,7 @@ std::map<std::vector<int64_t>, std::pair<int, int>> g_tune;
 constexpr double kTuneBudgetMs = 3.0;
 }  // namespace
 
-bool morph::engine::autotune(double budget_ms, std::vector<amh_tune_entry> *report) {
+bool morph::engine::autotune(double budget_ms, int rounds_fixed, std::vector<amh_tune_entry> *report) {
     const size_t nch = cw.size();
     if (nch == 0 || h < 2 || wtot == 0) {
         err = "autotune: no chains";
@@ -753,15 +753,16 @@ bool morph::engine::autotune(double budget_ms, std::vector<amh_tune_entry> *repo
     std::vector<amh_tune_entry> rep(64);
     int32_t t = -1, r = 0, n = 0;
     const bool ok = amh_refine_autotune(d_pts.as<uint64_t>(), d_tune.as<uint64_t>(), h, wtot, d_off.as<int64_t>(),
-                                        d_w.as<int32_t>(), cw.data(), int32_t(nch), budget_ms, mix_seed(), &t, &r,
-                                        rep.data(), int32_t(rep.size()), &n, cstream) == AMH_OK;
+                                        d_w.as<int32_t>(), cw.data(), int32_t(nch), budget_ms, mix_seed(),
+                                        int32_t(rounds_fixed), &t, &r, rep.data(), int32_t(rep.size()), &n,
+                                        cstream) == AMH_OK;
     d_tune.release();
     if (!ok) return fail("autotune");
     rep.resize(std::min<size_t>(size_t(std::max(n, 0)), rep.size()));
     if (report) *report = rep;
     tuned_tile = t;
     tuned_rounds = r;
-    std::vector<int64_t> key{int64_t(h)};
+    std::vector<int64_t> key{int64_t(h), int64_t(rounds_fixed)};
     key.insert(key.end(), cw.begin(), cw.end());
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[key] = {t, r};
@@ -777,9 +778,12 @@ bool morph::engine::morph_step() {
     if (nch == 0 || h < 2) return false;
     if (wopt.refine_mode == REFINE_BATCHED) {
         int tile = wopt.refine_tile, rounds = int(wopt.refine_rounds);
-        if (tile < 0 || rounds <= 0) {  // autotuned: once per chain shape
-            if (tuned_tile < 0) {
-                std::vector<int64_t> key{int64_t(h)};
+        if (tile < 0 || rounds <= 0) {  // autotuned: once per chain shape (and fixed rounds)
+            const int fixed = rounds > 0 ? rounds : 0;
+            if (tuned_tile < 0 || tuned_fixed != fixed) {
+                tuned_tile = -1;
+                tuned_fixed = fixed;
+                std::vector<int64_t> key{int64_t(h), int64_t(fixed)};
                 key.insert(key.end(), cw.begin(), cw.end());
                 {
                     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -789,7 +793,7 @@ bool morph::engine::morph_step() {
                         tuned_rounds = it->second.second;
                     }
                 }
-                if (tuned_tile < 0 && !autotune(kTuneBudgetMs, nullptr)) return true;
+                if (tuned_tile < 0 && !autotune(kTuneBudgetMs, fixed, nullptr)) return true;
             }
             if (tile < 0) tile = tuned_tile;
             if (rounds <= 0) rounds = tuned_rounds;
@@ -1626,7 +1630,9 @@ bool morph::autotune(double budget_ms, int *tile, unsigned *rounds, std::vector<
     engine &e = *E;
     if (is_busy() || e.wstate != STATE_ATOM_MORPHING) { e.err = "autotune: not paused in ATOM_MORPHING"; return false; }
     std::vector<amh_tune_entry> rep;
-    if (!e.autotune(budget_ms, &rep)) return false;
+    const int fixed = rounds && *rounds > 0 ? int(*rounds) : 0;
+    if (!e.autotune(budget_ms, fixed, &rep)) return false;
+    e.tuned_fixed = fixed;
     if (tile) *tile = e.tuned_tile;
     if (rounds) *rounds = unsigned(e.tuned_rounds);
     if (report) {

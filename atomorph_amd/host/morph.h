@@ -140,8 +140,10 @@ class morph {
     // ATOM_MORPHING): every kernel variant x rounds per pass races from the
     // chains' present state for budget_ms of device time on a scratch copy
     // (the chains are not changed); the largest exact cost decrease wins.
-    // The result is used by batched passes with automatic tile or rounds and
-    // remembered for chains of the same shapes.  report may be NULL.
+    // With *rounds > 0 on input only that rounds value is raced (the tile is
+    // tuned for it).  The result is used by batched passes with automatic
+    // tile or rounds and remembered for chains of the same shapes.  report
+    // may be NULL.
     bool        autotune            (double budget_ms, int *tile, unsigned *rounds,
                                      std::vector<tune_entry> *report);
     // Matching ends by itself after this many steps (0 = only at

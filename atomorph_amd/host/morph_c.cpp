@@ -157,7 +157,8 @@ int am_morph_put_chains(void *mp, const uint64_t *pts, size_t n) {
     std::vector<uint64_t> p(pts, pts + n);
     return static_cast<am::morph *>(mp)->put_chain_points(p) ? 0 : -1;
 }
-// report: max_report rows of (tile, rounds, passes, ms, gain_at_budget)
+// report: max_report rows of (tile, rounds, passes, ms, gain_at_budget);
+// *rounds > 0 on input: only that rounds value is raced
 int am_morph_autotune(void *mp, double budget_ms, int *tile, unsigned *rounds, double *report, size_t max_report,
                       size_t *nreport) {
     std::vector<am::tune_entry> rep;

@@ -191,11 +191,12 @@ class Morph:
             raise _capi.AmhError(self.error())
         return ms.value
 
-    def autotune(self, budget_ms=3.0):
+    def autotune(self, budget_ms=3.0, rounds=0):
         """Autotune the batched descent on the current chains (which it does
         not change): returns (tile, rounds, report), report one dict per
-        candidate with its passes, device ms and exact gain at the budget."""
-        t, r, n = C.c_int(), C.c_uint(), sz()
+        candidate with its passes, device ms and exact gain at the budget.
+        rounds > 0: only the tile is tuned, for that many rounds per pass."""
+        t, r, n = C.c_int(), C.c_uint(rounds), sz()
         rep = np.zeros((64, 5), np.float64)
         if self.lib.am_morph_autotune(self.h, budget_ms, C.byref(t), C.byref(r), rep.ctypes.data_as(vp), 64,
                                       C.byref(n)) != 0:

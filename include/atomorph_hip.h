@@ -193,11 +193,11 @@ int amh_graph_destroy(void *graph);
 
 /* Autotuner of the batched descent (no counterpart in the reference: its
  * descent has no tunable schedule).  Races every kernel variant
- * (amh_refine_tiles) x rounds per pass in {4, 8, 16, 32, 64} from the same
- * start -- a copy of pts_dev in work_dev (same size, distinct) -- for
- * budget_ms of device time each (<= 0: 3 ms), and returns in *tile_out /
- * *rounds_out the candidate with the largest exact cost decrease at the
- * budget.  pts_dev is not modified.  chain_w is the host copy of
+ * (amh_refine_tiles) x rounds per pass in {4, 8, 16, 32, 64} (only
+ * rounds_fixed when it is > 0) from the same start -- a copy of pts_dev in
+ * work_dev (same size, distinct) -- for budget_ms of device time each (<= 0:
+ * 3 ms), and returns in *tile_out / *rounds_out the candidate with the
+ * largest exact cost decrease at the budget.  pts_dev is not modified.  chain_w is the host copy of
  * chain_w_dev.  report (optional, max_report entries) receives every
  * candidate; *nreport their number.  Synchronous; allocates a small
  * scratch buffer. */
@@ -212,8 +212,9 @@ typedef struct amh_tune_entry {
 } amh_tune_entry;
 int amh_refine_autotune(const uint64_t *pts_dev, uint64_t *work_dev, int32_t nkeys, int64_t wtot,
                         const int64_t *chain_off_dev, const int32_t *chain_w_dev, const int32_t *chain_w,
-                        int32_t nchains, double budget_ms, uint64_t seed, int32_t *tile_out, int32_t *rounds_out,
-                        amh_tune_entry *report, int32_t max_report, int32_t *nreport, void *stream);
+                        int32_t nchains, double budget_ms, uint64_t seed, int32_t rounds_fixed, int32_t *tile_out,
+                        int32_t *rounds_out, amh_tune_entry *report, int32_t max_report, int32_t *nreport,
+                        void *stream);
 
 /* ------------------------------------------------------ host colour ----- */
 /* The kernels' colour arithmetic on the host (color.cpp:31-59, 71-178). */

@@ -37,7 +37,7 @@ def test_autotune_races_every_candidate_from_the_same_start():
     budget = 0.3
     G.log(f"autotune wtot={wtot} budget={budget} ms")
     G.check(lib.amh_refine_autotune(G.dptr(dev), G.dptr(work), h, wtot, G.dptr(off_d), G.dptr(w_d),
-                                    w_host.ctypes.data_as(C.c_void_p), len(wid), budget, 4242, C.byref(tile),
+                                    w_host.ctypes.data_as(C.c_void_p), len(wid), budget, 4242, 0, C.byref(tile),
                                     C.byref(rounds), rep, 64, C.byref(n), G.stream()), "amh_refine_autotune")
     G.sync()
     assert np.array_equal(G.to_host(dev, np.uint64), before)        # the caller's chains are untouched
@@ -78,3 +78,7 @@ def test_engine_automatic_schedule_runs_the_tuned_one():
     explicit, _ = ref.chains()
     assert np.array_equal(auto, explicit)
     assert m.cost() == ref.cost()
+    # a fixed rounds count: one candidate per kernel variant, all at it
+    tile16, rounds16, rows16 = m.autotune(1.0, rounds=16)
+    assert rounds16 == 16 and len(rows16) == len(_tiles()) and all(r["rounds"] == 16 for r in rows16)
+    assert tile16 == max(rows16, key=lambda r: r["gain_at_budget"])["tile"]
