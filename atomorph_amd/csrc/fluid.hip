@@ -1199,8 +1199,8 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, Frame F, Part
     }
     __syncthreads();
     const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
+    const int c = block_chain(R.chain_off_dev, R.nchains, a);
     if (a >= R.wtot) return;
-    const int c = find_chain(R.chain_off_dev, R.nchains, a);
     const int nA = surf[int64_t(F.fk) * R.nchains + c], nB = surf[int64_t(F.fnext) * R.nchains + c];
     const int limit = int(round((1.0 - F.tl) * double(nA) + F.tl * double(nB)));
     if (rank[int64_t(F.fk) * R.wtot + a] >= limit) {

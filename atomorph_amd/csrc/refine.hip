@@ -66,14 +66,35 @@ __global__ __launch_bounds__(kCostThreads) void k_chain_cost(const uint64_t *__r
     }
 }
 
+// The counters of a workgroup, summed over its waves first: one set of
+// atomics per workgroup on the three shared counters (device atomics to one
+// address serialise, so a set per wave cost as much as the refinement in a
+// pass).  Every thread of the workgroup calls it.
+template <int NT>
 __device__ __forceinline__ void flush_stats(amh_refine_stats *st, uint64_t att, uint64_t acc, int64_t gain) {
+    __shared__ uint64_t part[NT / 64][3];
     att = wave_sum_u64(att);
     acc = wave_sum_u64(acc);
     gain = wave_sum_i64(gain);
-    if ((threadIdx.x & 63) == 0 && att) {
-        atomicAdd((unsigned long long *)&st->attempts, (unsigned long long)att);
-        atomicAdd((unsigned long long *)&st->accepted, (unsigned long long)acc);
-        atomicAdd((unsigned long long *)&st->gain, (unsigned long long)gain);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[wave][0] = att;
+        part[wave][1] = acc;
+        part[wave][2] = uint64_t(gain);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0, b = 0, g = 0;
+        for (int w = 0; w < NT / 64; ++w) {
+            a += part[w][0];
+            b += part[w][1];
+            g += part[w][2];
+        }
+        if (a) {
+            atomicAdd((unsigned long long *)&st->attempts, (unsigned long long)a);
+            atomicAdd((unsigned long long *)&st->accepted, (unsigned long long)b);
+            atomicAdd((unsigned long long *)&st->gain, (unsigned long long)g);
+        }
     }
 }
 
@@ -113,7 +134,7 @@ __global__ __launch_bounds__(kRoundThreads) void k_refine_round(uint64_t *__rest
             }
         }
     }
-    flush_stats(st, att, acc, gain);
+    flush_stats<kRoundThreads>(st, att, acc, gain);
 }
 
 // ------------------------------------- batched: persistent, LDS resident
@@ -141,8 +162,15 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
     }
     uint64_t att = 0, acc = 0;
     int64_t gain = 0;
-    for (;;) {
-        if (tid == 0) item_s = int(atomicAdd(&queue[0], 1u));
+    // With a workgroup per item the items are taken in launch order;
+    // otherwise they come from the queue, largest first.
+    const bool one_each = nitems <= int(gridDim.x);
+    for (int turn = 0;; ++turn) {
+        if (one_each) {
+            if (tid == 0) item_s = turn == 0 ? int(blockIdx.x) : nitems;
+        } else if (tid == 0) {
+            item_s = int(atomicAdd(&queue[0], 1u));
+        }
         __syncthreads();
         const int it = item_s;
         if (it >= nitems) break;
@@ -194,7 +222,7 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
         }
         __syncthreads();
     }
-    flush_stats(st, att, acc, gain);
+    flush_stats<NT>(st, att, acc, gain);
     if (tid == 0) {
         __threadfence();
         const uint32_t done = atomicAdd(&queue[1], 1u);

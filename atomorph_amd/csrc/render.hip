@@ -47,10 +47,10 @@ __global__ __launch_bounds__(256) void k_prepare(amh_render R, Frame F, Rec *__r
     }
     __syncthreads();
     const int64_t a = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int c = block_chain(R.chain_off_dev, R.nchains, a);
     if (a >= R.wtot) return;
     Rec rec = {kNoPos, 0u, 0u, -1};
     const int W = R.width, H = R.height;
-    const int c = find_chain(R.chain_off_dev, R.nchains, a);
     const int b = R.rank_dev[int64_t(F.fk) * R.nchains + c];
     const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a];
     const uint64_t p2 = R.pts_dev[int64_t(F.fnext) * R.wtot + a];

@@ -173,6 +173,21 @@ __device__ __forceinline__ int find_chain(const int64_t *off, int nchains, int64
     return lo;
 }
 
+// Chain of atom a for a workgroup of consecutive atoms starting at a0 =
+// blockIdx.x * blockDim.x: one binary search per workgroup (for a0), then
+// each thread walks forward over the few chains that start inside the
+// workgroup -- instead of a dependent chain of ~log2(chains) global loads
+// per thread.  Every thread of the workgroup calls it.
+__device__ __forceinline__ int block_chain(const int64_t *off, int nchains, int64_t a) {
+    __shared__ int c0;
+    __syncthreads();  // c0 may still be read from a previous call
+    if (threadIdx.x == 0) c0 = find_chain(off, nchains, int64_t(blockIdx.x) * blockDim.x);
+    __syncthreads();
+    int c = c0;
+    while (c + 1 < nchains && off[c + 1] <= a) ++c;
+    return c;
+}
+
 // Host side: frame scalars of global time t (morph::normalize_time,
 // get_frame_key, morph.cpp:404-464); fk = nkeys when t maps past the keys.
 inline Frame frame_of(double t, int nkeys) {

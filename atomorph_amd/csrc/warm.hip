@@ -78,8 +78,8 @@ __global__ __launch_bounds__(kT) void k_row_stats(amh_render R, RowStat *__restr
 __global__ __launch_bounds__(kT) void k_features(amh_render R, int y, const RowStat *__restrict__ st,
                                                 float *__restrict__ x, float *__restrict__ t) {
     const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
+    const int c = block_chain(R.chain_off_dev, R.nchains, a);
     if (a >= R.wtot) return;
-    const int c = find_chain(R.chain_off_dev, R.nchains, a);
     const int yn = (y + 1) % R.nkeys;
     const uint64_t p = R.pts_dev[int64_t(y) * R.wtot + a];
     const RowStat s = st[int64_t(y) * R.nchains + c];
@@ -118,8 +118,8 @@ __global__ __launch_bounds__(kT) void k_match_keys(amh_render R, int yn, const R
                                                   const float *__restrict__ pred, int cb, int cx,
                                                   uint64_t *__restrict__ key, uint32_t *__restrict__ val) {
     const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
+    const int c = block_chain(R.chain_off_dev, R.nchains, a);
     if (a >= R.wtot) return;
-    const int c = find_chain(R.chain_off_dev, R.nchains, a);
     const RowStat s = st[int64_t(yn) * R.nchains + c];
     const double ux = s.cx + s.s * double(pred[a * 2 + 0]), uy = s.cy + s.s * double(pred[a * 2 + 1]);
     const uint64_t q = R.pts_dev[int64_t(yn) * R.wtot + a];
