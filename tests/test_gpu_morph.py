@@ -204,3 +204,80 @@ def test_chain_checkpoint_round_trip(tmp_path):
     other = _pipeline(synth.ellipse_frames(seed=2, size=64, count=6, nframes=2))
     with pytest.raises(A.AmhError, match="shape|widths"):
         other.load_chains(path)
+
+
+def _edge_frames(case):
+    """Key frames for the edge cases, all on a 64 x 64 canvas."""
+    if case == "three_keys":
+        return synth.ellipse_frames(seed=3, size=64, count=5, nframes=3)
+    if case == "uneven_blob_counts":   # 6 blobs, then 2: chains pad with empty blobs
+        a = synth.ellipse_frames(seed=4, size=64, count=6, nframes=1)[0]
+        b = synth.ellipse_frames(seed=5, size=64, count=2, nframes=1)[0]
+        return [a, b]
+    if case == "empty_key_frame":      # a key frame with no pixel at all
+        a = synth.ellipse_frames(seed=6, size=64, count=4, nframes=1)[0]
+        return [a, np.zeros_like(a), synth.ellipse_frames(seed=7, size=64, count=3, nframes=1)[0]]
+    if case == "ragged_sizes":         # key frames of different sizes
+        a = synth.ellipse_frames(seed=8, size=64, count=5, nframes=1)[0]
+        b = synth.ellipse_frames(seed=9, size=48, count=4, nframes=1)[0][:40]
+        return [a, b]
+    raise KeyError(case)
+
+
+@pytest.mark.parametrize("case", ["three_keys", "uneven_blob_counts", "empty_key_frame", "ragged_sizes"])
+def test_edge_cases_render_like_the_oracle(case):
+    """Through am::morph: rows stay permutations of the built chains while
+    refining, and 12 frames render within 1 LSB of the oracle's rendering of
+    the engine's chains and blob order."""
+    frames = _edge_frames(case)
+    m = M.Morph()
+    m.set(seed=1, cycle_length=1 << 12)
+    for i, f in enumerate(frames):
+        m.add_frame_rgba(i, f)
+    m.set_resolution(64, 64)
+    G.log(f"edge case {case}: pipeline")
+    assert m.run(1) == M.STATE_BLOB_UNIFICATION
+    assert m.run(501) == M.STATE_BLOB_MATCHING
+    m.next_state()
+    assert m.run(1) == M.STATE_ATOM_MORPHING, m.error()
+    pts0, widths = m.chains()
+    h = pts0.shape[0]
+    assert h == len(frames)
+    m.set_refine(M.REFINE_BATCHED, rounds=8, tile=0, graph_passes=0)
+    m.run(3)
+    pts, widths2 = m.chains()
+    assert np.array_equal(widths, widths2)
+    cid = np.repeat(np.arange(len(widths)), widths)
+    for y in range(h):
+        a, b = np.lexsort((pts[y], cid)), np.lexsort((pts0[y], cid))
+        assert np.array_equal(pts[y][a], pts0[y][b]), y
+    # the oracle on the engine's chains: keys on the 64 x 64 canvas
+    canvas = []
+    for f in frames:
+        c = np.zeros((64, 64, 4), np.uint8)
+        c[:f.shape[0], :f.shape[1]] = f
+        canvas.append(c)
+    keys = np.stack([op.ingest(synth.to_words(c))[1] for c in canvas]).view(np.uint8).reshape(h, 64, 64, 4)
+    present = np.any(np.stack([c[..., 3] > 0 for c in canvas]), axis=0)
+    ys, xs = np.nonzero(present)
+    nb = max(m.blob_count(f) for f in range(h))
+    order = np.full((h, nb), -1, np.int32)
+    for f in range(h):
+        for b in range(m.blob_count(f)):
+            order[f, b] = m.blob(f, b)["group"]
+    lag, slope = op.perlin_perm(1), op.perlin_perm(2)
+    cfg = op.RenderCfg(64, 64, h, int(xs.min()), int(ys.min()), int(xs.max()), int(ys.max()), 4, 6, 0, 1, 0, 0, 0,
+                       0, op.ptr(lag), op.ptr(slope))
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]]).astype(np.int64)
+    am = G.atom_major(pts.ravel(), h)
+    w64 = widths.astype(np.int64)
+    out = np.zeros((64, 64, 4), np.uint8)
+    worst = 0
+    for i in range(12):
+        t = m.get_time(i, 12)
+        got = m.render(t)
+        op.oracle().or_render_atoms(C.byref(cfg), op.ptr(keys), op.ptr(am), op.ptr(off), op.ptr(w64), len(widths),
+                                    op.ptr(order), nb, None, float(t), op.ptr(out))
+        worst = max(worst, int(np.abs(got.astype(int) - out.astype(int)).max()))
+    G.log(f"edge case {case}: worst channel difference {worst}")
+    assert worst <= 1
