@@ -290,6 +290,13 @@ struct Bins {
     int32_t *start;     // [tiles + 1]
     int32_t *rank;      // [4][n]: place of particle i's j-th entry in its bin
     int32_t *list;      // [<= 4n] slot | kHome, grouped by bin
+    // Chunks of kT entries, each one workgroup (NULL: one workgroup per
+    // tile): a dense bin is split over several workgroups, which add their
+    // node sums with atomics to nodes cleared when the bins were placed;
+    // a tile with one chunk (every tile has at least one) stores its nodes.
+    int32_t *chunk_start;  // [tiles + 1] first chunk of each tile
+    int32_t *chunk_tile;   // [max_chunks] tile of each chunk
+    int32_t max_chunks;    // launch size of the chunked kernels (an upper bound)
 };
 constexpr int kLdsTiles = 4096;  // up to this many tiles the placing blocks scan the counts in LDS
 
@@ -317,10 +324,21 @@ __device__ __forceinline__ int entry_tile(int j, int tx0, int tx1, int ty0, int 
 struct StencilTiles {  // the step: tiles holding the 3x3 stencil nodes
     Particles P;
     int gsx, gsy;
+    Grid G;
     __device__ __forceinline__ bool operator()(int i, int &tx0, int &tx1, int &ty0, int &ty1) const {
         if (!P.active[i]) return false;
         stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
         return true;
+    }
+    // Zeroes every field of tile t's nodes (the workgroup together).
+    __device__ __forceinline__ void clear_tile(int t, int nty) const {
+        const int x0 = (t / nty) * kTile, y0 = (t % nty) * kTile;
+        float *f[kGridFields] = {G.m, G.gx, G.gy, G.u, G.v, G.ax, G.ay, G.r, G.g, G.b, G.a, G.w};
+        for (int e = threadIdx.x; e < kTile * kTile * kGridFields; e += kT) {
+            const int node = e % (kTile * kTile), field = e / (kTile * kTile);
+            const int ni = x0 + node / kTile, nj = y0 + node % kTile;
+            if (ni < gsx && nj < gsy) f[field][int64_t(ni) * gsy + nj] = 0.0f;
+        }
     }
 };
 
@@ -331,6 +349,7 @@ constexpr uint32_t kNoTap = 0xFFFFFFFFu;  // splat: particle draws nothing (X, Y
 struct SplatTiles {  // the splat: pixel tiles holding the 2x2 taps at (X, Y)
     const uint32_t *xy;  // X | Y << 16, kNoTap = nothing to draw
     int W, H;
+    __device__ __forceinline__ void clear_tile(int, int) const {}  // not chunked
     __device__ __forceinline__ bool operator()(int i, int &tx0, int &tx1, int &ty0, int &ty1) const {
         const uint32_t v = xy[i];
         if (v == kNoTap) return false;
@@ -423,6 +442,32 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
         if (blockIdx.x == 0)
             for (int k = threadIdx.x; k <= tiles; k += kT) B.start[k] = sstart[k];
         start = sstart;
+        if (B.chunk_start && blockIdx.x == 0) {
+            // chunks per tile max(1, ceil(count / kT)), scanned like the counts
+            int csum = 0;
+            for (int k = lo; k < hi; ++k) csum += max(1, (sstart[k + 1] - sstart[k] + kT - 1) / kT);
+            int cincl = csum;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(cincl, o, 64);
+                if (lane >= o) cincl += v;
+            }
+            __syncthreads();
+            if (lane == 63) wsum[wave] = cincl;
+            __syncthreads();
+            int c0 = cincl - csum;
+            for (int w = 0; w < wave; ++w) c0 += wsum[w];
+            for (int k = lo; k < hi; ++k) {
+                const int nc = max(1, (sstart[k + 1] - sstart[k] + kT - 1) / kT);
+                B.chunk_start[k] = c0;
+                for (int j = 0; j < nc; ++j) B.chunk_tile[c0 + j] = k;
+                c0 += nc;
+            }
+            if (threadIdx.x == kT - 1) B.chunk_start[tiles] = c0;
+        }
+        if (B.chunk_start) {  // nodes of tiles split over chunks start from zero
+            for (int t = blockIdx.x; t < tiles; t += gridDim.x)
+                if (sstart[t + 1] - sstart[t] > kT) tiles_of.clear_tile(t, B.nty);
+        }
     }
     for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) B.count[par ^ 1][k] = 0;
     const int i = blockIdx.x * kT + threadIdx.x;
@@ -440,8 +485,7 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
 // chunk is processed, so a dense bin overlaps its memory latency.  Trip
 // counts are uniform over the block.
 template <class Fetch, class Body>
-__device__ __forceinline__ void for_bin(const Bins &B, int t, const Fetch &fetch, const Body &body) {
-    const int s0 = B.start[t], s1 = B.start[t + 1];
+__device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fetch &fetch, const Body &body) {
     auto entry = [&](int base, int &slot, bool &home) {
         const int k = base + int(threadIdx.x);
         slot = -1;
@@ -467,6 +511,35 @@ __device__ __forceinline__ void for_bin(const Bins &B, int t, const Fetch &fetch
         home = nhome;
         item = next;
     }
+}
+
+// The entries [s0, s1) of tile t this workgroup handles -- its chunk when
+// B is chunked, else the whole bin of tile blockIdx.x -- and whether it is
+// the tile's only workgroup (then it stores its nodes; otherwise it adds
+// them atomically).  false: a workgroup past the last chunk (nothing to do;
+// block-uniform).
+__device__ __forceinline__ bool block_work(const Bins &B, int &t, int &s0, int &s1, bool &single) {
+    if (!B.chunk_start) {
+        t = int(blockIdx.x);
+        s0 = B.start[t];
+        s1 = B.start[t + 1];
+        single = true;
+        return true;
+    }
+    const int tiles = B.ntx * B.nty;
+    if (int(blockIdx.x) >= B.chunk_start[tiles]) return false;
+    t = B.chunk_tile[blockIdx.x];
+    const int c0 = B.chunk_start[t], nc = B.chunk_start[t + 1] - c0;
+    single = nc == 1;
+    const int b1 = B.start[t + 1];
+    s0 = B.start[t] + (int(blockIdx.x) - c0) * kT;
+    s1 = single ? b1 : min(b1, s0 + kT);
+    return true;
+}
+
+__device__ __forceinline__ void put_node(float *f, int64_t k, float v, bool single) {
+    if (single) f[k] = v;
+    else if (v != 0.0f) atomicAdd(&f[k], v);
 }
 
 __device__ __forceinline__ bool in_reach(int cx, int cy, int x0, int y0) {
@@ -508,7 +581,10 @@ struct P2gItem {
 // tile's nodes from the moments of the cells that reach them.
 __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float cs[kCellSums][kReach * kReach];
-    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
+    int tile, s0, s1;
+    bool single;
+    if (!block_work(B, tile, s0, s1, single)) return;
+    const int tx = tile / B.nty, ty = tile % B.nty;
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
     __syncthreads();
@@ -521,7 +597,7 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
         }
         return it;
     };
-    for_bin(B, int(blockIdx.x), fetch, [&](int slot, bool, const P2gItem &it) {
+    for_bin(B, s0, s1, fetch, [&](int slot, bool, const P2gItem &it) {
         int cx = 0, cy = 0;
         bool valid = false;
         float c[kCellSums];
@@ -569,8 +645,8 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
                 for (int f = 0; f < 8; ++f) acc[f] += v[f];
             }
         const int64_t k = int64_t(ni) * gsy + nj;
-        Gd.m[k] = acc[0]; Gd.gx[k] = acc[1]; Gd.gy[k] = acc[2];
-        Gd.r[k] = acc[3]; Gd.g[k] = acc[4]; Gd.b[k] = acc[5]; Gd.a[k] = acc[6]; Gd.w[k] = acc[7];
+        float *f[8] = {Gd.m, Gd.gx, Gd.gy, Gd.r, Gd.g, Gd.b, Gd.a, Gd.w};
+        for (int q = 0; q < 8; ++q) put_node(f[q], k, acc[q], single);
     }
 }
 
@@ -589,13 +665,13 @@ __device__ __forceinline__ void tile_add2(float (&acc)[2][kTile * kTile], const 
 }
 
 __device__ __forceinline__ void store_tile2(const float (&acc)[2][kTile * kTile], float *f0, float *f1, int x0, int y0,
-                                            int gsx, int gsy) {
+                                            int gsx, int gsy, bool single) {
     for (int t = threadIdx.x; t < kTile * kTile; t += kT) {
         const int ni = x0 + t / kTile, nj = y0 + t % kTile;
         if (ni >= gsx || nj >= gsy) continue;
         const int64_t k = int64_t(ni) * gsy + nj;
-        f0[k] = acc[0][t];
-        f1[k] = acc[1][t];
+        put_node(f0, k, acc[0][t], single);
+        put_node(f1, k, acc[1][t], single);
     }
 }
 
@@ -603,7 +679,10 @@ __device__ __forceinline__ void store_tile2(const float (&acc)[2][kTile * kTile]
 __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float acc[2][kTile * kTile];
     __shared__ float nodes[3][kHalo * kHalo];  // m, gx, gy
-    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
+    int tile, s0, s1;
+    bool single;
+    if (!block_work(B, tile, s0, s1, single)) return;
+    const int tx = tile / B.nty, ty = tile % B.nty;
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
     stage_halo(nodes, x0, y0, gsx, gsy, [&](int f, int64_t k) { return f == 0 ? Gd.m[k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
@@ -614,7 +693,7 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
         if (slot >= 0) it = double2{P.x[slot], P.y[slot]};
         return it;
     };
-    for_bin(B, int(blockIdx.x), fetch, [&](int slot, bool, const double2 &it) {
+    for_bin(B, s0, s1, fetch, [&](int slot, bool, const double2 &it) {
         int cx = 0, cy = 0;
         bool valid = false;
         double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0};
@@ -635,7 +714,7 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
             }
     });
     __syncthreads();
-    store_tile2(acc, Gd.ax, Gd.ay, x0, y0, gsx, gsy);
+    store_tile2(acc, Gd.ax, Gd.ay, x0, y0, gsx, gsy, single);
 }
 
 struct G2pItem {
@@ -649,7 +728,10 @@ struct G2pItem {
 __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float acc[2][kTile * kTile];
     __shared__ float accel[2][kHalo * kHalo];  // node accelerations per mass
-    const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
+    int tile, s0, s1;
+    bool single;
+    if (!block_work(B, tile, s0, s1, single)) return;
+    const int tx = tile / B.nty, ty = tile % B.nty;
     const int x0 = tx * kTile, y0 = ty * kTile;
     for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
     stage_halo(accel, x0, y0, gsx, gsy, [&](int f, int64_t k) { return per_mass(f == 0 ? Gd.ax[k] : Gd.ay[k], Gd.m[k]); });
@@ -662,7 +744,7 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
         }
         return it;
     };
-    for_bin(B, int(blockIdx.x), fetch, [&](int slot, bool home, const G2pItem &it) {
+    for_bin(B, s0, s1, fetch, [&](int slot, bool home, const G2pItem &it) {
         int cx = 0, cy = 0;
         bool valid = false;
         double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3], gy[3];
@@ -712,7 +794,7 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
             }
     });
     __syncthreads();
-    store_tile2(acc, Gd.u, Gd.v, x0, y0, gsx, gsy);
+    store_tile2(acc, Gd.u, Gd.v, x0, y0, gsx, gsy, single);
 }
 
 __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
@@ -926,6 +1008,10 @@ struct amh_fluid_s {
 };
 
 namespace {
+// Upper bound of the chunks: every tile has at least one, and a particle
+// has at most 4 entries.
+size_t max_chunks(size_t tiles, size_t n) { return tiles + (4 * n + kT - 1) / kT + 1; }
+
 size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     const size_t n = size_t(count), nn = size_t(gsx) * gsy, npx = size_t(W) * H;
     const size_t nb = size_t(scan::blocks(int64_t(nn) + kIdleBuckets));
@@ -934,6 +1020,7 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     (void)nkeys;
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
            (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) +
+           (tiles + 1 + max_chunks(tiles, n)) * 4 + 2 * 256 +
            nn * 4 * kGridFields + npx * 4 * 2 + 64 * 256;
 }
 
@@ -980,6 +1067,9 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->bins.start = reinterpret_cast<int32_t *>(take((tiles + 1) * 4));
     f->bins.rank = reinterpret_cast<int32_t *>(take(n * 4 * 4));
     f->bins.list = reinterpret_cast<int32_t *>(take(n * 4 * 4));
+    f->bins.max_chunks = int32_t(max_chunks(tiles, n));
+    f->bins.chunk_start = reinterpret_cast<int32_t *>(take((tiles + 1) * 4));
+    f->bins.chunk_tile = reinterpret_cast<int32_t *>(take(size_t(f->bins.max_chunks) * 4));
     f->pbins.ntx = (width + kPxTile - 1) / kPxTile;
     f->pbins.nty = (height + kPxTile - 1) / kPxTile;
     const size_t ptiles = size_t(f->pbins.ntx) * size_t(f->pbins.nty);
@@ -988,6 +1078,8 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->pbins.start = reinterpret_cast<int32_t *>(take((ptiles + 1) * 4));
     f->pbins.rank = f->bins.rank;  // the splat bins after the last step of a frame
     f->pbins.list = f->bins.list;
+    f->pbins.chunk_start = f->pbins.chunk_tile = nullptr;  // one workgroup per pixel tile
+    f->pbins.max_chunks = 0;
     f->sxy = reinterpret_cast<uint32_t *>(take(n * 4));
     f->staps = reinterpret_cast<double *>(take(n * 32));
     f->scol = reinterpret_cast<uint32_t *>(take(n * 4));
@@ -1114,25 +1206,30 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
                   bool count_next = false) {
     amh::ProfScope step_("fluid_step", s);
     if (f->count == 0) return AMH_OK;
-    const Bins &B = f->bins;
-    const int tiles = B.ntx * B.nty;
+    const int tiles = f->bins.ntx * f->bins.nty;
+    // dense bins are split over workgroups when the placing blocks can scan
+    // the counts in LDS (where the chunks are laid out)
+    Bins B = f->bins;
+    const bool chunked = tiles <= kLdsTiles;
+    if (!chunked) B.chunk_start = B.chunk_tile = nullptr;
+    const uint32_t grid = uint32_t(chunked ? B.max_chunks : tiles);
     {
         amh::ProfScope ps_("fluid_bins", s);
         const int par = f->bin_par;
         f->bin_par ^= 1;
-        bin_particles(StencilTiles{Q, f->gsx, f->gsy}, f->count, B, tiles, par, s, counted);
+        bin_particles(StencilTiles{Q, f->gsx, f->gsy, f->G}, f->count, B, tiles, par, s, counted);
     }
     {
         amh::ProfScope ps_("k_p2g", s);
-        hipLaunchKernelGGL(k_p2g, dim3(uint32_t(tiles)), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+        hipLaunchKernelGGL(k_p2g, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     }
     {
         amh::ProfScope ps_("k_forces", s);
-        hipLaunchKernelGGL(k_forces, dim3(uint32_t(tiles)), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+        hipLaunchKernelGGL(k_forces, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     }
     {
         amh::ProfScope ps_("k_g2p", s);
-        hipLaunchKernelGGL(k_g2p, dim3(uint32_t(tiles)), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+        hipLaunchKernelGGL(k_g2p, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     }
     {
         amh::ProfScope ps_("k_advect", s);
@@ -1338,7 +1435,7 @@ __global__ __launch_bounds__(kT) void k_splat_tile(Bins B, const uint32_t *__res
         }
         return it;
     };
-    for_bin(B, int(blockIdx.x), fetch, [&](int slot, bool, const SplatItem &it) {
+    for_bin(B, B.start[blockIdx.x], B.start[blockIdx.x + 1], fetch, [&](int slot, bool, const SplatItem &it) {
         if (slot < 0 || it.xy == kNoTap) return;
         const int X = int(it.xy & 0xFFFFu), Y = int(it.xy >> 16);
         for (int q = 0; q < 4; ++q) {
