@@ -294,9 +294,11 @@ struct Bins {
     // tile): a dense bin is split over several workgroups, which add their
     // node sums with atomics to nodes cleared when the bins were placed;
     // a tile with one chunk (every tile has at least one) stores its nodes.
-    int32_t *chunk_start;  // [tiles + 1] first chunk of each tile
-    int32_t *chunk_tile;   // [max_chunks] tile of each chunk
-    int32_t max_chunks;    // launch size of the chunked kernels (an upper bound)
+    // chunk[c] = (tile, first entry, end entry, only chunk of its tile),
+    // c < *nchunks.
+    int4 *chunk;
+    int32_t *nchunks;      // [1] chunks written by the last placing
+    int32_t max_chunks;    // capacity of chunk (an upper bound of *nchunks)
 };
 constexpr int kLdsTiles = 4096;  // up to this many tiles the placing blocks scan the counts in LDS
 
@@ -442,7 +444,7 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
         if (blockIdx.x == 0)
             for (int k = threadIdx.x; k <= tiles; k += kT) B.start[k] = sstart[k];
         start = sstart;
-        if (B.chunk_start && blockIdx.x == 0) {
+        if (B.chunk && blockIdx.x == 0) {
             // chunks per tile max(1, ceil(count / kT)), scanned like the counts
             int csum = 0;
             for (int k = lo; k < hi; ++k) csum += max(1, (sstart[k + 1] - sstart[k] + kT - 1) / kT);
@@ -457,14 +459,15 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
             int c0 = cincl - csum;
             for (int w = 0; w < wave; ++w) c0 += wsum[w];
             for (int k = lo; k < hi; ++k) {
-                const int nc = max(1, (sstart[k + 1] - sstart[k] + kT - 1) / kT);
-                B.chunk_start[k] = c0;
-                for (int j = 0; j < nc; ++j) B.chunk_tile[c0 + j] = k;
+                const int b0 = sstart[k], b1 = sstart[k + 1];
+                const int nc = max(1, (b1 - b0 + kT - 1) / kT);
+                for (int j = 0; j < nc; ++j)
+                    B.chunk[c0 + j] = make_int4(k, b0 + j * kT, nc == 1 ? b1 : min(b1, b0 + (j + 1) * kT), nc == 1);
                 c0 += nc;
             }
-            if (threadIdx.x == kT - 1) B.chunk_start[tiles] = c0;
+            if (threadIdx.x == kT - 1) *B.nchunks = c0;
         }
-        if (B.chunk_start) {  // nodes of tiles split over chunks start from zero
+        if (B.chunk) {  // nodes of tiles split over chunks start from zero
             for (int t = blockIdx.x; t < tiles; t += gridDim.x)
                 if (sstart[t + 1] - sstart[t] > kT) tiles_of.clear_tile(t, B.nty);
         }
@@ -479,13 +482,23 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
     }
 }
 
-// Calls body(slot, home, item) for the entries of bin t, kT at a time (slot
+// Calls body(slot, home, item) for the entries [s0, s1), kT at a time (slot
 // -1 past the end), item = fetch(slot) holding the particle fields the body
-// reads.  The next chunk's entries and fields are loaded before the current
-// chunk is processed, so a dense bin overlaps its memory latency.  Trip
-// counts are uniform over the block.
-template <class Fetch, class Body>
-__device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fetch &fetch, const Body &body) {
+// reads.  kPrefetch: the next kT entries and their fields are loaded before
+// the current ones are processed, so a whole dense bin overlaps its memory
+// latency; without it (work items of at most kT entries, where the loads of
+// the next item belong to another workgroup) the registers of the second
+// item are saved.  Trip counts are uniform over the block.
+struct NoPrologue {
+    __device__ void operator()() const {}
+};
+
+// prologue() runs once the first chunk's loads are issued and before it is
+// processed, so its own memory traffic (a node halo staged in LDS, with the
+// barrier that publishes it) overlaps them.
+template <bool kPrefetch = true, class Fetch, class Body, class Prologue = NoPrologue>
+__device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fetch &fetch, const Body &body,
+                                        const Prologue &prologue = Prologue()) {
     auto entry = [&](int base, int &slot, bool &home) {
         const int k = base + int(threadIdx.x);
         slot = -1;
@@ -496,10 +509,23 @@ __device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fet
             home = e < 0;
         }
     };
+    if constexpr (!kPrefetch) {
+        if (s0 >= s1) prologue();
+        for (int base = s0; base < s1; base += kT) {
+            int slot;
+            bool home;
+            entry(base, slot, home);
+            const auto item = fetch(slot);
+            if (base == s0) prologue();
+            body(slot, home, item);
+        }
+        return;
+    }
     int slot, nslot = -1;
     bool home, nhome = false;
     entry(s0, slot, home);
     auto item = fetch(slot);
+    prologue();
     for (int base = s0; base < s1; base += kT) {
         auto next = item;
         if (base + kT < s1) {
@@ -513,27 +539,28 @@ __device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fet
     }
 }
 
-// The entries [s0, s1) of tile t this workgroup handles -- its chunk when
-// B is chunked, else the whole bin of tile blockIdx.x -- and whether it is
-// the tile's only workgroup (then it stores its nodes; otherwise it adds
-// them atomically).  false: a workgroup past the last chunk (nothing to do;
-// block-uniform).
-__device__ __forceinline__ bool block_work(const Bins &B, int &t, int &s0, int &s1, bool &single) {
-    if (!B.chunk_start) {
-        t = int(blockIdx.x);
+// Work item b of a tile kernel: the entries [s0, s1) of tile t -- chunk b
+// when B is chunked, else the whole bin of tile b -- and whether it is the
+// tile's only work item (then it stores its nodes; otherwise it adds them
+// atomically).  false: past the last item.  Workgroups take items b =
+// blockIdx.x, blockIdx.x + gridDim.x, ...; nchunks = *B.nchunks when B is
+// chunked.
+__device__ __forceinline__ bool block_work(const Bins &B, int nchunks, int b, int &t, int &s0, int &s1,
+                                           bool &single) {
+    if (!B.chunk) {
+        if (b >= B.ntx * B.nty) return false;
+        t = b;
         s0 = B.start[t];
         s1 = B.start[t + 1];
         single = true;
         return true;
     }
-    const int tiles = B.ntx * B.nty;
-    if (int(blockIdx.x) >= B.chunk_start[tiles]) return false;
-    t = B.chunk_tile[blockIdx.x];
-    const int c0 = B.chunk_start[t], nc = B.chunk_start[t + 1] - c0;
-    single = nc == 1;
-    const int b1 = B.start[t + 1];
-    s0 = B.start[t] + (int(blockIdx.x) - c0) * kT;
-    s1 = single ? b1 : min(b1, s0 + kT);
+    if (b >= nchunks) return false;
+    const int4 c = B.chunk[b];
+    t = c.x;
+    s0 = c.y;
+    s1 = c.z;
+    single = c.w != 0;
     return true;
 }
 
@@ -579,74 +606,91 @@ struct P2gItem {
 
 // Mass (= density), density gradient and strength-weighted colour of the
 // tile's nodes from the moments of the cells that reach them.
+template <bool kChunked>
 __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float cs[kCellSums][kReach * kReach];
-    int tile, s0, s1;
-    bool single;
-    if (!block_work(B, tile, s0, s1, single)) return;
-    const int tx = tile / B.nty, ty = tile % B.nty;
-    const int x0 = tx * kTile, y0 = ty * kTile;
-    for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
-    __syncthreads();
-    auto fetch = [&](int slot) {
-        P2gItem it = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (slot >= 0) {
-            it.x = P.x[slot]; it.y = P.y[slot]; it.s = P.strength[slot];
-            it.r = P.r[slot]; it.g = P.g[slot]; it.b = P.b[slot]; it.a = P.a[slot];
-            it.mature = P.mature[slot];
-        }
-        return it;
-    };
-    for_bin(B, s0, s1, fetch, [&](int slot, bool, const P2gItem &it) {
-        int cx = 0, cy = 0;
-        bool valid = false;
-        float c[kCellSums];
-        for (int k = 0; k < kCellSums; ++k) c[k] = 0.0f;
-        if (slot >= 0) {
-            cx = static_cast<int>(it.x - 0.5);
-            cy = static_cast<int>(it.y - 0.5);
-            valid = in_reach(cx, cy, x0, y0);
-            if (valid) {
-                const double e = it.x - cx - 1.0, h = it.y - cy - 1.0;
-                const double ep[3] = {1.0, e, e * e}, hp[3] = {1.0, h, h * h};
-                for (int p = 0; p < 3; ++p)
-                    for (int q = 0; q < 3; ++q) c[p * 3 + q] = float(ep[p] * hp[q]);
-                if (it.mature && it.s > 0.0) {
-                    // strength-weighted running mean == sum(s c) / sum(s)
-                    c[9] = float(it.s * it.r);
-                    c[10] = float(it.s * it.g);
-                    c[11] = float(it.s * it.b);
-                    c[12] = float(it.s * it.a);
-                    c[13] = float(it.s);
+    const int nchunks = kChunked ? *B.nchunks : 0;
+    for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
+        int tile, s0, s1;
+        bool single;
+        if (!block_work(B, nchunks, item, tile, s0, s1, single)) break;
+        const int tx = tile / B.nty, ty = tile % B.nty;
+        const int x0 = tx * kTile, y0 = ty * kTile;
+        for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
+        __syncthreads();
+        auto fetch = [&](int slot) {
+            P2gItem it = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (slot >= 0) {
+                it.x = P.x[slot]; it.y = P.y[slot]; it.s = P.strength[slot];
+                it.r = P.r[slot]; it.g = P.g[slot]; it.b = P.b[slot]; it.a = P.a[slot];
+                it.mature = P.mature[slot];
+            }
+            return it;
+        };
+        for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool, const P2gItem &it) {
+            int cx = 0, cy = 0;
+            bool valid = false;
+            float c[kCellSums];
+            for (int k = 0; k < kCellSums; ++k) c[k] = 0.0f;
+            if (slot >= 0) {
+                cx = static_cast<int>(it.x - 0.5);
+                cy = static_cast<int>(it.y - 0.5);
+                valid = in_reach(cx, cy, x0, y0);
+                if (valid) {
+                    const double e = it.x - cx - 1.0, h = it.y - cy - 1.0;
+                    const double ep[3] = {1.0, e, e * e}, hp[3] = {1.0, h, h * h};
+                    for (int p = 0; p < 3; ++p)
+                        for (int q = 0; q < 3; ++q) c[p * 3 + q] = float(ep[p] * hp[q]);
+                    if (it.mature && it.s > 0.0) {
+                        // strength-weighted running mean == sum(s c) / sum(s)
+                        c[9] = float(it.s * it.r);
+                        c[10] = float(it.s * it.g);
+                        c[11] = float(it.s * it.b);
+                        c[12] = float(it.s * it.a);
+                        c[13] = float(it.s);
+                    }
                 }
             }
-        }
-        const Seg S = segments(valid, cx, cy);
-        for (int k = 0; k < kCellSums; ++k) c[k] = seg_sum(c[k], S);
-        if (valid && S.tail) {
-            const int idx = (cx - x0 + 2) * kReach + (cy - y0 + 2);
-            for (int k = 0; k < kCellSums; ++k)
-                if (c[k] != 0.0f) atomicAdd(&cs[k][idx], c[k]);
-        }
-    });
-    __syncthreads();
-    for (int t = threadIdx.x; t < kTile * kTile; t += kT) {
-        const int ni = x0 + t / kTile, nj = y0 + t % kTile;
-        if (ni >= gsx || nj >= gsy) continue;
-        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) {
-                const int cell = (ni - a - x0 + 2) * kReach + (nj - b - y0 + 2);
-                if (cs[0][cell] == 0.0f) continue;  // no particle (each adds 1 to M[0][0])
-                float m[kCellSums];
-                for (int k = 0; k < kCellSums; ++k) m[k] = cs[k][cell];
-                float v[8];
-                cell_to_node(m, a, b, v);
-                for (int f = 0; f < 8; ++f) acc[f] += v[f];
+            const Seg S = segments(valid, cx, cy);
+            for (int k = 0; k < kCellSums; ++k) c[k] = seg_sum(c[k], S);
+            if (valid && S.tail) {
+                const int idx = (cx - x0 + 2) * kReach + (cy - y0 + 2);
+                for (int k = 0; k < kCellSums; ++k)
+                    if (c[k] != 0.0f) atomicAdd(&cs[k][idx], c[k]);
             }
-        const int64_t k = int64_t(ni) * gsy + nj;
-        float *f[8] = {Gd.m, Gd.gx, Gd.gy, Gd.r, Gd.g, Gd.b, Gd.a, Gd.w};
-        for (int q = 0; q < 8; ++q) put_node(f[q], k, acc[q], single);
+        });
+        __syncthreads();
+        // Nodes from the cells: thread (a, node) sums the three cells
+        // (node - (a, b)), then one thread per node adds the three partials.
+        __shared__ float part[3][8][kTile * kTile];
+        static_assert(3 * kTile * kTile <= kT, "one thread per (a, node)");
+        {
+            const int t = threadIdx.x, node = t % (kTile * kTile), a = t / (kTile * kTile);
+            if (a < 3) {
+                const int ni = x0 + node / kTile, nj = y0 + node % kTile;
+                float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                for (int b = 0; b < 3; ++b) {
+                    const int cell = (ni - a - x0 + 2) * kReach + (nj - b - y0 + 2);
+                    if (cs[0][cell] == 0.0f) continue;  // no particle (each adds 1 to M[0][0])
+                    float m[kCellSums];
+                    for (int k = 0; k < kCellSums; ++k) m[k] = cs[k][cell];
+                    float v[8];
+                    cell_to_node(m, a, b, v);
+                    for (int f = 0; f < 8; ++f) acc[f] += v[f];
+                }
+                for (int f = 0; f < 8; ++f) part[a][f][node] = acc[f];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kTile * kTile) {
+            const int node = threadIdx.x, ni = x0 + node / kTile, nj = y0 + node % kTile;
+            if (ni < gsx && nj < gsy) {
+                const int64_t k = int64_t(ni) * gsy + nj;
+                float *f[8] = {Gd.m, Gd.gx, Gd.gy, Gd.r, Gd.g, Gd.b, Gd.a, Gd.w};
+                for (int q = 0; q < 8; ++q) put_node(f[q], k, part[0][q][node] + part[1][q][node] + part[2][q][node], single);
+            }
+        }
+        __syncthreads();  // the LDS sums are reused by the next item
     }
 }
 
@@ -676,45 +720,53 @@ __device__ __forceinline__ void store_tile2(const float (&acc)[2][kTile * kTile]
 }
 
 // Node accelerations: pressure and wall forces (fluidmodel.cpp:251-357).
+template <bool kChunked>
 __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float acc[2][kTile * kTile];
     __shared__ float nodes[3][kHalo * kHalo];  // m, gx, gy
-    int tile, s0, s1;
-    bool single;
-    if (!block_work(B, tile, s0, s1, single)) return;
-    const int tx = tile / B.nty, ty = tile % B.nty;
-    const int x0 = tx * kTile, y0 = ty * kTile;
-    for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
-    stage_halo(nodes, x0, y0, gsx, gsy, [&](int f, int64_t k) { return f == 0 ? Gd.m[k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
-    __syncthreads();
-    const auto node = [&](int f, int i, int j) { return nodes[f][halo_index(i, j, x0, y0)]; };
-    auto fetch = [&](int slot) {
-        double2 it = {0.0, 0.0};
-        if (slot >= 0) it = double2{P.x[slot], P.y[slot]};
-        return it;
-    };
-    for_bin(B, s0, s1, fetch, [&](int slot, bool, const double2 &it) {
-        int cx = 0, cy = 0;
-        bool valid = false;
-        double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0};
-        double pressure = 0.0, fx = 0.0, fy = 0.0;
-        if (slot >= 0) {
-            const double x = it.x, y = it.y;
-            weights(x, y, cx, cy, px, py, gx, gy);
-            valid = in_reach(cx, cy, x0, y0);
-            if (valid) particle_forces(x, y, node, gsx, gsy, pressure, fx, fy);
-        }
-        const Seg S = segments(valid, cx, cy);
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) {
-                const double phi = px[a] * py[b];
-                const float v0 = valid ? float(-((gx[a] * py[b]) * pressure) + fx * phi) : 0.0f;
-                const float v1 = valid ? float(-((px[a] * gy[b]) * pressure) + fy * phi) : 0.0f;
-                tile_add2(acc, S, valid, cx, cy, x0, y0, a, b, v0, v1);
+    const int nchunks = kChunked ? *B.nchunks : 0;
+    for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
+        int tile, s0, s1;
+        bool single;
+        if (!block_work(B, nchunks, item, tile, s0, s1, single)) break;
+        const int tx = tile / B.nty, ty = tile % B.nty;
+        const int x0 = tx * kTile, y0 = ty * kTile;
+        for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
+        const auto stage = [&] {
+            stage_halo(nodes, x0, y0, gsx, gsy,
+                       [&](int f, int64_t k) { return f == 0 ? Gd.m[k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
+            __syncthreads();
+        };
+        const auto node = [&](int f, int i, int j) { return nodes[f][halo_index(i, j, x0, y0)]; };
+        auto fetch = [&](int slot) {
+            double2 it = {0.0, 0.0};
+            if (slot >= 0) it = double2{P.x[slot], P.y[slot]};
+            return it;
+        };
+        for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool, const double2 &it) {
+            int cx = 0, cy = 0;
+            bool valid = false;
+            double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0};
+            double pressure = 0.0, fx = 0.0, fy = 0.0;
+            if (slot >= 0) {
+                const double x = it.x, y = it.y;
+                weights(x, y, cx, cy, px, py, gx, gy);
+                valid = in_reach(cx, cy, x0, y0);
+                if (valid) particle_forces(x, y, node, gsx, gsy, pressure, fx, fy);
             }
-    });
-    __syncthreads();
-    store_tile2(acc, Gd.ax, Gd.ay, x0, y0, gsx, gsy, single);
+            const Seg S = segments(valid, cx, cy);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double phi = px[a] * py[b];
+                    const float v0 = valid ? float(-((gx[a] * py[b]) * pressure) + fx * phi) : 0.0f;
+                    const float v1 = valid ? float(-((px[a] * gy[b]) * pressure) + fy * phi) : 0.0f;
+                    tile_add2(acc, S, valid, cx, cy, x0, y0, a, b, v0, v1);
+                }
+        }, stage);
+        __syncthreads();
+        store_tile2(acc, Gd.ax, Gd.ay, x0, y0, gsx, gsy, single);
+        __syncthreads();  // the LDS sums are reused by the next item
+    }
 }
 
 struct G2pItem {
@@ -725,76 +777,84 @@ struct G2pItem {
 // Particle velocity from node accelerations plus the attractor (written by
 // the particle's home tile), then the momentum of the tile's nodes
 // (fluidmodel.cpp:359-437).
+template <bool kChunked>
 __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float acc[2][kTile * kTile];
     __shared__ float accel[2][kHalo * kHalo];  // node accelerations per mass
-    int tile, s0, s1;
-    bool single;
-    if (!block_work(B, tile, s0, s1, single)) return;
-    const int tx = tile / B.nty, ty = tile % B.nty;
-    const int x0 = tx * kTile, y0 = ty * kTile;
-    for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
-    stage_halo(accel, x0, y0, gsx, gsy, [&](int f, int64_t k) { return per_mass(f == 0 ? Gd.ax[k] : Gd.ay[k], Gd.m[k]); });
-    __syncthreads();
-    auto fetch = [&](int slot) {
-        G2pItem it = {0, 0, 0, 0, 0, 0, 0};
-        if (slot >= 0) {
-            it.x = P.x[slot]; it.y = P.y[slot]; it.gx = P.gx[slot]; it.gy = P.gy[slot];
-            it.u = P.u[slot]; it.v = P.v[slot]; it.mature = P.mature[slot];
-        }
-        return it;
-    };
-    for_bin(B, s0, s1, fetch, [&](int slot, bool home, const G2pItem &it) {
-        int cx = 0, cy = 0;
-        bool valid = false;
-        double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3], gy[3];
-        double mu = 0.0, mv = 0.0;
-        if (slot >= 0) {
-            const double x = it.x, y = it.y;
-            weights(x, y, cx, cy, px, py, gx, gy);
-            valid = in_reach(cx, cy, x0, y0);
-            if (valid || home) {
-                double cax, cay;  // attractor, 0.03 per step (fluidmodel.cpp:383-410)
-                toward(x, y, it.gx, it.gy, 0.03, cax, cay);
-                double u = it.u, v = it.v;
-                for (int a = 0; a < 3; ++a)
-                    for (int b = 0; b < 3; ++b) {
-                        const double phi = px[a] * py[b];
-                        double ax, ay;
-                        if (valid) {  // in reach: the node is staged
-                            const int e = halo_index(cx + a, cy + b, x0, y0);
-                            ax = accel[0][e];
-                            ay = accel[1][e];
-                        } else {
-                            ax = nd_m(Gd.ax, Gd.m, cx + a, cy + b, gsx, gsy);
-                            ay = nd_m(Gd.ay, Gd.m, cx + a, cy + b, gsx, gsy);
+    const int nchunks = kChunked ? *B.nchunks : 0;
+    for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
+        int tile, s0, s1;
+        bool single;
+        if (!block_work(B, nchunks, item, tile, s0, s1, single)) break;
+        const int tx = tile / B.nty, ty = tile % B.nty;
+        const int x0 = tx * kTile, y0 = ty * kTile;
+        for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
+        const auto stage = [&] {
+            stage_halo(accel, x0, y0, gsx, gsy,
+                       [&](int f, int64_t k) { return per_mass(f == 0 ? Gd.ax[k] : Gd.ay[k], Gd.m[k]); });
+            __syncthreads();
+        };
+        auto fetch = [&](int slot) {
+            G2pItem it = {0, 0, 0, 0, 0, 0, 0};
+            if (slot >= 0) {
+                it.x = P.x[slot]; it.y = P.y[slot]; it.gx = P.gx[slot]; it.gy = P.gy[slot];
+                it.u = P.u[slot]; it.v = P.v[slot]; it.mature = P.mature[slot];
+            }
+            return it;
+        };
+        for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool home, const G2pItem &it) {
+            int cx = 0, cy = 0;
+            bool valid = false;
+            double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3], gy[3];
+            double mu = 0.0, mv = 0.0;
+            if (slot >= 0) {
+                const double x = it.x, y = it.y;
+                weights(x, y, cx, cy, px, py, gx, gy);
+                valid = in_reach(cx, cy, x0, y0);
+                if (valid || home) {
+                    double cax, cay;  // attractor, 0.03 per step (fluidmodel.cpp:383-410)
+                    toward(x, y, it.gx, it.gy, 0.03, cax, cay);
+                    double u = it.u, v = it.v;
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) {
+                            const double phi = px[a] * py[b];
+                            double ax, ay;
+                            if (valid) {  // in reach: the node is staged
+                                const int e = halo_index(cx + a, cy + b, x0, y0);
+                                ax = accel[0][e];
+                                ay = accel[1][e];
+                            } else {
+                                ax = nd_m(Gd.ax, Gd.m, cx + a, cy + b, gsx, gsy);
+                                ay = nd_m(Gd.ay, Gd.m, cx + a, cy + b, gsx, gsy);
+                            }
+                            u += phi * (ax + cax);
+                            v += phi * (ay + cay);
                         }
-                        u += phi * (ax + cax);
-                        v += phi * (ay + cay);
+                    if (home) {  // other tiles still read the old velocity
+                        P.u2[slot] = u;
+                        P.v2[slot] = v;
                     }
-                if (home) {  // other tiles still read the old velocity
-                    P.u2[slot] = u;
-                    P.v2[slot] = v;
-                }
-                mu = u;
-                mv = v;
-                if (!it.mature) {
-                    mu *= 0.0;
-                    mv *= 0.0;
+                    mu = u;
+                    mv = v;
+                    if (!it.mature) {
+                        mu *= 0.0;
+                        mv *= 0.0;
+                    }
                 }
             }
-        }
-        const Seg S = segments(valid, cx, cy);
-        for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) {
-                const double phi = px[a] * py[b];
-                const float v0 = valid ? float(phi * mu) : 0.0f;
-                const float v1 = valid ? float(phi * mv) : 0.0f;
-                tile_add2(acc, S, valid, cx, cy, x0, y0, a, b, v0, v1);
-            }
-    });
-    __syncthreads();
-    store_tile2(acc, Gd.u, Gd.v, x0, y0, gsx, gsy, single);
+            const Seg S = segments(valid, cx, cy);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double phi = px[a] * py[b];
+                    const float v0 = valid ? float(phi * mu) : 0.0f;
+                    const float v1 = valid ? float(phi * mv) : 0.0f;
+                    tile_add2(acc, S, valid, cx, cy, x0, y0, a, b, v0, v1);
+                }
+        }, stage);
+        __syncthreads();
+        store_tile2(acc, Gd.u, Gd.v, x0, y0, gsx, gsy, single);
+        __syncthreads();  // the LDS sums are reused by the next item
+    }
 }
 
 __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
@@ -1020,7 +1080,7 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     (void)nkeys;
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
            (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) +
-           (tiles + 1 + max_chunks(tiles, n)) * 4 + 2 * 256 +
+           max_chunks(tiles, n) * 16 + 4 + 3 * 256 +
            nn * 4 * kGridFields + npx * 4 * 2 + 64 * 256;
 }
 
@@ -1068,8 +1128,8 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->bins.rank = reinterpret_cast<int32_t *>(take(n * 4 * 4));
     f->bins.list = reinterpret_cast<int32_t *>(take(n * 4 * 4));
     f->bins.max_chunks = int32_t(max_chunks(tiles, n));
-    f->bins.chunk_start = reinterpret_cast<int32_t *>(take((tiles + 1) * 4));
-    f->bins.chunk_tile = reinterpret_cast<int32_t *>(take(size_t(f->bins.max_chunks) * 4));
+    f->bins.chunk = reinterpret_cast<int4 *>(take(size_t(f->bins.max_chunks) * 16));
+    f->bins.nchunks = reinterpret_cast<int32_t *>(take(4));
     f->pbins.ntx = (width + kPxTile - 1) / kPxTile;
     f->pbins.nty = (height + kPxTile - 1) / kPxTile;
     const size_t ptiles = size_t(f->pbins.ntx) * size_t(f->pbins.nty);
@@ -1078,7 +1138,8 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->pbins.start = reinterpret_cast<int32_t *>(take((ptiles + 1) * 4));
     f->pbins.rank = f->bins.rank;  // the splat bins after the last step of a frame
     f->pbins.list = f->bins.list;
-    f->pbins.chunk_start = f->pbins.chunk_tile = nullptr;  // one workgroup per pixel tile
+    f->pbins.chunk = nullptr;  // one workgroup per pixel tile
+    f->pbins.nchunks = nullptr;
     f->pbins.max_chunks = 0;
     f->sxy = reinterpret_cast<uint32_t *>(take(n * 4));
     f->staps = reinterpret_cast<double *>(take(n * 32));
@@ -1211,8 +1272,10 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     // the counts in LDS (where the chunks are laid out)
     Bins B = f->bins;
     const bool chunked = tiles <= kLdsTiles;
-    if (!chunked) B.chunk_start = B.chunk_tile = nullptr;
-    const uint32_t grid = uint32_t(chunked ? B.max_chunks : tiles);
+    if (!chunked) B.chunk = nullptr;
+    // about one chunk per workgroup (a bin averages under one chunk); a
+    // workgroup takes further chunks when there are more
+    const uint32_t grid = uint32_t(chunked ? std::min(B.max_chunks, 2 * tiles) : tiles);
     {
         amh::ProfScope ps_("fluid_bins", s);
         const int par = f->bin_par;
@@ -1221,15 +1284,15 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     }
     {
         amh::ProfScope ps_("k_p2g", s);
-        hipLaunchKernelGGL(k_p2g, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+        hipLaunchKernelGGL(chunked ? k_p2g<true> : k_p2g<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     }
     {
         amh::ProfScope ps_("k_forces", s);
-        hipLaunchKernelGGL(k_forces, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+        hipLaunchKernelGGL(chunked ? k_forces<true> : k_forces<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     }
     {
         amh::ProfScope ps_("k_g2p", s);
-        hipLaunchKernelGGL(k_g2p, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+        hipLaunchKernelGGL(chunked ? k_g2p<true> : k_g2p<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     }
     {
         amh::ProfScope ps_("k_advect", s);
