@@ -4,6 +4,7 @@
  * HIP streams through the C ABI of libatomorph_hip.so.
  */
 #include "atomorph.h"
+#include "blobify.h"
 
 #include <atomic>
 #include <chrono>
@@ -456,14 +457,48 @@ bool morph::engine::upload_frames() {
 }
 
 // BLOB_DETECTION + UNIFICATION in one step per frame set: ingest, segment,
-// per-blob statistics and the row-major pixel lists of every blob.
+// per-blob statistics and the row-major pixel lists of every blob.  With
+// the default blob options the blobs are the 4-connected components of the
+// foreground (device segmentation); otherwise the greedy merge and dust
+// unification run on the host over the stored colours (blobify.cpp).
 bool morph::engine::detect() {
     const size_t npx = size_t(CW) * CH;
     nblobs.assign(h, 0);
     stats.assign(h, {});
     if (!d_bws.ensure(amh_blob_workspace_bytes(CW, CH))) return fail("device allocation");
     std::vector<int64_t> first_off(h + 1, 0);
-    for (int y = 0; y < h; ++y) {
+    am_blobify::options bo;
+    bo.threshold = wopt.blob_threshold;
+    bo.max_size = wopt.blob_max_size;
+    bo.min_size = wopt.blob_min_size;
+    bo.number = wopt.blob_number;
+    bo.box_samples = wopt.blob_box_samples;
+    bo.box_grip = wopt.blob_box_grip;
+    const bool greedy = am_blobify::needed(bo);
+    std::vector<am_blobify::frame_result> gres;
+    std::vector<std::vector<uint32_t>> gstored;
+    if (greedy) {
+        gstored.assign(h, std::vector<uint32_t>(npx));
+        std::vector<const uint32_t *> ptrs(h);
+        for (int y = 0; y < h; ++y) {
+            uint32_t *stored = d_stored.as<uint32_t>() + size_t(y) * npx;
+            if (amh_ingest_frame(d_rgba.as<uint32_t>() + size_t(y) * npx, CW, CH, wopt.blob_delimiter == HSP ? 1 : 0,
+                                 stored, d_shown.as<uint32_t>() + size_t(y) * npx, cstream) ||
+                amh_memcpy_d2h(gstored[y].data(), stored, npx * 4, cstream))
+                return fail("ingest");
+            ptrs[y] = gstored[y].data();
+        }
+        if (amh_stream_sync(cstream)) return fail("ingest sync");
+        am_blobify::segment(ptrs, CW, CH, bo, wopt.seed, gres);
+        for (int y = 0; y < h; ++y) {
+            nblobs[y] = int32_t(gres[y].blobs.size());
+            if (amh_memcpy_h2d(d_bop.as<int32_t>() + size_t(y) * npx, gres[y].bop.data(), npx * 4, cstream))
+                return fail("blob upload");
+            first_off[y + 1] = first_off[y] + nblobs[y] + 1;
+        }
+        if (amh_stream_sync(cstream)) return fail("blob upload sync");
+    }
+    for (int y = 0; y < h && !greedy; ++y) {
         uint32_t *rgba = d_rgba.as<uint32_t>() + size_t(y) * npx;
         uint32_t *stored = d_stored.as<uint32_t>() + size_t(y) * npx;
         uint32_t *shown = d_shown.as<uint32_t>() + size_t(y) * npx;
@@ -501,6 +536,32 @@ bool morph::engine::detect() {
     for (int y = 0; y < h; ++y) {
         double sx = 0, sy = 0, sc[4] = {0, 0, 0, 0};
         size_t n = 0;
+        if (greedy) {  // the same sums over the stored pixels
+            const auto &bop = gres[y].bop;
+            std::vector<int> first(nblobs[y], -1);
+            int64_t ssx = 0, ssy = 0, ssc[4] = {0, 0, 0, 0};
+            for (size_t i = 0; i < npx; ++i) {
+                if (bop[i] < 0) continue;
+                if (first[bop[i]] < 0) first[bop[i]] = int(i);
+                ssx += int64_t(i % CW);
+                ssy += int64_t(i / CW);
+                const color c = word_color(gstored[y][i]);
+                ssc[0] += c.r; ssc[1] += c.g; ssc[2] += c.b; ssc[3] += c.a;
+                ++n;
+            }
+            sx = double(ssx); sy = double(ssy);
+            for (int k = 0; k < 4; ++k) sc[k] = double(ssc[k]);
+            for (int b = 0; b < nblobs[y]; ++b) {
+                const am_blobify::blob_info &g = gres[y].blobs[b];
+                BlobInfo bi;
+                bi.id = b;
+                bi.size = g.size;
+                bi.x = g.x; bi.y = g.y;
+                bi.r = g.r; bi.g = g.g; bi.b = g.b; bi.a = g.a;
+                bi.first = first[b];
+                fb[y].push_back(bi);
+            }
+        }
         for (const auto &s : stats[y]) {
             sx += double(s.sum_x); sy += double(s.sum_y);
             for (int k = 0; k < 4; ++k) sc[k] += double(s.sum_c[k]);
@@ -514,7 +575,7 @@ bool morph::engine::detect() {
             m[0] = bbox_x1 <= bbox_x2 ? (bbox_x1 + bbox_x2) / 2.0 : 0.0;
             m[1] = bbox_y1 <= bbox_y2 ? (bbox_y1 + bbox_y2) / 2.0 : 0.0;
         }
-        for (int b = 0; b < nblobs[y]; ++b) {
+        for (int b = 0; b < nblobs[y] && !greedy; ++b) {
             const amh_blob_stat &s = stats[y][b];
             BlobInfo bi;
             bi.id = b;

@@ -72,6 +72,8 @@ _PROTOS = {
     "am_morph_background": (C.c_uint32, [vp, C.c_uint, C.c_uint, C.c_double]),
     "am_morph_frame_count": (sz, [vp]),
     "am_morph_pixel_count": (sz, [vp, sz]),
+    "am_blobify_frames": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, u64, u64, u64, u64,
+                                    C.c_uint32, C.c_uint, vp, vp, vp, vp, vp]),
 }
 
 _lib = None
@@ -90,6 +92,41 @@ def load():
             fn.argtypes = args
         _lib = lib
     return _lib
+
+
+SIZE_MAX = (1 << 64) - 1
+
+
+def blobify_frames(stored, threshold=1.0, max_size=SIZE_MAX, min_size=1, number=1, box_samples=10,
+                   box_grip=65535, seed=0):
+    """Greedy blob detection and dust unification on the host
+    (thread.cpp:225-596), as the engine runs them for non-default blob
+    options.
+
+    stored: (frames, H, W) uint32 RGBA words of the key frames as the morph
+    stores them (after the colour conversion); a pixel is present when its
+    alpha is non-zero.  Returns (bop, blobs): bop (frames, H*W) int32, the
+    blob ordinal of each pixel (-1: no pixel); blobs[f], frame f's blobs in
+    list order as dicts of means (x, y, r, g, b, a), group (y*65536 + x of
+    the pixel it grew from) and size."""
+    lib = load()
+    st = np.ascontiguousarray(stored, np.uint32)
+    nf, H, W = st.shape
+    npx = H * W
+    bop = np.zeros((nf, npx), np.int32)
+    nb = np.zeros(nf, np.int32)
+    means = np.zeros((nf, npx, 6))
+    groups = np.zeros((nf, npx), np.uint64)
+    sizes = np.zeros((nf, npx), np.uint64)
+    rc = lib.am_blobify_frames(st.ctypes.data_as(vp), nf, W, H, float(threshold), int(max_size), int(min_size),
+                               int(number), int(box_samples), int(box_grip), int(seed), bop.ctypes.data_as(vp),
+                               nb.ctypes.data_as(vp), means.ctypes.data_as(vp), groups.ctypes.data_as(vp),
+                               sizes.ctypes.data_as(vp))
+    if rc != 0:
+        raise ValueError("am_blobify_frames: invalid argument")
+    blobs = [[{"means": means[f, b].copy(), "group": int(groups[f, b]), "size": int(sizes[f, b])}
+              for b in range(nb[f])] for f in range(nf)]
+    return bop, blobs
 
 
 class Morph:

@@ -70,6 +70,7 @@ _REF = {
     "ref_morph_add_rgba": (None, [vp, sz, vp, C.c_uint, C.c_uint]),
     "ref_morph_set_resolution": (None, [vp, C.c_uint, C.c_uint]),
     "ref_morph_run": (C.c_uint, [vp, sz]),
+    "ref_morph_run_until": (C.c_uint, [vp, C.c_uint, sz]),
     "ref_morph_next_state": (None, [vp]),
     "ref_morph_state": (C.c_uint, [vp]),
     "ref_morph_energy": (f64, [vp]),

@@ -156,6 +156,12 @@ void ref_morph_set_option(void *mp, const char *name, double v) {
     else if (n == "show_blobs") m->set_show_blobs(unsigned(v));
     else if (n == "blob_delimiter") m->set_blob_delimiter((unsigned char)v);
     else if (n == "degeneration") m->set_degeneration(size_t(v));
+    else if (n == "blob_threshold") m->set_blob_threshold(v);
+    else if (n == "blob_max_size") m->set_blob_max_size(size_t(v));
+    else if (n == "blob_min_size") m->set_blob_min_size(size_t(v));
+    else if (n == "blob_box_grip") m->set_blob_box_grip(uint16_t(v));
+    else if (n == "blob_box_samples") m->set_blob_box_samples(size_t(v));
+    else if (n == "blob_number") m->set_blob_number(size_t(v));
 }
 
 /* demo/main.cpp:96-127 (fill_morph): skips alpha == 0 pixels. */
@@ -182,6 +188,15 @@ unsigned ref_morph_run(void *mp, size_t n) {
         for (size_t i = 0; i < n; ++i) m->worker.step();
         m->synchronize();
     }
+    return m->get_state();
+}
+/* Worker steps until its state reaches `state` (or max_steps were taken),
+ * then synchronize; returns the state. */
+unsigned ref_morph_run_until(void *mp, unsigned state, size_t max_steps) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    m->synchronize();
+    for (size_t i = 0; i < max_steps && m->worker.get_state() < state; ++i) m->worker.step();
+    m->synchronize();
     return m->get_state();
 }
 void ref_morph_next_state(void *mp) {

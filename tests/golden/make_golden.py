@@ -13,6 +13,14 @@ Fixtures
                  threads = 0: blob membership, chains before and after one
                  2^16-attempt visit per chain, and 16 rendered frames for
                  several option sets.
+  blobify.npz    greedy blob detection and dust unification with
+                 non-default blob options (threshold below 1, size cap,
+                 minimum size, blob count, grip box): 3 key frames of 32x28
+                 noisy ellipses per case; the blob list of every frame when
+                 the reference reaches BLOB_MATCHING (order, running means,
+                 starting pixel, membership).
+
+    python tests/golden/make_golden.py [functions] [cfg1] [blobify]
 """
 import ctypes as C
 import os
@@ -233,17 +241,116 @@ def run_cfg1():
     return d
 
 
+# Blob options of the greedy cases: threshold, max size, min size, blob
+# number, box samples, box grip, delimiter (0 RGB, 1 HSP), seed.
+BLOBIFY_CASES = [
+    (0.1, None, 1, 1, 10, 65535, 1, 1),
+    (0.05, None, 8, 1, 10, 65535, 1, 2),
+    (0.2, 30, 1, 1, 10, 65535, 1, 3),
+    (1.0, None, 1, 3, 10, 65535, 1, 4),
+    (0.1, None, 20, 1, 4, 6, 1, 5),
+    (0.15, None, 1, 20, 10, 65535, 0, 6),
+    (1.0, None, 40, 1, 10, 65535, 1, 7),
+]
+SIZE_MAX = (1 << 64) - 1
+
+
+def noisy_frames(rng, n, W, H, nell=5):
+    """n RGBA8 frames of filled ellipses with per-pixel colour noise,
+    quantised to steps of 16 so that equal colour distances (ties) occur."""
+    out = []
+    yy, xx = np.mgrid[0:H, 0:W]
+    for _ in range(n):
+        img = np.zeros((H, W, 4), np.uint8)
+        for _ in range(nell):
+            cx, cy = rng.uniform(0, W), rng.uniform(0, H)
+            rx, ry = rng.uniform(3, W / 3), rng.uniform(3, H / 3)
+            m = ((xx - cx) / rx) ** 2 + ((yy - cy) / ry) ** 2 <= 1
+            col = np.clip(rng.integers(0, 256, 3) + rng.integers(-30, 31, (H, W, 3)), 0, 255)
+            img[m, :3] = col[m]
+            img[m, 3] = 255
+        img[..., :3] = (img[..., :3] // 16) * 16
+        out.append(img)
+    return out
+
+
+def blobify_fixture():
+    r = op.ref()
+    rng = np.random.default_rng(2024)
+    W, H, nf = 32, 28, 3
+    d = {"bf_size": np.array([W, H, nf], np.int32)}
+    frames, stored, opts_f, opts_i, bop, nblobs, means, groups, sizes = [], [], [], [], [], [], [], [], []
+    for (thr, mx, mn, num, samples, grip, delim, seed) in BLOBIFY_CASES:
+        fr = noisy_frames(rng, nf, W, H)
+        frames.append(np.stack(fr))
+        st = []
+        for img in fr:
+            w = np.ascontiguousarray(img).view(np.uint32).reshape(-1)
+            st.append(np.array([r.ref_rgb_to_hsp(int(c)) for c in w], np.uint32) if delim else w.copy())
+        stored.append(np.stack(st))
+        opts_f.append(thr)
+        opts_i.append([SIZE_MAX if mx is None else mx, mn, num, samples, grip, delim, seed])
+        m = r.ref_morph_new()
+        for k, v in (("seed", seed), ("threads", 0), ("blob_delimiter", delim), ("blob_threshold", thr),
+                     ("blob_min_size", mn), ("blob_number", num), ("blob_box_samples", samples),
+                     ("blob_box_grip", grip)):
+            r.ref_morph_set_option(m, k.encode(), float(v))
+        if mx is not None:
+            r.ref_morph_set_option(m, b"blob_max_size", float(mx))
+        for f, img in enumerate(fr):
+            a = np.ascontiguousarray(img)
+            r.ref_morph_add_rgba(m, f, op.ptr(a), W, H)
+        r.ref_morph_set_resolution(m, W, H)
+        state = r.ref_morph_run_until(m, 2, 1 << 40)
+        assert state == 2, STATE.get(state)
+        case_bop = -np.ones((nf, W * H), np.int16)
+        case_n = []
+        for f in range(nf):
+            nb = r.ref_morph_blob_count(m, f)
+            case_n.append(nb)
+            for b in range(nb):
+                g = C.c_size_t()
+                mm = np.zeros(6)
+                cnt = r.ref_morph_blob(m, f, b, C.byref(g), op.ptr(mm), None)
+                surf = np.zeros(cnt, np.uint64)
+                r.ref_morph_blob(m, f, b, C.byref(g), None, op.ptr(surf))
+                case_bop[f, ((surf // 65536) * W + surf % 65536).astype(np.int64)] = b
+                means.append(mm)
+                groups.append(g.value)
+                sizes.append(cnt)
+        r.ref_morph_free(m)
+        bop.append(case_bop)
+        nblobs.append(case_n)
+    d["bf_frames"] = np.stack(frames)
+    d["bf_stored"] = np.stack(stored)
+    d["bf_threshold"] = np.array(opts_f)
+    d["bf_opts"] = np.array(opts_i, np.uint64)   # max, min, number, samples, grip, delimiter, seed
+    d["bf_bop"] = np.stack(bop)
+    d["bf_nblobs"] = np.array(nblobs, np.int32)
+    d["bf_means"] = np.array(means)
+    d["bf_groups"] = np.array(groups, np.uint64)
+    d["bf_sizes"] = np.array(sizes, np.int64)
+    return d
+
+
 def main():
     if not op.have_ref():
         sys.exit("oracle/_ref/libamref.so is missing: run `make -C oracle ref` where the reference exists")
-    f = functions_fixture()
-    np.savez_compressed(os.path.join(OUT, "functions.npz"), **f)
-    c = run_cfg1()
-    np.savez_compressed(os.path.join(OUT, "cfg1.npz"), **c)
-    for name in ("functions.npz", "cfg1.npz"):
-        print(name, os.path.getsize(os.path.join(OUT, name)), "bytes")
-    print("cfg1: blobify steps", int(c["blobify_steps"]), "chains", len(c["chain_w"]),
-          "widths", c["chain_w"].tolist())
+    which = set(sys.argv[1:]) or {"functions", "cfg1", "blobify"}
+    if "functions" in which:
+        f = functions_fixture()
+        np.savez_compressed(os.path.join(OUT, "functions.npz"), **f)
+    if "cfg1" in which:
+        c = run_cfg1()
+        np.savez_compressed(os.path.join(OUT, "cfg1.npz"), **c)
+        print("cfg1: blobify steps", int(c["blobify_steps"]), "chains", len(c["chain_w"]),
+              "widths", c["chain_w"].tolist())
+    if "blobify" in which:
+        b = blobify_fixture()
+        np.savez_compressed(os.path.join(OUT, "blobify.npz"), **b)
+        print("blobify: blobs per case and frame", b["bf_nblobs"].tolist())
+    for name in sorted(which):
+        print(name + ".npz", os.path.getsize(os.path.join(OUT, name + ".npz")), "bytes")
 
 
 if __name__ == "__main__":

@@ -281,3 +281,47 @@ def test_edge_cases_render_like_the_oracle(case):
         worst = max(worst, int(np.abs(got.astype(int) - out.astype(int)).max()))
     G.log(f"edge case {case}: worst channel difference {worst}")
     assert worst <= 1
+
+
+def _blobify_fixture():
+    import os
+    return dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "blobify.npz")))
+
+
+@pytest.mark.parametrize("c", range(7))
+def test_greedy_blob_options_through_the_engine(c):
+    """Non-default blob options: detection runs the host greedy merge and
+    dust unification over the device-ingested stored colours; the blobs the
+    engine reports (order, running means, membership) are the compiled
+    reference's, and the pipeline runs on to rendered frames."""
+    fx = _blobify_fixture()
+    W, H, nf = (int(v) for v in fx["bf_size"])
+    mx, mn, num, samples, grip, delim, seed = (int(v) for v in fx["bf_opts"][c])
+    opts = dict(blob_threshold=float(fx["bf_threshold"][c]), blob_min_size=mn, blob_number=num,
+                blob_box_samples=samples, blob_box_grip=grip, blob_delimiter=delim)
+    if mx != M.SIZE_MAX:
+        opts["blob_max_size"] = mx
+    m = M.Morph()
+    m.set(seed=seed, cycle_length=1 << 12, **opts)
+    for i in range(nf):
+        m.add_frame_rgba(i, fx["bf_frames"][c, i])
+    m.set_resolution(W, H)
+    G.log(f"greedy blobs case {c}: {opts}")
+    assert m.run(1) == M.STATE_BLOB_UNIFICATION
+    base = int(fx["bf_nblobs"][:c].sum())
+    for f in range(nf):
+        n = int(fx["bf_nblobs"][c, f])
+        assert m.blob_count(f) == n
+        for b in range(n):
+            k = base + int(fx["bf_nblobs"][c, :f].sum()) + b
+            got = m.blob(f, b)
+            assert np.array_equal(got["means"], fx["bf_means"][k]), (f, b)
+            idx = np.flatnonzero(fx["bf_bop"][c, f] == b)
+            want = ((idx // W) * 65536 + idx % W).astype(np.uint64)
+            assert np.array_equal(np.sort(got["surface"]), want), (f, b)
+    assert m.run(201) == M.STATE_BLOB_MATCHING
+    m.next_state()
+    assert m.run(1) == M.STATE_ATOM_MORPHING
+    m.run(4)
+    img = m.render_pixels(0.5)
+    assert img.shape == (H, W, 4) and img[..., 3].any()
