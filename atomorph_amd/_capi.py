@@ -87,6 +87,7 @@ PROTOTYPES = {
     "amh_refine_tiles": (C.c_int, [C.POINTER(i32), vp, vp, i32]),
     "amh_refine_items": (C.c_int, [vp, i32, i32, vp, i32, C.POINTER(i32)]),
     "amh_refine_batched": (C.c_int, [vp, i32, i64, vp, vp, vp, i32, i32, u64, i32, u64, vp, vp, vp]),
+    "amh_refine_knn": (C.c_int, [vp, i32, i64, vp, vp, vp, i32, i32, u64, i32, u64, vp, vp, vp]),
     "amh_refine_graph_create": (C.c_int, [C.POINTER(vp), vp, i32, i64, vp, vp, vp, i32, i32, i32, i32, u64, vp,
                                           vp, vp]),
     "amh_graph_launch": (C.c_int, [vp, vp]),

@@ -107,6 +107,35 @@ __host__ __device__ __forceinline__ uint32_t pass_row(uint64_t seed, uint64_t ch
     return uint32_t(mix64(seed ^ 0x5851F42D4C957F2Dull ^ mix64((chain << 32) ^ pass)) % h);
 }
 
+// ------------------------------------------------- spatial keys (kNN)
+// Morton (Z-order) key of integer pixel coordinates (x, y < 2^16), and the
+// coordinates back from a key.
+__host__ __device__ __forceinline__ uint32_t spread16(uint32_t v) {
+    v &= 0xFFFFu;
+    v = (v | (v << 8)) & 0x00FF00FFu;
+    v = (v | (v << 4)) & 0x0F0F0F0Fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+__host__ __device__ __forceinline__ uint32_t squeeze16(uint32_t v) {
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0F0F0F0Fu;
+    v = (v | (v >> 4)) & 0x00FF00FFu;
+    v = (v | (v >> 8)) & 0x0000FFFFu;
+    return v;
+}
+__host__ __device__ __forceinline__ uint32_t morton2(uint32_t x, uint32_t y) {
+    return spread16(x) | (spread16(y) << 1);
+}
+// The draw of slot l in round r of a kNN-guided pass (stream = pass_stream):
+// bits 12..31 its claim priority, bits 32..47 a random other slot, bits
+// 48..63 the pick among its candidates.
+__host__ __device__ __forceinline__ uint64_t knn_draw(uint64_t seed, uint64_t stream, uint32_t round, uint32_t l) {
+    return mix64(seed ^ 0x2545F4914F6CDD1Dull ^ mix64(stream ^ (uint64_t(round) << 32) ^ l));
+}
+
 // ------------------------------------------------------------ reductions
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);

@@ -234,6 +234,211 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
     }
 }
 
+// ------------------------------------------- batched, kNN-guided pairs
+// The reference's OpenCV build draws the second slot of a swap from a
+// nearest-neighbour search instead of at random (thread.cpp:893-986): for
+// slot x1 of row y it takes the two points of row y+1 nearest to x1's point
+// in row y (slots whose row-(y+1) point is where x1's point would like to
+// go), the two nearest to x1's own point in row y+1, and one random slot,
+// and picks one of them at random.
+//
+// Here every slot of a work item proposes one partner per round that way,
+// over an index of row y+1 built once per item in LDS: the slots sorted by
+// the Morton key of their row-(y+1) pixel.  A point's two nearest are taken
+// from the four entries around its place in that order -- an approximate
+// search, as the reference's FLANN search with one check is.  Proposals
+// overlap, so each claims both of its slots with its priority (atomicMin in
+// LDS) and only those holding both claims run their swap test: the winners'
+// pairs are disjoint, so they run in parallel and equal the same attempts
+// made one after another.  Row y+1 is not permuted while row y is refined,
+// so the index holds for the whole pass.
+constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
+
+// The two entries of sk[lo, hi) (clipped to [0, n)) whose pixels are
+// nearest to (qx, qy), other than slot `self`; ties go to the earlier entry.
+__device__ __forceinline__ void knn_nearest2(const uint64_t *sk, int n, int lo, int hi, uint32_t self, uint32_t qx,
+                                             uint32_t qy, uint32_t *cand, int &m) {
+    uint64_t d1 = ~0ull, d2 = ~0ull;
+    int b1 = -1, b2 = -1;
+    for (int k = max(lo, 0); k < min(hi, n); ++k) {
+        const uint64_t e = sk[k];
+        const uint32_t s = uint32_t(e & 0xFFFFu);
+        if (s == self) continue;
+        const uint32_t key = uint32_t(e >> 16);
+        const int64_t dx = int64_t(squeeze16(key)) - qx, dy = int64_t(squeeze16(key >> 1)) - qy;
+        const uint64_t dd = uint64_t(dx * dx + dy * dy);
+        if (dd < d1) {
+            d2 = d1;
+            b2 = b1;
+            d1 = dd;
+            b1 = int(s);
+        } else if (dd < d2) {
+            d2 = dd;
+            b2 = int(s);
+        }
+    }
+    if (b1 >= 0) cand[m++] = uint32_t(b1);
+    if (b2 >= 0) cand[m++] = uint32_t(b2);
+}
+
+// The partner slot slot l proposes with draw d (knn_draw).
+__device__ __forceinline__ uint32_t knn_partner(const uint64_t *sk, const uint16_t *rk, const uint64_t *cur, int n,
+                                                uint32_t l, uint64_t d) {
+    uint32_t cand[5];
+    int m = 0;
+    // near the slot's point in row y: where it would like to go
+    const uint64_t q = cur[l];
+    const uint32_t qx = pt_x(q), qy = pt_y(q);
+    const uint64_t kq = uint64_t(morton2(qx, qy)) << 16;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (sk[mid] < kq) lo = mid + 1;
+        else hi = mid;
+    }
+    knn_nearest2(sk, n, lo - 2, lo + 2, l, qx, qy, cand, m);
+    // near the slot's own point in row y+1
+    const int kd = rk[l];
+    const uint32_t key = uint32_t(sk[kd] >> 16);
+    const uint32_t nx = squeeze16(key), ny = squeeze16(key >> 1);
+    knn_nearest2(sk, n, kd - 2, kd + 3, l, nx, ny, cand, m);
+    // and one random other slot
+    const uint32_t u = uint32_t((((d >> 32) & 0xFFFFu) * uint64_t(n - 1)) >> 16);
+    cand[m++] = u < l ? u : u + 1u;
+    return cand[((d >> 48) * uint64_t(m)) >> 16];
+}
+
+template <int CAP, int NT>
+__global__ __launch_bounds__(NT) void k_refine_knn(uint64_t *__restrict__ pts, int h, int64_t wtot,
+                                                   const int64_t *__restrict__ off, const int32_t *__restrict__ cw,
+                                                   const int32_t *__restrict__ items, int nitems, uint64_t pass,
+                                                   int rounds, uint64_t seed, uint32_t *queue,
+                                                   amh_refine_stats *st) {
+    static_assert(CAP <= 4096 && (CAP & (CAP - 1)) == 0 && CAP % NT == 0, "slots in 12 bits, whole rounds");
+    constexpr int PER = CAP / NT;
+    __shared__ uint64_t C[CAP];
+    __shared__ int32_t SX[CAP], SY[CAP];
+    __shared__ uint64_t SK[CAP];  // Morton key of the row-(y+1) pixel << 16 | slot, sorted
+    __shared__ uint16_t RK[CAP];  // place of each slot in SK
+    __shared__ uint32_t CL[CAP];  // claims of the round
+    __shared__ int item_s;
+    const int tid = threadIdx.x;
+    uint64_t att = 0, acc = 0;
+    int64_t gain = 0;
+    const bool one_each = nitems <= int(gridDim.x);
+    for (int turn = 0;; ++turn) {
+        if (one_each) {
+            if (tid == 0) item_s = turn == 0 ? int(blockIdx.x) : nitems;
+        } else if (tid == 0) {
+            item_s = int(atomicAdd(&queue[0], 1u));
+        }
+        __syncthreads();
+        const int it = item_s;
+        if (it >= nitems) break;
+        const int c = items[2 * it], g = items[2 * it + 1];
+        const int32_t w = cw[c];
+        const int64_t o = off[c];
+        const int y = int(pass_row(seed, uint64_t(c), pass, uint32_t(h)));
+        const int yp = y > 0 ? y - 1 : h - 1, yn = (y + 1) % h;
+        const int ng = (w + CAP - 1) / CAP;
+        const int gs = (w + ng - 1) / ng;
+        const int g0 = g * gs;
+        const int n = min(gs, w - g0);
+        const bool permuted = ng > 1;
+        const uint64_t pkey = pass_stream(uint64_t(c), pass, 0xFFFFFFFFull);
+        const uint64_t *rprev = pts + int64_t(yp) * wtot + o;
+        const uint64_t *rnext = pts + int64_t(yn) * wtot + o;
+        uint64_t *row = pts + int64_t(y) * wtot + o;
+        int P = 2;
+        while (P < n) P <<= 1;
+        for (int l = tid; l < P; l += NT) {
+            if (l < n) {
+                const uint32_t s = permuted ? feistel_perm(uint32_t(g0 + l), uint32_t(w), pkey) : uint32_t(l);
+                const uint64_t p = rprev[s], q = rnext[s];
+                C[l] = row[s];
+                SX[l] = pt_fx(p) + pt_fx(q);
+                SY[l] = pt_fy(p) + pt_fy(q);
+                SK[l] = (uint64_t(morton2(pt_x(q), pt_y(q))) << 16) | uint64_t(l);
+                CL[l] = kNoClaim;
+            } else {
+                SK[l] = ~0ull;
+            }
+        }
+        __syncthreads();
+        // bitonic sort of SK[0, P): keys are distinct (the slot is in them)
+        for (int k = 2; k <= P; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = tid; i < P; i += NT) {
+                    const int ij = i ^ j;
+                    if (ij > i) {
+                        const uint64_t a = SK[i], b = SK[ij];
+                        if ((a > b) == ((i & k) == 0)) {
+                            SK[i] = b;
+                            SK[ij] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int k = tid; k < n; k += NT) RK[SK[k] & 0xFFFFu] = uint16_t(k);
+        __syncthreads();
+        if (n >= 2) {
+            const uint64_t sid = pass_stream(uint64_t(c), pass, uint64_t(g));
+            for (int r = 0; r < rounds; ++r) {
+                uint32_t part[PER], pri[PER];
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int l = tid + k * NT;
+                    part[k] = 0;
+                    pri[k] = kNoClaim;
+                    if (l < n) {
+                        const uint64_t d = knn_draw(seed, sid, uint32_t(r), uint32_t(l));
+                        part[k] = knn_partner(SK, RK, C, n, uint32_t(l), d);
+                        pri[k] = (uint32_t(d) & 0xFFFFF000u) | uint32_t(l);
+                        atomicMin(&CL[l], pri[k]);
+                        atomicMin(&CL[part[k]], pri[k]);
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < PER; ++k) {
+                    const int i = tid + k * NT;
+                    if (i < n && CL[i] == pri[k] && CL[part[k]] == pri[k]) {
+                        const int j = int(part[k]);
+                        const uint64_t ci = C[i], cj = C[j];
+                        const int64_t d = swap_test(ci, cj, SX[i], SY[i], SX[j], SY[j]);
+                        ++att;
+                        if (d <= 0) {
+                            C[i] = cj;
+                            C[j] = ci;
+                            ++acc;
+                            gain += -2 * d;
+                        }
+                    }
+                }
+                __syncthreads();
+                for (int l = tid; l < n; l += NT) CL[l] = kNoClaim;
+                __syncthreads();
+            }
+        }
+        for (int l = tid; l < n; l += NT) {
+            const uint32_t s = permuted ? feistel_perm(uint32_t(g0 + l), uint32_t(w), pkey) : uint32_t(l);
+            row[s] = C[l];
+        }
+        __syncthreads();
+    }
+    flush_stats<NT>(st, att, acc, gain);
+    if (tid == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&queue[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&queue[0], 0u);
+            atomicExch(&queue[1], 0u);
+            atomicAdd(&queue[2], 1u);
+        }
+    }
+}
+
 struct TileVariant {
     int cap, threads;
     void (*fn)(uint64_t *, int, int64_t, const int64_t *, const int32_t *, const int32_t *, int, uint64_t, int,
@@ -246,19 +451,30 @@ const TileVariant kTiles[] = {
 };
 constexpr int kNumTiles = int(sizeof(kTiles) / sizeof(kTiles[0]));
 int g_tile_grid[kNumTiles] = {0};
+// kNN-guided variants of the first four tiles (slot indices fit 12 bits)
+const TileVariant kKnnTiles[] = {
+    {1024, 256, k_refine_knn<1024, 256>},
+    {2048, 256, k_refine_knn<2048, 256>},
+    {4096, 256, k_refine_knn<4096, 256>},
+    {4096, 512, k_refine_knn<4096, 512>},
+};
+constexpr int kNumKnnTiles = int(sizeof(kKnnTiles) / sizeof(kKnnTiles[0]));
+int g_knn_grid[kNumKnnTiles] = {0};
 
-int tile_grid(int tile) {
-    if (g_tile_grid[tile] == 0) {
+// Workgroups that are resident at once: the persistent grid of a variant.
+int resident_grid(const TileVariant &v, int &cached) {
+    if (cached == 0) {
         int dev = 0, ncu = 0, occ = 0;
         if (hipGetDevice(&dev) != hipSuccess) return 0;
         if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(kTiles[tile].fn),
-                                                         kTiles[tile].threads, 0) != hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(v.fn), v.threads, 0) !=
+            hipSuccess)
             return 0;
-        g_tile_grid[tile] = std::max(1, ncu * std::max(occ, 1));
+        cached = std::max(1, ncu * std::max(occ, 1));
     }
-    return g_tile_grid[tile];
+    return cached;
 }
+int tile_grid(int tile) { return resident_grid(kTiles[tile], g_tile_grid[tile]); }
 
 }  // namespace
 
@@ -367,6 +583,33 @@ extern "C" int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot
                            stats_dev);
     }
     AMH_LAUNCH_CHECK("amh_refine_batched");
+    return AMH_OK;
+}
+
+extern "C" int amh_refine_knn(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
+                              const int32_t *chain_w_dev, const int32_t *items_dev, int32_t nitems, int32_t tile,
+                              uint64_t pass, int32_t rounds, uint64_t seed, uint32_t *queue_dev,
+                              amh_refine_stats *stats_dev, void *stream) {
+    if (!pts_dev || !chain_off_dev || !chain_w_dev || !queue_dev || !stats_dev || tile < 0 || tile >= kNumKnnTiles ||
+        nkeys <= 0 || nitems < 0 || rounds < 0) {
+        set_error("amh_refine_knn: invalid argument (tile %d of %d kNN variants)", tile, kNumKnnTiles);
+        return AMH_E_ARG;
+    }
+    if (nkeys < 2 || nitems == 0 || rounds == 0) return AMH_OK;
+    if (!items_dev) {
+        set_error("amh_refine_knn: items_dev is NULL");
+        return AMH_E_ARG;
+    }
+    const int grid = std::min(resident_grid(kKnnTiles[tile], g_knn_grid[tile]), int(nitems));
+    if (grid <= 0) return check_hip(hipGetLastError(), "amh_refine_knn occupancy");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    {
+        amh::ProfScope ps_("k_refine_knn", s);
+        hipLaunchKernelGGL(kKnnTiles[tile].fn, dim3(grid), dim3(kKnnTiles[tile].threads), 0, s, pts_dev, nkeys, wtot,
+                           chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed, queue_dev,
+                           stats_dev);
+    }
+    AMH_LAUNCH_CHECK("amh_refine_knn");
     return AMH_OK;
 }
 

@@ -361,8 +361,13 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tdata = json.load(f)
-        if tdata.get("workload") == args.workload and dom in tdata.get("bytes_per_launch", {}):
-            traffic = int(tdata["bytes_per_launch"][dom])
+        if tdata.get("workload") == args.workload:
+            # kernel names as the profiler prints them (templates with their
+            # arguments); the launch of this build's variant of `dom`
+            for name, b in tdata.get("bytes_per_launch", {}).items():
+                if name == dom or name.split("<")[0] == dom:
+                    traffic = int(b)
+                    break
     # rendering, for comparison: per atom and frame h points + 2 colours read,
     # per output pixel 4 B written
     render_bytes = FRAMES_OUT * (int(widths.sum()) * (8 * h + 8) + H * W * 4)

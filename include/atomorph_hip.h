@@ -181,6 +181,20 @@ int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int
                        const int32_t *chain_w_dev, const int32_t *items_dev, int32_t nitems,
                        int32_t tile, uint64_t pass, int32_t rounds, uint64_t seed,
                        uint32_t *queue_dev, amh_refine_stats *stats_dev, void *stream);
+/* kNN-guided batched descent: replaces the OpenCV build's thread::morph
+ * (thread.cpp:893-986), whose second slot is drawn from the two nearest
+ * row-(y+1) points of slot x1's point in row y, the two nearest of x1's own
+ * row-(y+1) point (FLANN search), or one random slot.  Same items, queue,
+ * pass and row schedule as amh_refine_batched; per round every slot of a
+ * work item proposes a partner that way (an index of row y+1 in LDS: slots
+ * sorted by the Morton key of their pixel, nearest taken among the four
+ * entries around a point's place), and the proposals holding both of their
+ * slots' claims run.  `tile` < 4 (capacities up to 4096; build the items
+ * with the same tile). */
+int amh_refine_knn(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
+                   const int32_t *chain_w_dev, const int32_t *items_dev, int32_t nitems, int32_t tile,
+                   uint64_t pass, int32_t rounds, uint64_t seed, uint32_t *queue_dev,
+                   amh_refine_stats *stats_dev, void *stream);
 /* hipGraph of `npasses` batched launches whose pass numbers come from the
  * queue's counter, so every amh_graph_launch runs the next npasses passes
  * with one submission.  Capture needs a stream made by amh_stream_create. */

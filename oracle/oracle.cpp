@@ -370,6 +370,119 @@ uint64_t or_refine_pass(uint64_t *pts, const int64_t *off, const int32_t *widths
     return acc;
 }
 
+/* kNN-guided batched pass (amh_refine_knn) replayed on the CPU.  The
+ * proposal rule is the reference's OpenCV build of thread::morph
+ * (thread.cpp:893-986): for slot x1, the two row-(y+1) points nearest to
+ * x1's point in row y, the two nearest to x1's own row-(y+1) point, and one
+ * random slot, one of them drawn at random.  "Nearest" is searched among
+ * the four entries around a point's place in the Z-order (Morton) sort of
+ * row y+1, ties to the earlier entry.  All slots propose at once each round;
+ * a proposal runs only when it holds the smallest claim on both of its
+ * slots, so the pairs that run are disjoint. */
+namespace {
+uint32_t zorder_(uint32_t x, uint32_t y) {
+    uint32_t m = 0;
+    for (int b = 0; b < 16; ++b) m |= (((x >> b) & 1u) << (2 * b)) | (((y >> b) & 1u) << (2 * b + 1));
+    return m;
+}
+}  // namespace
+
+uint64_t or_refine_knn_pass(uint64_t *pts, const int64_t *off, const int32_t *widths, int32_t nchains,
+                            int32_t h, int32_t cap, uint64_t pass, int32_t rounds, uint64_t seed,
+                            int64_t *gain_out, uint64_t *attempts_out) {
+    auto pass_stream = [](uint64_t c, uint64_t p, uint64_t g) { return mix64_((c << 32) ^ p) ^ g; };
+    uint64_t acc = 0, att = 0;
+    int64_t gain = 0;
+    if (h >= 2)
+        for (int32_t c = 0; c < nchains; ++c) {
+            const int32_t w = widths[c];
+            if (w < 2) continue;
+            uint64_t *cp = pts + size_t(off[c]) * h;
+            const uint32_t y = uint32_t(mix64_(seed ^ 0x5851F42D4C957F2Dull ^ mix64_((uint64_t(c) << 32) ^ pass)) % uint32_t(h));
+            const uint32_t yp = y > 0 ? y - 1 : h - 1, yn = (y + 1) % h;
+            const int ng = (w + cap - 1) / cap, gs = (w + ng - 1) / ng;
+            const uint64_t pkey = pass_stream(uint64_t(c), pass, 0xFFFFFFFFull);
+            for (int g = 0; g < ng; ++g) {
+                const int g0 = g * gs, n = std::min(gs, w - g0);
+                std::vector<uint32_t> slot(n);
+                std::vector<uint64_t> C(n), Pv(n), Nx(n);
+                for (int l = 0; l < n; ++l) {
+                    slot[l] = ng > 1 ? feistel_(uint32_t(g0 + l), uint32_t(w), pkey) : uint32_t(l);
+                    Pv[l] = cp[size_t(slot[l]) * h + yp];
+                    Nx[l] = cp[size_t(slot[l]) * h + yn];
+                    C[l] = cp[size_t(slot[l]) * h + y];
+                }
+                if (n >= 2) {
+                    // the Z-order of row y+1: entries (key << 16 | slot)
+                    std::vector<uint64_t> order(n);
+                    for (int l = 0; l < n; ++l)
+                        order[l] = (uint64_t(zorder_(px_(Nx[l]), py_(Nx[l]))) << 16) | uint64_t(l);
+                    std::sort(order.begin(), order.end());
+                    std::vector<int> place(n);
+                    for (int k = 0; k < n; ++k) place[order[k] & 0xFFFFu] = k;
+                    auto nearest2 = [&](int lo, int hi, int self, uint64_t at, std::vector<uint32_t> &out) {
+                        int b1 = -1, b2 = -1;
+                        uint64_t d1 = 0, d2 = 0;
+                        for (int k = std::max(lo, 0); k < std::min(hi, n); ++k) {
+                            const int s = int(order[k] & 0xFFFFu);
+                            if (s == self) continue;
+                            const int64_t dx = int64_t(px_(Nx[s])) - px_(at), dy = int64_t(py_(Nx[s])) - py_(at);
+                            const uint64_t dd = uint64_t(dx * dx + dy * dy);
+                            if (b1 < 0 || dd < d1) {
+                                b2 = b1; d2 = d1;
+                                b1 = s; d1 = dd;
+                            } else if (b2 < 0 || dd < d2) {
+                                b2 = s; d2 = dd;
+                            }
+                        }
+                        if (b1 >= 0) out.push_back(uint32_t(b1));
+                        if (b2 >= 0) out.push_back(uint32_t(b2));
+                    };
+                    const uint64_t sid = pass_stream(uint64_t(c), pass, uint64_t(g));
+                    std::vector<uint32_t> part(n), pri(n), claim(n);
+                    for (int r = 0; r < rounds; ++r) {
+                        std::fill(claim.begin(), claim.end(), 0xFFFFFFFFu);
+                        for (int l = 0; l < n; ++l) {
+                            const uint64_t d =
+                                mix64_(seed ^ 0x2545F4914F6CDD1Dull ^ mix64_(sid ^ (uint64_t(r) << 32) ^ uint64_t(l)));
+                            std::vector<uint32_t> cand;
+                            const uint64_t q = C[l];
+                            const uint64_t kq = uint64_t(zorder_(px_(q), py_(q))) << 16;
+                            const int lo = int(std::lower_bound(order.begin(), order.end(), kq) - order.begin());
+                            nearest2(lo - 2, lo + 2, l, q, cand);
+                            const int kd = place[l];
+                            nearest2(kd - 2, kd + 3, l, Nx[l], cand);
+                            const uint32_t u = uint32_t((((d >> 32) & 0xFFFFu) * uint64_t(n - 1)) >> 16);
+                            cand.push_back(u < uint32_t(l) ? u : u + 1u);
+                            part[l] = cand[((d >> 48) * uint64_t(cand.size())) >> 16];
+                            pri[l] = (uint32_t(d) & 0xFFFFF000u) | uint32_t(l);
+                            claim[l] = std::min(claim[l], pri[l]);
+                            claim[part[l]] = std::min(claim[part[l]], pri[l]);
+                        }
+                        for (int i = 0; i < n; ++i) {
+                            const int j = int(part[i]);
+                            if (claim[i] != pri[i] || claim[j] != pri[i]) continue;
+                            int64_t c1 = int64_t(dist_(Pv[i], C[i]) + dist_(C[i], Nx[i]) +
+                                                 dist_(Pv[j], C[j]) + dist_(C[j], Nx[j]));
+                            int64_t c2 = int64_t(dist_(Pv[j], C[i]) + dist_(C[i], Nx[j]) +
+                                                 dist_(Pv[i], C[j]) + dist_(C[j], Nx[i]));
+                            ++att;
+                            if (c1 >= c2) {
+                                std::swap(C[i], C[j]);
+                                gain += c1 - c2;
+                                ++acc;
+                            }
+                        }
+                    }
+                }
+                for (int l = 0; l < n; ++l) cp[size_t(slot[l]) * h + y] = C[l];
+            }
+        }
+    if (gain_out) *gain_out = gain;
+    if (attempts_out) *attempts_out = att;
+    return acc;
+}
+
 /* color.cpp:31-44 */
 uint32_t or_rgb_to_hsp(uint32_t c) {
     rgba_ v = unpack_(c);

@@ -80,6 +80,14 @@ uint64_t or_refine_rounds(uint64_t *pts, size_t w, size_t h, size_t y,
 uint64_t or_refine_pass(uint64_t *pts, const int64_t *off, const int32_t *widths, int32_t nchains,
                         int32_t h, int32_t cap, uint64_t pass, int32_t rounds, uint64_t seed,
                         int64_t *gain_out, uint64_t *attempts_out);
+/* One kNN-guided pass (amh_refine_knn) replayed on the CPU: the schedule of
+ * or_refine_pass with partners proposed by the OpenCV build's nearest-
+ * neighbour rule (thread.cpp:893-986) and overlapping proposals resolved by
+ * the smallest claim.  Returns accepted swaps; exact gain; attempts = the
+ * proposals that ran. */
+uint64_t or_refine_knn_pass(uint64_t *pts, const int64_t *off, const int32_t *widths, int32_t nchains,
+                            int32_t h, int32_t cap, uint64_t pass, int32_t rounds, uint64_t seed,
+                            int64_t *gain_out, uint64_t *attempts_out);
 /* Feistel permutation of [0, n) used by the kernels (exposed for tests). */
 uint32_t or_perm_index(uint32_t i, uint32_t n, uint64_t key);
 uint64_t or_hash64(uint64_t x);

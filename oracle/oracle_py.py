@@ -32,6 +32,7 @@ _ORACLE = {
     "or_morph_visit": (u64, [vp, sz, sz, sz, C.POINTER(u64), C.POINTER(f64)]),
     "or_refine_rounds": (u64, [vp, sz, sz, sz, u64, u64, u64, C.POINTER(i64), C.POINTER(u64)]),
     "or_refine_pass": (u64, [vp, vp, vp, i32, i32, i32, u64, i32, u64, C.POINTER(i64), C.POINTER(u64)]),
+    "or_refine_knn_pass": (u64, [vp, vp, vp, i32, i32, i32, u64, i32, u64, C.POINTER(i64), C.POINTER(u64)]),
     "or_perm_index": (u32, [u32, u32, u64]),
     "or_hash64": (u64, [u64]),
     "or_rgb_to_hsp": (u32, [u32]),
@@ -169,6 +170,17 @@ def refine_pass(pts_atom_major, offsets, widths, h, cap, pas, rounds, seed):
     att = u64(0)
     acc = oracle().or_refine_pass(ptr(pts), ptr(off), ptr(wid), len(wid), h, cap, pas, rounds, seed,
                                   C.byref(g), C.byref(att))
+    return pts, acc, g.value, att.value
+
+
+def refine_knn_pass(pts_atom_major, offsets, widths, h, cap, pas, rounds, seed):
+    pts = np.ascontiguousarray(pts_atom_major, np.uint64).copy()
+    off = np.ascontiguousarray(offsets, np.int64)
+    wid = np.ascontiguousarray(widths, np.int32)
+    g = i64(0)
+    att = u64(0)
+    acc = oracle().or_refine_knn_pass(ptr(pts), ptr(off), ptr(wid), len(wid), h, cap, pas, rounds, seed,
+                                      C.byref(g), C.byref(att))
     return pts, acc, g.value, att.value
 
 

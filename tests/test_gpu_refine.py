@@ -128,6 +128,49 @@ def test_refine_batched_bitexact_all_tiles():
             assert G.to_host(queue, np.int32).tolist() == [0, 0, 3, 0]
 
 
+def test_refine_knn_bitexact():
+    """kNN-guided passes, every variant: the same points, counters and queue
+    state as the oracle's replay, chains split over groups included."""
+    T = G.torch()
+    lib = G.lib()
+    rng = np.random.default_rng(12)
+    widths = [1, 2, 3, 40, 1023, 1025, 2600, 5000, 9000]
+    for h in (2, 3):
+        pts, off, wid = _chains(rng, widths, h)
+        wtot = int(wid.sum())
+        for tile, cap, thr in _tiles()[:4]:
+            items, nitems = _items(wid, tile)
+            dev = G.to_dev(G.key_major(pts, h))
+            off_d, w_d, it_d = G.to_dev(off), G.to_dev(wid.astype(np.int32)), G.to_dev(items)
+            queue = G.zeros(4, T.int32)
+            stats = G.zeros(4, T.int64)
+            want = pts.copy()
+            att_w = acc_w = gain_w = 0
+            for p in range(3):
+                G.log(f"refine_knn h={h} tile={tile} cap={cap} threads={thr} pass={p} items={nitems}")
+                G.check(lib.amh_refine_knn(G.dptr(dev), h, wtot, G.dptr(off_d), G.dptr(w_d), G.dptr(it_d), nitems,
+                                           tile, p, 5, 4321, G.dptr(queue), G.dptr(stats), G.stream()),
+                        "amh_refine_knn")
+                want, acc, gain, att = op.refine_knn_pass(want, off, wid, h, cap, p, 5, 4321)
+                att_w += att; acc_w += acc; gain_w += gain
+            got = G.atom_major(G.to_host(dev, np.uint64), h)
+            assert (got == want).all(), (h, tile)
+            st = G.to_host(stats, np.int64)
+            assert (st[0], st[1], st[2]) == (att_w, acc_w, gain_w)
+            assert G.to_host(queue, np.int32).tolist() == [0, 0, 3, 0]
+
+
+def test_refine_knn_rejects_wide_tiles():
+    T = G.torch()
+    lib = G.lib()
+    dev = G.zeros(8, T.int64)
+    q = G.zeros(4, T.int32)
+    st = G.zeros(4, T.int64)
+    for tile in (-1, 4, 5):
+        assert lib.amh_refine_knn(G.dptr(dev), 2, 2, G.dptr(dev), G.dptr(dev), G.dptr(dev), 1, tile, 0, 1, 0,
+                                  G.dptr(q), G.dptr(st), G.stream()) != 0
+
+
 def test_refine_batched_full_size_properties():
     """The 512^2 two-key-frame workload's shape (40 chains, ~94k atoms) and a
     long run: rows stay permutations, the cost never rises, gain is exact."""

@@ -159,6 +159,89 @@ def test_refine_pass_split_groups_invariants():
             assert sorted(a[:, j].tolist()) == sorted(b[:, j].tolist())
 
 
+@pytest.mark.parametrize("h", [2, 3])
+def test_refine_knn_pass_invariants(h):
+    """kNN-guided passes keep every row a permutation of its points, record
+    the exact cost drop, and run at most one attempt per slot pair claim."""
+    rng = np.random.default_rng(16 + h)
+    widths = [1, 2, 3, 50, 1000, 2500]
+    off = np.concatenate([[0], np.cumsum(widths)[:-1]])
+    p = np.concatenate([_pts(rng, w, h) for w in widths])
+    before = sum(op.chain_cost(p[o_ * h:(o_ + w) * h], h) for o_, w in zip(off, widths))
+    q = p
+    gains = atts = 0
+    for pas in range(3):
+        q, acc, gain, att = op.refine_knn_pass(q, off, widths, h, cap=1024, pas=pas, rounds=6, seed=3)
+        gains += gain
+        atts += att
+        assert acc <= att <= 6 * sum(w // 2 for w in widths)
+    after = sum(op.chain_cost(q[o_ * h:(o_ + w) * h], h) for o_, w in zip(off, widths))
+    assert before - after == gains and gains > 0 and atts > 0
+    for o_, w in zip(off, widths):
+        a, b = p[o_ * h:(o_ + w) * h].reshape(w, h), q[o_ * h:(o_ + w) * h].reshape(w, h)
+        for j in range(h):
+            assert sorted(a[:, j].tolist()) == sorted(b[:, j].tolist())
+
+
+def test_refine_knn_two_slots_is_one_reference_attempt():
+    """Two slots: every candidate is the other slot, so each round runs
+    exactly one attempt, accepted by the reference's rule c1 >= c2
+    (thread.cpp:1007-1022) on the four point distances."""
+    rng = np.random.default_rng(23)
+    for trial in range(200):
+        h = int(rng.integers(2, 5))
+        p = _pts(rng, 2, h, lim=64)
+        q, acc, gain, att = op.refine_knn_pass(p, [0], [2], h, cap=1024, pas=trial, rounds=1, seed=5)
+        assert att == 1
+        a, b = p.reshape(2, h), q.reshape(2, h)
+        rows = [j for j in range(h) if a[0, j] != b[0, j]]
+        y = rows[0] if rows else None
+        if acc:
+            assert len(rows) == 1 and b[0, y] == a[1, y] and b[1, y] == a[0, y]
+        # brute force over the row the pass chose: the swap is taken iff it
+        # does not raise the cost
+        cands = range(h) if y is None else [y]
+        ok = False
+        for yy in cands:
+            sw = a.copy()
+            sw[0, yy], sw[1, yy] = a[1, yy], a[0, yy]
+            c1, c2 = op.chain_cost(a.ravel(), h), op.chain_cost(sw.ravel(), h)
+            if acc and yy == y:
+                ok = c2 <= c1 and c1 - c2 == gain
+            elif not acc:
+                ok = ok or c2 > c1
+        assert ok
+
+
+def test_refine_knn_finds_twins():
+    """Row 1 a shuffled copy of row 0: the optimum is 0, every point's twin
+    sits at distance 0 in the other row, and the nearest-neighbour proposals
+    find it; the same number of uniform random rounds does not get close."""
+    rng = np.random.default_rng(29)
+    w, h = 800, 2
+    xy = rng.choice(256 * 256, w, replace=False)
+    base = op.pack_points(xy % 256, xy // 256, np.zeros(w, int), np.zeros(w, int), np.ones(w, int))
+    pts = np.stack([base, base[rng.permutation(w)]], axis=1).ravel()
+    c0 = op.chain_cost(pts, h)
+    knn, rnd = pts, pts
+    att_k = att_r = 0
+    for pas in range(10):
+        knn, _, _, a = op.refine_knn_pass(knn, [0], [w], h, cap=1024, pas=pas, rounds=8, seed=9)
+        att_k += a
+        rnd, _, _, a = op.refine_pass(rnd, [0], [w], h, cap=1024, pas=pas, rounds=8, seed=9)
+        att_r += a
+
+    def twins(q):
+        q = q.reshape(w, h)
+        return int((q[:, 0] == q[:, 1]).sum())
+    ck, cr = op.chain_cost(knn, h), op.chain_cost(rnd, h)
+    # fewer attempts (only the proposals holding both claims run), more
+    # pairs at distance 0 and a lower cost
+    assert att_k < att_r
+    assert twins(knn) >= w // 2 and twins(knn) > 10 * twins(rnd)
+    assert ck < 0.03 * c0 and ck < 0.6 * cr
+
+
 def test_round_schedule_statistical_parity_with_reference(golden_cfg1):
     """After the same number of attempts, the round schedule's cost is within
     0.5% of the reference descent's on the cfg1 chains (2^16 attempts each,
