@@ -41,6 +41,7 @@ const unsigned char HAS_FLUID         = 2;
 /* Refinement schedules of this build (extension, see morph::set_refine). */
 const unsigned REFINE_PLAIN   = 0; // one chain per step, like thread::morph
 const unsigned REFINE_BATCHED = 1; // all chains per step, one persistent launch
+const unsigned REFINE_KNN     = 2; // as BATCHED, partners from nearest neighbours
 
 typedef struct pixel {
     uint16_t x;

@@ -13,6 +13,8 @@
  *                     replays of 16 passes) instead of one chain per step
  *   --tile N, --rounds N   fixed batched kernel variant / rounds per pass
  *                     instead of the autotuned ones
+ *   --knn             batched refinement with nearest-neighbour partners
+ *                     (the reference's OpenCV build, thread.cpp:893-986)
  *   --learned FILE    learned warm start with the MLP parameters in FILE
  *                     (.npz written by python -m atomorph_amd.learned)
  *   --fluid-sparseness X   fluid grid cells per pixel (reference: 1)
@@ -115,13 +117,15 @@ void usage(FILE *s) {
                  "      --batched                 Refine all chains per step (persistent, autotuned).\n"
                  "      --tile              INT   Batched kernel variant (default: autotuned).\n"
                  "      --rounds            INT   Batched rounds per pass (default: autotuned).\n"
+                 "      --knn                     Batched, with nearest-neighbour swap partners.\n"
                  "      --learned           FILE  Learned warm start with these MLP parameters (.npz).\n"
                  "      --fluid-sparseness  REAL  Fluid grid cells per pixel (default 1).\n"
                  "      --load-chains       FILE  Resume the refinement from a chain checkpoint.\n"
                  "      --save-chains       FILE  Write the refined chains to a checkpoint.\n");
 }
 
-enum { kMatchSteps = 1000, kMorphSteps, kBatched, kLearned, kSparseness, kTile, kRounds, kLoadChains, kSaveChains };
+enum { kMatchSteps = 1000, kMorphSteps, kBatched, kLearned, kSparseness, kTile, kRounds, kLoadChains, kSaveChains,
+       kKnn };
 
 bool parse(int argc, char **argv) {  // demo/options.h:126-224
     opt.name = argv[0];
@@ -170,6 +174,7 @@ bool parse(int argc, char **argv) {  // demo/options.h:126-224
         {"match-steps", required_argument, nullptr, kMatchSteps},
         {"morph-steps", required_argument, nullptr, kMorphSteps},
         {"batched", no_argument, nullptr, kBatched},
+        {"knn", no_argument, nullptr, kKnn},
         {"learned", required_argument, nullptr, kLearned},
         {"fluid-sparseness", required_argument, nullptr, kSparseness},
         {"tile", required_argument, nullptr, kTile},
@@ -213,6 +218,7 @@ bool parse(int argc, char **argv) {  // demo/options.h:126-224
             case kMatchSteps: opt.match_steps = std::atoll(optarg); break;
             case kMorphSteps: opt.morph_steps = std::atoll(optarg); break;
             case kBatched: opt.batched = 1; break;
+            case kKnn: opt.batched = 2; break;
             case kLearned: opt.learned = optarg; break;
             case kSparseness: opt.fluid_sparseness = std::atof(optarg); break;
             case kTile: opt.tile = std::atoi(optarg); break;
@@ -445,7 +451,8 @@ int main(int argc, char **argv) {
         m.set_show_blobs((unsigned char)opt.show_blobs);
         m.set_fluid(opt.fluid);
         if (opt.fluid_sparseness > 0.0) m.set_fluid_sparseness(opt.fluid_sparseness);
-        if (opt.batched) m.set_refine(am::REFINE_BATCHED, unsigned(opt.rounds), opt.tile, 16);
+        if (opt.batched == 1) m.set_refine(am::REFINE_BATCHED, unsigned(opt.rounds), opt.tile, 16);
+        if (opt.batched == 2) m.set_refine(am::REFINE_KNN, unsigned(opt.rounds), opt.tile, 0);
         std::vector<float> params;
         if (!opt.learned.empty()) {
             std::string err;

@@ -830,16 +830,25 @@ bool morph::engine::autotune(double budget_ms, int rounds_fixed, std::vector<amh
     return true;
 }
 
+// kNN-guided refinement: its kernel variants (the first four tiles), the
+// default one (4096 slots, 512 threads) and rounds per pass.
+constexpr int kKnnTiles = 4, kKnnTile = 3, kKnnRounds = 16;
+
 // ATOM_MORPHING: one refinement step.  Plain: one visit of one chain
 // (thread::morph, thread.cpp:1043-1068: chain counter % W, one random key
 // frame, cycle_length attempts as conflict-free rounds).  Batched: one pass
-// over every chain, or one hipGraph replay of graph_passes passes.
+// over every chain, or one hipGraph replay of graph_passes passes.  kNN: one
+// pass over every chain with nearest-neighbour partners (thread.cpp:893-986).
 bool morph::engine::morph_step() {
     const size_t nch = cw.size();
     if (nch == 0 || h < 2) return false;
-    if (wopt.refine_mode == REFINE_BATCHED) {
+    if (wopt.refine_mode == REFINE_BATCHED || wopt.refine_mode == REFINE_KNN) {
+        const bool knn = wopt.refine_mode == REFINE_KNN;
         int tile = wopt.refine_tile, rounds = int(wopt.refine_rounds);
-        if (tile < 0 || rounds <= 0) {  // autotuned: once per chain shape (and fixed rounds)
+        if (knn) {  // kernel variants of up to 4096 slots; not autotuned
+            if (tile < 0 || tile >= kKnnTiles) tile = kKnnTile;
+            if (rounds <= 0) rounds = kKnnRounds;
+        } else if (tile < 0 || rounds <= 0) {  // autotuned: once per chain shape (and fixed rounds)
             const int fixed = rounds > 0 ? rounds : 0;
             if (tuned_tile < 0 || tuned_fixed != fixed) {
                 tuned_tile = -1;
@@ -870,7 +879,13 @@ bool morph::engine::morph_step() {
             items_tile = tile;
         }
         if (nitems == 0) return false;
-        if (wopt.graph_passes > 0) {
+        if (knn) {
+            if (amh_refine_knn(d_pts.as<uint64_t>(), h, wtot, d_off.as<int64_t>(), d_w.as<int32_t>(),
+                               d_items.as<int32_t>(), nitems, tile, passes, rounds, mix_seed(),
+                               d_queue.as<uint32_t>(), d_rstats.as<amh_refine_stats>(), cstream))
+                return fail("refine knn");
+            ++passes;
+        } else if (wopt.graph_passes > 0) {
             if (!graph || graph_tile != tile || graph_np != wopt.graph_passes || graph_rounds != unsigned(rounds)) {
                 if (graph) amh_graph_destroy(graph);
                 graph = nullptr;

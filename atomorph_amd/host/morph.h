@@ -134,7 +134,11 @@ class morph {
     // visit, 0 = cycle_length attempts like the reference) or REFINE_BATCHED
     // (`rounds` rounds per pass, kernel variant `tile`; rounds 0 or tile < 0
     // = autotuned on the first pass over new chains, see autotune) with
-    // `graph_passes` passes per hipGraph replay (0 = no graph).
+    // `graph_passes` passes per hipGraph replay (0 = no graph), or REFINE_KNN
+    // (batched passes whose swap partners come from a nearest-neighbour
+    // search of the next key frame's row, as the reference's OpenCV build
+    // draws them; `tile` 0-3, default 3; `rounds` per pass, default 16; no
+    // graph).
     void        set_refine          (unsigned mode, unsigned rounds, int tile, unsigned graph_passes);
     // Autotunes the batched descent on the current chains now (paused in
     // ATOM_MORPHING): every kernel variant x rounds per pass races from the

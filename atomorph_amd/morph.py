@@ -17,7 +17,7 @@ import numpy as np
 from . import _capi
 
 STATE_BLOB_DETECTION, STATE_BLOB_UNIFICATION, STATE_BLOB_MATCHING, STATE_ATOM_MORPHING, STATE_DONE = range(5)
-REFINE_PLAIN, REFINE_BATCHED = 0, 1
+REFINE_PLAIN, REFINE_BATCHED, REFINE_KNN = 0, 1, 2
 
 vp, sz, u64, i64 = C.c_void_p, C.c_size_t, C.c_uint64, C.c_int64
 

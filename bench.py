@@ -10,9 +10,10 @@ frames are uploaded before the timed region.
 
 Printed (one JSON line): frames/s of the whole job, plus
   time_to_cost  plain GPU descent (one chain at a time) run to convergence vs
-                the batched persistent descent, and the learned warm start +
-                batched descent, timed until they reach the plain descent's
-                final cost;
+                the batched persistent descent, the learned warm start +
+                batched descent, and the batched descent with nearest-
+                neighbour partners, timed until they reach the plain
+                descent's final cost;
   roofline      the dominant kernel's achieved bytes/s against 8 TB/s, from
                 per-kernel device times of one extra profiled job;
   mlp           the warm start's matrix-core forward against the BF16 peak;
@@ -121,7 +122,9 @@ def time_to_cost(m, M, learned=None, log_every=20):
     out["batched"] = race(batched)
     if learned is not None:
         out["learned_batched"] = race(batched, learned)
-    for k in ("batched", "learned_batched"):
+    # the non-learned proposal baseline: nearest-neighbour partners
+    out["knn_batched"] = race(lambda: m.set_refine(M.REFINE_KNN, rounds=16, tile=3, graph_passes=0))
+    for k in ("batched", "learned_batched", "knn_batched"):
         if k in out:
             b = out[k]["ms_to_within_0.5pct_of_plain_cost"]
             out[f"ratio_{k}_vs_plain"] = round(b / t_plain, 5) if (b is not None and t_plain > 0) else None

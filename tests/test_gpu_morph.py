@@ -106,8 +106,14 @@ def test_refinement_plain_then_batched_then_graph(golden_cfg1):
     m.run(5)
     c3 = m.cost()
     assert c3 <= c2
+    m.set_refine(M.REFINE_KNN, rounds=8, tile=1, graph_passes=0)
+    G.log("knn refine")
+    m.run(10)
+    c4 = m.cost()
+    assert c4 <= c3
     a3, _ = m.counters()
     assert a3 > a1
+    c3 = c4
     # rows remain permutations of the initial rows
     pts3, _ = m.chains()
     off = np.concatenate([[0], np.cumsum(widths)[:-1]])
