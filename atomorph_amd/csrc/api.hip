@@ -73,6 +73,26 @@ void prof_end(const char *kernel, hipStream_t s) {
     }
 }
 
+bool prof_take(hipEvent_t &start, hipEvent_t &stop) {
+    std::lock_guard<std::mutex> lk(prof::mu);
+    if (!prof::pool.empty()) {
+        start = prof::pool.back().first;
+        stop = prof::pool.back().second;
+        prof::pool.pop_back();
+        return true;
+    }
+    return hipEventCreate(&start) == hipSuccess && hipEventCreate(&stop) == hipSuccess;
+}
+
+void prof_put(const char *kernel, hipEvent_t start, hipEvent_t stop) {
+    prof::Pending p;
+    p.name = kernel;
+    p.a = start;
+    p.b = stop;
+    std::lock_guard<std::mutex> lk(prof::mu);
+    prof::pending.push_back(p);
+}
+
 }  // namespace amh
 
 using amh::check_hip;

@@ -1,5 +1,6 @@
 // Shared device helpers for the atomorph MI355X kernels (gfx950, wave64).
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -174,6 +175,25 @@ struct ProfScope {
         if (profiling()) prof_end(k, s);
     }
 };
+// An event pair for one timed launch of `kernel` (false: none available),
+// and its hand-back once the launch is enqueued.
+bool prof_take(hipEvent_t &start, hipEvent_t &stop);
+void prof_put(const char *kernel, hipEvent_t start, hipEvent_t stop);
+
+// Launches one kernel.  With per-kernel timing on, the dispatch itself
+// records the start and stop events (hipExtLaunchKernelGGL), so the time is
+// the kernel's own execution, as a kernel trace reports it, without the
+// gaps that events recorded around the launch add.
+template <typename F, typename... Args>
+void launch_timed(const char *name, F kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t s, Args... args) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (profiling() && prof_take(a, b)) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, a, b, 0, args...);
+        prof_put(name, a, b);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+    }
+}
 }  // namespace amh
 
 #define AMH_LAUNCH_CHECK(what)                                              \

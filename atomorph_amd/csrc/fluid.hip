@@ -1282,24 +1282,15 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
         f->bin_par ^= 1;
         bin_particles(StencilTiles{Q, f->gsx, f->gsy, f->G}, f->count, B, tiles, par, s, counted);
     }
-    {
-        amh::ProfScope ps_("k_p2g", s);
-        hipLaunchKernelGGL(chunked ? k_p2g<true> : k_p2g<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
-    }
-    {
-        amh::ProfScope ps_("k_forces", s);
-        hipLaunchKernelGGL(chunked ? k_forces<true> : k_forces<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
-    }
-    {
-        amh::ProfScope ps_("k_g2p", s);
-        hipLaunchKernelGGL(chunked ? k_g2p<true> : k_g2p<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
-    }
-    {
-        amh::ProfScope ps_("k_advect", s);
-        hipLaunchKernelGGL(k_advect, dim3(uint32_t((f->count + kT - 1) / kT)), dim3(kT), 0, s, Q, f->count, f->G,
-                           f->gsx, f->gsy, int(steps_left), freedom_radius, seed, atom, B,
-                           count_next ? f->bin_par : -1);
-    }
+    amh::launch_timed("k_p2g", chunked ? k_p2g<true> : k_p2g<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx,
+                      f->gsy, B);
+    amh::launch_timed("k_forces", chunked ? k_forces<true> : k_forces<false>, dim3(grid), dim3(kT), 0, s, Q, f->G,
+                      f->gsx, f->gsy, B);
+    amh::launch_timed("k_g2p", chunked ? k_g2p<true> : k_g2p<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx,
+                      f->gsy, B);
+    amh::launch_timed("k_advect", k_advect, dim3(uint32_t((f->count + kT - 1) / kT)), dim3(kT), 0, s, Q, f->count,
+                      f->G, f->gsx, f->gsy, int(steps_left), freedom_radius, seed, atom, B,
+                      count_next ? f->bin_par : -1);
     AMH_LAUNCH_CHECK("amh_fluid_step");
     return AMH_OK;
 }
@@ -1723,11 +1714,8 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
     const uint32_t gpx = uint32_t((npx + kT - 1) / kT);
     if (F.fk < R.nkeys && f->count > 0) {
         const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
-        {
-            amh::ProfScope ps_("k_fluid_update", s);
-            hipLaunchKernelGGL(k_fluid_update, dim3(gp), dim3(kT), 0, s, R, F, f->P, fa->rank_dev, fa->surface_dev,
-                               fa->blob_col_dev, fa->sparseness);
-        }
+        amh::launch_timed("k_fluid_update", k_fluid_update, dim3(gp), dim3(kT), 0, s, R, F, f->P, fa->rank_dev,
+                          fa->surface_dev, fa->blob_col_dev, fa->sparseness);
         // morph.cpp:1050-1054
         const double freedom = sqrt(double(f->W * f->W + f->H * f->H)) * fa->sparseness / 4.0;
         const double radius = freedom * (1.0 - F.tl);

@@ -576,12 +576,9 @@ extern "C" int amh_refine_batched(uint64_t *pts_dev, int32_t nkeys, int64_t wtot
     const int grid = std::min(tile_grid(tile), int(nitems));
     if (grid <= 0) return check_hip(hipGetLastError(), "amh_refine_batched occupancy");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    {
-        amh::ProfScope ps_("k_refine_pass", s);
-        hipLaunchKernelGGL(kTiles[tile].fn, dim3(grid), dim3(kTiles[tile].threads), 0, s, pts_dev, nkeys, wtot,
-                           chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed, queue_dev,
-                           stats_dev);
-    }
+    amh::launch_timed("k_refine_pass", kTiles[tile].fn, dim3(grid), dim3(kTiles[tile].threads), 0, s, pts_dev, nkeys,
+                      wtot, chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed, queue_dev,
+                      stats_dev);
     AMH_LAUNCH_CHECK("amh_refine_batched");
     return AMH_OK;
 }
@@ -603,12 +600,9 @@ extern "C" int amh_refine_knn(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, co
     const int grid = std::min(resident_grid(kKnnTiles[tile], g_knn_grid[tile]), int(nitems));
     if (grid <= 0) return check_hip(hipGetLastError(), "amh_refine_knn occupancy");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    {
-        amh::ProfScope ps_("k_refine_knn", s);
-        hipLaunchKernelGGL(kKnnTiles[tile].fn, dim3(grid), dim3(kKnnTiles[tile].threads), 0, s, pts_dev, nkeys, wtot,
-                           chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed, queue_dev,
-                           stats_dev);
-    }
+    amh::launch_timed("k_refine_knn", kKnnTiles[tile].fn, dim3(grid), dim3(kKnnTiles[tile].threads), 0, s, pts_dev,
+                      nkeys, wtot, chain_off_dev, chain_w_dev, items_dev, int(nitems), pass, int(rounds), seed,
+                      queue_dev, stats_dev);
     AMH_LAUNCH_CHECK("amh_refine_knn");
     return AMH_OK;
 }

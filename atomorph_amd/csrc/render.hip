@@ -449,25 +449,17 @@ extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t 
     const uint32_t gpx = uint32_t((L.npx + 255) / 256);
     const uint32_t gat = uint32_t((Rc.wtot + 255) / 256);
     if (Rc.wtot > 0) {
-        {
-            amh::ProfScope ps_("k_prepare", s);
-            hipLaunchKernelGGL(k_prepare, dim3(gat), dim3(256), 0, s, Rc, F, recs, bmin);
-        }
-        {
-            amh::ProfScope ps_("k_splat", s);
-            hipLaunchKernelGGL(k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc, ovcount);
-        }
+        amh::launch_timed("k_prepare", k_prepare, dim3(gat), dim3(256), 0, s, Rc, F, recs, bmin);
+        amh::launch_timed("k_splat", k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc,
+                          ovcount);
     }
     scan::exclusive(ovcount, L.npx, sums, offs, ovstart, s);
     if (Rc.wtot > 0)
         hipLaunchKernelGGL(k_overflow, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, ovcount, ovstart,
                            ov);
     const int separate_clear = Rc.feather > 0 ? 1 : 0;
-    {
-        amh::ProfScope ps_("k_composite", s);
-        hipLaunchKernelGGL(k_composite, dim3(gpx), dim3(256), 0, s, Rc, F, bmin, acc, ovstart, ov, out,
-                           separate_clear ? 0 : 1);
-    }
+    amh::launch_timed("k_composite", k_composite, dim3(gpx), dim3(256), 0, s, Rc, F, bmin, acc, ovstart, ov, out,
+                      separate_clear ? 0 : 1);
     if (separate_clear)
         hipLaunchKernelGGL(k_fill_i32, dim3(gpx), dim3(256), 0, s, bmin, L.npx, 0x7FFFFFFF);
     AMH_LAUNCH_CHECK("amh_render_frame");

@@ -562,11 +562,8 @@ extern "C" int amh_blob_stats(const int32_t *labels, const uint32_t *stored, int
     const int64_t n = int64_t(W) * H;
     if (nblobs > 0)
         hipLaunchKernelGGL(k_stats_init, dim3((nblobs + 255) / 256), dim3(256), 0, s, st, int(nblobs));
-    {
-        amh::ProfScope ps_("k_blob_stats", s);
-        hipLaunchKernelGGL(k_blob_stats, dim3(uint32_t((n + kStatChunk - 1) / kStatChunk)), dim3(kStatThreads), 0, s,
-                           labels, stored, int(W), n, static_cast<const int *>(ws), bop, st);
-    }
+    amh::launch_timed("k_blob_stats", k_blob_stats, dim3(uint32_t((n + kStatChunk - 1) / kStatChunk)),
+                      dim3(kStatThreads), 0, s, labels, stored, int(W), n, static_cast<const int *>(ws), bop, st);
     AMH_LAUNCH_CHECK("amh_blob_stats");
     return AMH_OK;
 }
