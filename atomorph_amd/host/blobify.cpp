@@ -58,6 +58,7 @@ struct Frame {
     size_t first_expansion = 0;    // [0, first_expansion): blobs that cannot expand
     size_t first_dust = 0;         // [0, first_dust): blobs unification is done with
     uint32_t stamp = 0;
+    bool started = false;          // the one-pixel blobs were made
     std::vector<int32_t> erased;   // border entries to drop after a pass
 
     bool present(int i) const { return (px[i] >> 24) != 0; }
@@ -103,6 +104,7 @@ void merge_means(Blob &into, const Blob &from) {
 
 // One blob per pixel in canvas order, then shuffled (thread.cpp:231-265).
 bool start_frame(Frame &f, const options &o, std::default_random_engine &e1) {
+    f.started = true;
     const int W = f.W, H = f.H;
     const int32_t npx = W * H;
     f.parent.assign(npx, -1);
@@ -139,7 +141,9 @@ bool start_frame(Frame &f, const options &o, std::default_random_engine &e1) {
 
 // One detection step on a frame (thread.cpp:225-412).  true: done.
 bool blobify_step(Frame &f, const options &o, std::default_random_engine &e1) {
-    if (f.blobs.empty()) return start_frame(f, o, e1);
+    // the reference makes the blobs whenever the list is empty; for a frame
+    // without pixels that makes nothing and draws nothing, every step
+    if (f.blobs.empty()) return f.started ? true : start_frame(f, o, e1);
     size_t sz;
     int32_t id;
     for (;;) {  // drop empty slots and empty blobs from the end

@@ -242,15 +242,18 @@ def run_cfg1():
 
 
 # Blob options of the greedy cases: threshold, max size, min size, blob
-# number, box samples, box grip, delimiter (0 RGB, 1 HSP), seed.
+# number, box samples, box grip, delimiter (0 RGB, 1 HSP), seed, and the key
+# frame left without pixels (-1: none).
 BLOBIFY_CASES = [
-    (0.1, None, 1, 1, 10, 65535, 1, 1),
-    (0.05, None, 8, 1, 10, 65535, 1, 2),
-    (0.2, 30, 1, 1, 10, 65535, 1, 3),
-    (1.0, None, 1, 3, 10, 65535, 1, 4),
-    (0.1, None, 20, 1, 4, 6, 1, 5),
-    (0.15, None, 1, 20, 10, 65535, 0, 6),
-    (1.0, None, 40, 1, 10, 65535, 1, 7),
+    (0.1, None, 1, 1, 10, 65535, 1, 1, -1),
+    (0.05, None, 8, 1, 10, 65535, 1, 2, -1),
+    (0.2, 30, 1, 1, 10, 65535, 1, 3, -1),
+    (1.0, None, 1, 3, 10, 65535, 1, 4, -1),
+    (0.1, None, 20, 1, 4, 6, 1, 5, -1),
+    (0.15, None, 1, 20, 10, 65535, 0, 6, -1),
+    (1.0, None, 40, 1, 10, 65535, 1, 7, -1),
+    (0.1, None, 6, 1, 10, 65535, 1, 8, 1),
+    (0.3, 200, 5, 2, 3, 4, 1, 9, 0),
 ]
 SIZE_MAX = (1 << 64) - 1
 
@@ -280,8 +283,10 @@ def blobify_fixture():
     W, H, nf = 32, 28, 3
     d = {"bf_size": np.array([W, H, nf], np.int32)}
     frames, stored, opts_f, opts_i, bop, nblobs, means, groups, sizes = [], [], [], [], [], [], [], [], []
-    for (thr, mx, mn, num, samples, grip, delim, seed) in BLOBIFY_CASES:
+    for (thr, mx, mn, num, samples, grip, delim, seed, empty) in BLOBIFY_CASES:
         fr = noisy_frames(rng, nf, W, H)
+        if empty >= 0:
+            fr[empty] = np.zeros_like(fr[empty])
         frames.append(np.stack(fr))
         st = []
         for img in fr:

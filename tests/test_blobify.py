@@ -3,10 +3,10 @@ path the engine takes for non-default blob options (thread.cpp:225-596).
 
 - The blob lists of every key frame -- order, running means, starting pixel,
   sizes and membership -- equal the compiled reference's (fixture
-  tests/golden/blobify.npz) for seven option sets: thresholds below 1, a
+  tests/golden/blobify.npz) for nine option sets: thresholds below 1, a
   size cap, minimum sizes (dust unification and the threshold-free retry of
   undersized blobs), preferred blob counts, a grip box, RGB and HSP stored
-  colours.
+  colours, and a key frame without pixels (first or middle).
 - At threshold 1 with the other options at their defaults the partition is
   the 4-connected components of the foreground (scipy's labelling), whatever
   the seed: the reduction the device segmentation relies on.
@@ -19,7 +19,7 @@ from scipy import ndimage
 
 from atomorph_amd import morph as M
 
-CASES = range(7)
+CASES = range(9)
 
 
 @pytest.fixture(scope="module")

@@ -19,13 +19,15 @@ pytestmark = pytest.mark.gpu
 CLI = os.path.join(A.LIB_DIR, "atomorph_cli")
 
 
-def _library_run(frames, match_steps, morph_steps, frames_out, learned=None, batched=False):
+def _library_run(frames, match_steps, morph_steps, frames_out, learned=None, batched=False, knn=False, **blob):
     """The driver's counted loop through the library API (CLI defaults:
-    density 2, degeneration 10000, cycle length 100000)."""
+    density 2, degeneration 10000, cycle length 100000, 100 dust samples)."""
     m = M.Morph()
-    m.set(seed=1, cycle_length=100000, degeneration=10000, density=2, threads=8)
+    m.set(seed=1, cycle_length=100000, degeneration=10000, density=2, threads=8, blob_box_samples=100, **blob)
     if batched:
         m.set_refine(M.REFINE_BATCHED, rounds=8, tile=2, graph_passes=16)
+    if knn:
+        m.set_refine(M.REFINE_KNN, rounds=8, tile=1, graph_passes=0)
     if learned is not None:
         m.set_learned(learned)
     for i, f in enumerate(frames):
@@ -43,7 +45,7 @@ def _library_run(frames, match_steps, morph_steps, frames_out, learned=None, bat
     return [m.render(m.get_time(i, frames_out)) for i in range(frames_out)]
 
 
-@pytest.mark.parametrize("variant", ["plain", "batched_learned"])
+@pytest.mark.parametrize("variant", ["plain", "batched_learned", "knn_greedy_blobs"])
 def test_cli_frames_equal_library_frames(tmp_path, variant):
     if not os.path.exists(CLI):
         pytest.fail("atomorph_cli not built: python -m atomorph_amd.build")
@@ -57,13 +59,20 @@ def test_cli_frames_equal_library_frames(tmp_path, variant):
     cmd = [CLI, "-i", str(tmp_path), "-o", str(out), "-f", "t", "-s", "1", "-F", "8", "--match-steps", "500",
            "--morph-steps", "6"]
     learned = None
+    blob = {}
     if variant == "batched_learned":
         learned = L.load_default()
         cmd += ["--batched", "--tile", "2", "--rounds", "8", "--learned", L.DEFAULT_PARAMS]
+    if variant == "knn_greedy_blobs":
+        # nearest-neighbour refinement; greedy blobs below the default colour
+        # threshold (-t 40 = 40/255) with dust unification (-m 6)
+        cmd += ["--knn", "--tile", "1", "--rounds", "8", "-t", "40", "-m", "6"]
+        blob = dict(blob_threshold=40 / 255.0, blob_min_size=6)
     G.log(f"cli {variant}: {' '.join(cmd[1:])}")
     r = subprocess.run(cmd + names, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    want = _library_run(frames, 500, 6, 8, learned=learned, batched=variant == "batched_learned")
+    want = _library_run(frames, 500, 6, 8, learned=learned, batched=variant == "batched_learned",
+                        knn=variant == "knn_greedy_blobs", **blob)
     for i in range(8):
         got = P.read_rgba8(out / f"t_{i + 1:04d}.png")
         assert got.shape == (64, 64, 4)

@@ -294,7 +294,7 @@ def _blobify_fixture():
     return dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "blobify.npz")))
 
 
-@pytest.mark.parametrize("c", range(7))
+@pytest.mark.parametrize("c", range(9))
 def test_greedy_blob_options_through_the_engine(c):
     """Non-default blob options: detection runs the host greedy merge and
     dust unification over the device-ingested stored colours; the blobs the

@@ -135,7 +135,7 @@ def test_refine_knn_bitexact():
     lib = G.lib()
     rng = np.random.default_rng(12)
     widths = [1, 2, 3, 40, 1023, 1025, 2600, 5000, 9000]
-    for h in (2, 3):
+    for h in (2, 3, 4):
         pts, off, wid = _chains(rng, widths, h)
         wtot = int(wid.sum())
         for tile, cap, thr in _tiles()[:4]:
