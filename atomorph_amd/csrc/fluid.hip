@@ -45,10 +45,16 @@ constexpr int kParticleInts = 5;
 // FP32 node fields, [gsx * gsy], node (i, j) at i * gsy + j.  FluidModel's
 // node density accumulates the same weights as its mass, so one field (m)
 // serves both.
+// Node fields, FP32.  Mass, momentum and colour weight are one 16-byte
+// record per node and the colour sums another, so advection reads a node's
+// values with two loads; the density gradient and accelerations are read
+// in tiles through LDS and stay one array per field.
 struct Grid {
-    float *m, *gx, *gy, *u, *v, *ax, *ay, *r, *g, *b, *a, *w;
+    float *muvw;  // [nodes][4]: m, u, v, w
+    float *col;   // [nodes][4]: r, g, b, a
+    float *gx, *gy, *ax, *ay;
 };
-constexpr int kGridFields = 12;
+constexpr int kGridFields = 12;  // floats per node
 
 __device__ __forceinline__ void weights(double x, double y, int &cx, int &cy, double px[3], double py[3],
                                         double gx[3], double gy[3]) {
@@ -205,10 +211,11 @@ __device__ __forceinline__ float nd(const float *f, int i, int j, int gsx, int g
 // them by the node mass where it is positive (fluidmodel.cpp:359-364,
 // 437-443); the readers divide, so no pass over the grid is needed.
 __device__ __forceinline__ float per_mass(float v, float m) { return m > 0.0f ? v / m : v; }
-__device__ __forceinline__ float nd_m(const float *f, const float *m, int i, int j, int gsx, int gsy) {
+// f[k] / mass of node (i, j); muvw holds the mass of node k at 4k.
+__device__ __forceinline__ float nd_m(const float *f, const float *muvw, int i, int j, int gsx, int gsy) {
     if (!node_ok(i, j, gsx, gsy)) return 0.0f;
     const int64_t k = int64_t(i) * gsy + j;
-    return per_mass(f[k], m[k]);
+    return per_mass(f[k], muvw[4 * k]);
 }
 
 // pressure from the bicubic density interpolation, wall forces
@@ -335,11 +342,17 @@ struct StencilTiles {  // the step: tiles holding the 3x3 stencil nodes
     // Zeroes every field of tile t's nodes (the workgroup together).
     __device__ __forceinline__ void clear_tile(int t, int nty) const {
         const int x0 = (t / nty) * kTile, y0 = (t % nty) * kTile;
-        float *f[kGridFields] = {G.m, G.gx, G.gy, G.u, G.v, G.ax, G.ay, G.r, G.g, G.b, G.a, G.w};
         for (int e = threadIdx.x; e < kTile * kTile * kGridFields; e += kT) {
             const int node = e % (kTile * kTile), field = e / (kTile * kTile);
             const int ni = x0 + node / kTile, nj = y0 + node % kTile;
-            if (ni < gsx && nj < gsy) f[field][int64_t(ni) * gsy + nj] = 0.0f;
+            if (ni >= gsx || nj >= gsy) continue;
+            const int64_t k = int64_t(ni) * gsy + nj;
+            if (field < 4) G.muvw[4 * k + field] = 0.0f;
+            else if (field < 8) G.col[4 * k + field - 4] = 0.0f;
+            else {
+                float *f[4] = {G.gx, G.gy, G.ax, G.ay};
+                f[field - 8][k] = 0.0f;
+            }
         }
     }
 };
@@ -686,8 +699,11 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
             const int node = threadIdx.x, ni = x0 + node / kTile, nj = y0 + node % kTile;
             if (ni < gsx && nj < gsy) {
                 const int64_t k = int64_t(ni) * gsy + nj;
-                float *f[8] = {Gd.m, Gd.gx, Gd.gy, Gd.r, Gd.g, Gd.b, Gd.a, Gd.w};
-                for (int q = 0; q < 8; ++q) put_node(f[q], k, part[0][q][node] + part[1][q][node] + part[2][q][node], single);
+                // fields m, gx, gy, r, g, b, a, w: their arrays and indices
+                float *f[8] = {Gd.muvw, Gd.gx, Gd.gy, Gd.col, Gd.col, Gd.col, Gd.col, Gd.muvw};
+                const int64_t at[8] = {4 * k, k, k, 4 * k, 4 * k + 1, 4 * k + 2, 4 * k + 3, 4 * k + 3};
+                for (int q = 0; q < 8; ++q)
+                    put_node(f[q], at[q], part[0][q][node] + part[1][q][node] + part[2][q][node], single);
             }
         }
         __syncthreads();  // the LDS sums are reused by the next item
@@ -708,14 +724,16 @@ __device__ __forceinline__ void tile_add2(float (&acc)[2][kTile * kTile], const 
     }
 }
 
-__device__ __forceinline__ void store_tile2(const float (&acc)[2][kTile * kTile], float *f0, float *f1, int x0, int y0,
-                                            int gsx, int gsy, bool single) {
+// Stores (or adds) the tile's two LDS sums into fields f0, f1: node k's
+// values at f0[stride*k], f1[stride*k].
+__device__ __forceinline__ void store_tile2(const float (&acc)[2][kTile * kTile], float *f0, float *f1, int stride,
+                                            int x0, int y0, int gsx, int gsy, bool single) {
     for (int t = threadIdx.x; t < kTile * kTile; t += kT) {
         const int ni = x0 + t / kTile, nj = y0 + t % kTile;
         if (ni >= gsx || nj >= gsy) continue;
         const int64_t k = int64_t(ni) * gsy + nj;
-        put_node(f0, k, acc[0][t], single);
-        put_node(f1, k, acc[1][t], single);
+        put_node(f0, stride * k, acc[0][t], single);
+        put_node(f1, stride * k, acc[1][t], single);
     }
 }
 
@@ -734,7 +752,7 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
         for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
         const auto stage = [&] {
             stage_halo(nodes, x0, y0, gsx, gsy,
-                       [&](int f, int64_t k) { return f == 0 ? Gd.m[k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
+                       [&](int f, int64_t k) { return f == 0 ? Gd.muvw[4 * k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
             __syncthreads();
         };
         const auto node = [&](int f, int i, int j) { return nodes[f][halo_index(i, j, x0, y0)]; };
@@ -764,7 +782,7 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
                 }
         }, stage);
         __syncthreads();
-        store_tile2(acc, Gd.ax, Gd.ay, x0, y0, gsx, gsy, single);
+        store_tile2(acc, Gd.ax, Gd.ay, 1, x0, y0, gsx, gsy, single);
         __syncthreads();  // the LDS sums are reused by the next item
     }
 }
@@ -791,7 +809,7 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
         for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
         const auto stage = [&] {
             stage_halo(accel, x0, y0, gsx, gsy,
-                       [&](int f, int64_t k) { return per_mass(f == 0 ? Gd.ax[k] : Gd.ay[k], Gd.m[k]); });
+                       [&](int f, int64_t k) { return per_mass(f == 0 ? Gd.ax[k] : Gd.ay[k], Gd.muvw[4 * k]); });
             __syncthreads();
         };
         auto fetch = [&](int slot) {
@@ -824,8 +842,8 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
                                 ax = accel[0][e];
                                 ay = accel[1][e];
                             } else {
-                                ax = nd_m(Gd.ax, Gd.m, cx + a, cy + b, gsx, gsy);
-                                ay = nd_m(Gd.ay, Gd.m, cx + a, cy + b, gsx, gsy);
+                                ax = nd_m(Gd.ax, Gd.muvw, cx + a, cy + b, gsx, gsy);
+                                ay = nd_m(Gd.ay, Gd.muvw, cx + a, cy + b, gsx, gsy);
                             }
                             u += phi * (ax + cax);
                             v += phi * (ay + cay);
@@ -852,7 +870,7 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
                 }
         }, stage);
         __syncthreads();
-        store_tile2(acc, Gd.u, Gd.v, x0, y0, gsx, gsy, single);
+        store_tile2(acc, Gd.muvw + 1, Gd.muvw + 2, 4, x0, y0, gsx, gsy, single);
         __syncthreads();  // the LDS sums are reused by the next item
     }
 }
@@ -878,17 +896,18 @@ __device__ __forceinline__ double2 advect_one(Particles P, int i, const Grid &Gd
             if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
             const int64_t k = int64_t(cx + a) * gsy + cy + b;
             const double phi = px[a] * py[b];
-            const float mk = Gd.m[k];
-            gu += phi * per_mass(Gd.u[k], mk);
-            gv += phi * per_mass(Gd.v[k], mk);
-            const double w = Gd.w[k];
+            const float4 q = reinterpret_cast<const float4 *>(Gd.muvw)[k];  // m, u, v, w
+            const float4 c = reinterpret_cast<const float4 *>(Gd.col)[k];   // r, g, b, a
+            gu += phi * per_mass(q.y, q.x);
+            gv += phi * per_mass(q.z, q.x);
+            const double w = q.w;
             if (w > 0.0) {
                 // node colour = sum(s c)/sum(s); weighted by sum(s)
                 wsum += w;
-                nR += Gd.r[k];
-                nG += Gd.g[k];
-                nB += Gd.b[k];
-                nA += Gd.a[k];
+                nR += c.x;
+                nG += c.y;
+                nB += c.z;
+                nA += c.w;
             }
         }
     if (wsum > 0.0) {  // colour diffusion (fluidmodel.cpp:470-495)
@@ -1081,7 +1100,7 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
            (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) +
            max_chunks(tiles, n) * 16 + 4 + 3 * 256 +
-           nn * 4 * kGridFields + npx * 4 * 2 + 64 * 256;
+           nn * 4 * kGridFields + 6 * 256 + npx * 4 * 2 + 64 * 256;
 }
 
 void carve_particles(Particles &Q, size_t n, const std::function<char *(size_t)> &take) {
@@ -1144,10 +1163,10 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->sxy = reinterpret_cast<uint32_t *>(take(n * 4));
     f->staps = reinterpret_cast<double *>(take(n * 32));
     f->scol = reinterpret_cast<uint32_t *>(take(n * 4));
-    float **g[kGridFields] = {&f->G.m, &f->G.gx, &f->G.gy, &f->G.u, &f->G.v, &f->G.ax,
-                              &f->G.ay, &f->G.r, &f->G.g, &f->G.b, &f->G.a, &f->G.w};
-    float *grid = reinterpret_cast<float *>(take(nn * 4 * kGridFields));
-    for (int k = 0; k < kGridFields; ++k) *g[k] = grid + size_t(k) * nn;
+    f->G.muvw = reinterpret_cast<float *>(take(nn * 16));
+    f->G.col = reinterpret_cast<float *>(take(nn * 16));
+    float **g[4] = {&f->G.gx, &f->G.gy, &f->G.ax, &f->G.ay};
+    for (float **q : g) *q = reinterpret_cast<float *>(take(nn * 4));
     f->mask = reinterpret_cast<int32_t *>(take(npx * 4));
     f->img = reinterpret_cast<uint32_t *>(take(npx * 4));
     // cells ranked in Morton (Z) order of (x, y): a run of consecutive ranks
