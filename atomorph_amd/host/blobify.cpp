@@ -43,7 +43,6 @@ struct Blob {
     size_t index = 0;    // place in the frame's blob list
     int32_t root = -1;   // union-find root of its pixels
     bool unified = false;
-    bool alive = true;
     uint32_t stamp = 0;  // == Frame::stamp: checked in the current pass
     std::set<int32_t> border;
 };
@@ -81,11 +80,8 @@ struct Frame {
         t.size += f.size;
         f.size = 0;
     }
-    void kill(int32_t id) {
-        Blob &bl = pool[id];
-        bl.alive = false;
-        std::set<int32_t>().swap(bl.border);
-    }
+    // a blob merged away: its border's memory back
+    void kill(int32_t id) { std::set<int32_t>().swap(pool[id].border); }
 };
 
 color mean_color(const Blob &b) { return am::create_color(b.r, b.g, b.b, b.a); }

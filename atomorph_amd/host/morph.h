@@ -13,10 +13,14 @@
  * are non-blocking; suspend() blocks; synchronize() returns false while
  * busy; get_pixels() renders the state of the last synchronize().
  * Differences: get_energy() in ATOM_MORPHING is the exact transport cost of
- * the chains (the reference's running value is not the cost); blob detection
- * implements the default blob_threshold 1.0 (blobs = 4-connected components
- * of present pixels) and ignores the other blob_* shaping options; frames
- * live on a canvas of max(resolution, pixel bounding box).
+ * the chains (the reference's running value is not the cost); with the
+ * default blob options the blobs are the 4-connected components of present
+ * pixels, found on the device, listed in label order with exact means (the
+ * reference's merge gives the same blobs in a shuffled order with running
+ * means); any other blob option runs the reference's greedy merge and dust
+ * unification on the host with its random draws; detection and unification
+ * take one worker step; frames live on a canvas of max(resolution, pixel
+ * bounding box).
  */
 #ifndef AM_MORPH_H
 #define AM_MORPH_H
