@@ -154,6 +154,56 @@ def time_to_cost(m, M, learned=None, log_every=20):
     return out
 
 
+def greedy_blobs_beside_reference(M, synth):
+    """Blob detection with non-default options (colour threshold 0.1, dust
+    below 4 px): the engine (device ingest, greedy merge and unification on
+    the host, device pixel grouping) and the compiled reference on the same
+    2 x 256^2 noisy frames; wall times and whether the blob lists (order,
+    running means, membership) are identical."""
+    from oracle import oracle_py as op
+    import numpy as np
+    frames = synth.noisy_ellipse_frames(np.random.default_rng(77), 2, 256, 256, nell=12)
+    opts = {"seed": 3, "blob_threshold": 0.1, "blob_min_size": 4}
+    m = M.Morph()
+    m.set(**opts)
+    for i, f in enumerate(frames):
+        m.add_frame_rgba(i, f)
+    m.set_resolution(256, 256)
+    m.run(0)
+    t0 = time.perf_counter()
+    m.run(1)
+    t_engine = time.perf_counter() - t0
+    got = [[m.blob(f, b) for b in range(m.blob_count(f))] for f in range(2)]
+    r = op.ref()
+    rm = r.ref_morph_new()
+    for k, v in dict(opts, threads=0).items():
+        r.ref_morph_set_option(rm, k.encode(), float(v))
+    for i, f in enumerate(frames):
+        a = np.ascontiguousarray(f)
+        r.ref_morph_add_rgba(rm, i, op.ptr(a), 256, 256)
+    r.ref_morph_set_resolution(rm, 256, 256)
+    r.ref_morph_run(rm, 0)
+    t0 = time.perf_counter()
+    r.ref_morph_run_until(rm, 2, 1 << 40)
+    t_ref = time.perf_counter() - t0
+    same = True
+    for f in range(2):
+        nb = r.ref_morph_blob_count(rm, f)
+        same &= nb == len(got[f])
+        for b in range(min(nb, len(got[f]))):
+            g = C.c_size_t()
+            mm = np.zeros(6)
+            cnt = r.ref_morph_blob(rm, f, b, C.byref(g), op.ptr(mm), None)
+            surf = np.zeros(cnt, np.uint64)
+            r.ref_morph_blob(rm, f, b, C.byref(g), None, op.ptr(surf))
+            same &= bool(np.array_equal(mm, got[f][b]["means"]) and
+                         np.array_equal(np.sort(got[f][b]["surface"]), surf))
+    r.ref_morph_free(rm)
+    return {"sample": "2 x 256^2 noisy ellipses, blob_threshold 0.1, blob_min_size 4, seed 3",
+            "blobs": [len(g) for g in got], "engine_s": round(t_engine, 4), "reference_s": round(t_ref, 3),
+            "identical_blobs": bool(same)}
+
+
 def cpu_baseline(frames, gpu_attempts, match_steps, fluid_steps):
     """The compiled reference on this host, one core (threads = 0), on a
     bounded sample of the same job: full blob detection, matching and chain
@@ -416,6 +466,9 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline: compiled reference, bounded sample")
         cpu = cpu_baseline(frames, attempts_job, args.match_steps, args.fluid_steps)
+        if cpu is not None:
+            log("greedy blob options beside the reference")
+            cpu["greedy_blobs"] = greedy_blobs_beside_reference(M, synth)
 
     res = {
         "metric": "time to equal OT cost, learned+batched vs plain GPU, 1024²; frames/s",

@@ -259,22 +259,7 @@ SIZE_MAX = (1 << 64) - 1
 
 
 def noisy_frames(rng, n, W, H, nell=5):
-    """n RGBA8 frames of filled ellipses with per-pixel colour noise,
-    quantised to steps of 16 so that equal colour distances (ties) occur."""
-    out = []
-    yy, xx = np.mgrid[0:H, 0:W]
-    for _ in range(n):
-        img = np.zeros((H, W, 4), np.uint8)
-        for _ in range(nell):
-            cx, cy = rng.uniform(0, W), rng.uniform(0, H)
-            rx, ry = rng.uniform(3, W / 3), rng.uniform(3, H / 3)
-            m = ((xx - cx) / rx) ** 2 + ((yy - cy) / ry) ** 2 <= 1
-            col = np.clip(rng.integers(0, 256, 3) + rng.integers(-30, 31, (H, W, 3)), 0, 255)
-            img[m, :3] = col[m]
-            img[m, 3] = 255
-        img[..., :3] = (img[..., :3] // 16) * 16
-        out.append(img)
-    return out
+    return synth.noisy_ellipse_frames(rng, n, W, H, nell)
 
 
 def blobify_fixture():
