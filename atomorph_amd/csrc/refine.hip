@@ -151,7 +151,7 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
                                                     int rounds, uint64_t seed, uint32_t *queue,
                                                     amh_refine_stats *st) {
     __shared__ uint64_t C[CAP];
-    __shared__ int32_t SX[CAP], SY[CAP];
+    __shared__ int2 S[CAP];  // neighbour sums (x, y), one 8-byte LDS load per slot
     __shared__ int item_s;
     __shared__ uint32_t pass_s;
     const int tid = threadIdx.x;
@@ -192,8 +192,7 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
             const uint32_t s = permuted ? feistel_perm(uint32_t(g0 + l), uint32_t(w), pkey) : uint32_t(l);
             const uint64_t p = rprev[s], q = rnext[s];
             C[l] = row[s];
-            SX[l] = pt_fx(p) + pt_fx(q);
-            SY[l] = pt_fy(p) + pt_fy(q);
+            S[l] = make_int2(pt_fx(p) + pt_fx(q), pt_fy(p) + pt_fy(q));
         }
         __syncthreads();
         if (n >= 2) {
@@ -204,7 +203,8 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
                     uint32_t i, j;
                     round_pair(rp, uint32_t(n), t, i, j);
                     const uint64_t ci = C[i], cj = C[j];
-                    const int64_t d = swap_test(ci, cj, SX[i], SY[i], SX[j], SY[j]);
+                    const int2 si = S[i], sj = S[j];
+                    const int64_t d = swap_test(ci, cj, si.x, si.y, sj.x, sj.y);
                     ++att;
                     if (d <= 0) {
                         C[i] = cj;
