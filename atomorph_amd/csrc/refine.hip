@@ -199,16 +199,31 @@ __global__ __launch_bounds__(NT) void k_refine_pass(uint64_t *__restrict__ pts, 
             const uint64_t sid = pass_stream(uint64_t(c), pass, uint64_t(g));
             for (int r = 0; r < rounds; ++r) {
                 const RoundPar rp = round_params(seed, sid, uint64_t(r), uint32_t(n));
-                for (uint32_t t = tid; t < rp.npairs; t += NT) {
-                    uint32_t i, j;
-                    round_pair(rp, uint32_t(n), t, i, j);
-                    const uint64_t ci = C[i], cj = C[j];
-                    const int2 si = S[i], sj = S[j];
-                    const int64_t d = swap_test(ci, cj, si.x, si.y, sj.x, sj.y);
+                // A thread's pairs of the round (at most kPer): the pairs are
+                // disjoint, so all their loads are issued before any swap.
+                constexpr int kPer = (CAP / 2 + NT - 1) / NT;
+                uint32_t pi[kPer], pj[kPer];
+                uint64_t ci[kPer], cj[kPer];
+                int2 si[kPer], sj[kPer];
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    const uint32_t t = uint32_t(tid + k * NT);
+                    if (t < rp.npairs) {
+                        round_pair(rp, uint32_t(n), t, pi[k], pj[k]);
+                        ci[k] = C[pi[k]];
+                        cj[k] = C[pj[k]];
+                        si[k] = S[pi[k]];
+                        sj[k] = S[pj[k]];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    if (uint32_t(tid + k * NT) >= rp.npairs) continue;
+                    const int64_t d = swap_test(ci[k], cj[k], si[k].x, si[k].y, sj[k].x, sj[k].y);
                     ++att;
                     if (d <= 0) {
-                        C[i] = cj;
-                        C[j] = ci;
+                        C[pi[k]] = cj[k];
+                        C[pj[k]] = ci[k];
                         ++acc;
                         gain += -2 * d;
                     }
