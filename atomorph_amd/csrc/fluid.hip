@@ -133,19 +133,19 @@ struct Seg {
     bool tail;        // last lane of its segment
 };
 
+// The neighbours' origins come by DPP wave shifts and the segment starts
+// from a ballot of the heads, so no LDS round trip is on this path.
 __device__ __forceinline__ Seg segments(bool valid, int cx, int cy) {
     Seg s;
     s.lane = threadIdx.x & 63;
-    const uint64_t vm = __ballot(valid);
-    const int pcx = __shfl_up(cx, 1, 64), pcy = __shfl_up(cy, 1, 64);
-    const int ncx = __shfl_down(cx, 1, 64), ncy = __shfl_down(cy, 1, 64);
-    const bool pv = s.lane > 0 && ((vm >> (s.lane - 1)) & 1ull);
-    const bool nv = s.lane < 63 && ((vm >> (s.lane + 1)) & 1ull);
-    const bool head = !valid || !pv || pcx != cx || pcy != cy;
-    s.tail = !valid || !nv || ncx != cx || ncy != cy;
-    int st = head ? s.lane : 0;
-    for (int off = 1; off < 64; off <<= 1) st = max(st, __shfl_up(st, off, 64));
-    s.start = st;
+    // origin key; an invalid lane matches no origin (nodes are < 65535)
+    const int key = valid ? int((uint32_t(cx) << 16) | (uint32_t(cy) & 0xFFFFu)) : -1;
+    const int prev = __builtin_amdgcn_update_dpp(-1, key, 0x138, 0xF, 0xF, false);  // wave_shr:1, lane - 1
+    const int next = __builtin_amdgcn_update_dpp(-1, key, 0x130, 0xF, 0xF, false);  // wave_shl:1, lane + 1
+    const bool head = !valid || s.lane == 0 || prev != key;
+    s.tail = !valid || s.lane == 63 || next != key;
+    const uint64_t heads = __ballot(head) & ((2ull << s.lane) - 1ull);  // heads at or below this lane
+    s.start = 63 - __clzll(static_cast<long long>(heads));               // lane 0 is always a head
     return s;
 }
 
