@@ -683,8 +683,9 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
                 const int ni = x0 + node / kTile, nj = y0 + node % kTile;
                 float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                 for (int b = 0; b < 3; ++b) {
+                    // an empty cell's sums are zero and add nothing; no branch on
+                    // it, so the three cells' LDS loads are in flight together
                     const int cell = (ni - a - x0 + 2) * kReach + (nj - b - y0 + 2);
-                    if (cs[0][cell] == 0.0f) continue;  // no particle (each adds 1 to M[0][0])
                     float m[kCellSums];
                     for (int k = 0; k < kCellSums; ++k) m[k] = cs[k][cell];
                     float v[8];
