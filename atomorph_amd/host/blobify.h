@@ -41,7 +41,9 @@ bool needed(const options &o);
 // stored[k]: the W x H RGBA words (red in the low byte) of key frame k as the
 // morph stores them (after the colour conversion); a pixel is present when
 // its alpha is non-zero.  seed: the morph's seed.  Frames are in key order.
-void segment(const std::vector<const uint32_t *> &stored, int W, int H, const options &o, unsigned seed,
+// Returns false (and leaves out empty) when W*H exceeds the 31-bit pixel
+// indices.
+bool segment(const std::vector<const uint32_t *> &stored, int W, int H, const options &o, unsigned seed,
              std::vector<frame_result> &out);
 
 }  // namespace am_blobify

@@ -489,7 +489,8 @@ bool morph::engine::detect() {
             ptrs[y] = gstored[y].data();
         }
         if (amh_stream_sync(cstream)) return fail("ingest sync");
-        am_blobify::segment(ptrs, CW, CH, bo, wopt.seed, gres);
+        if (!am_blobify::segment(ptrs, CW, CH, bo, wopt.seed, gres))
+            return fail("greedy blob detection: canvas too large for 31-bit pixel indices");
         for (int y = 0; y < h; ++y) {
             nblobs[y] = int32_t(gres[y].blobs.size());
             if (amh_memcpy_h2d(d_bop.as<int32_t>() + size_t(y) * npx, gres[y].bop.data(), npx * 4, cstream))
