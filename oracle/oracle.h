@@ -84,7 +84,10 @@ uint64_t or_refine_pass(uint64_t *pts, const int64_t *off, const int32_t *widths
  * or_refine_pass with partners proposed by the OpenCV build's nearest-
  * neighbour rule (thread.cpp:893-986) and overlapping proposals resolved by
  * the smallest claim.  Returns accepted swaps; exact gain; attempts = the
- * proposals that ran. */
+ * proposals that ran.  Parity unpinned for the choice of neighbours: the
+ * reference draws them from OpenCV's FLANN search, which is not available
+ * here; the acceptance rule, the exact gains, the row invariants and the
+ * two-slot reduction to one reference attempt are pinned by tests. */
 uint64_t or_refine_knn_pass(uint64_t *pts, const int64_t *off, const int32_t *widths, int32_t nchains,
                             int32_t h, int32_t cap, uint64_t pass, int32_t rounds, uint64_t seed,
                             int64_t *gain_out, uint64_t *attempts_out);
