@@ -171,6 +171,43 @@ def test_refine_knn_rejects_wide_tiles():
                                   G.dptr(q), G.dptr(st), G.stream()) != 0
 
 
+def test_refine_knn_full_size_properties():
+    """kNN-guided passes at the 512^2 workload's shape (40 chains, wider
+    than one LDS tile): rows stay permutations, the cost never rises, the
+    recorded gain is the exact cost drop."""
+    T = G.torch()
+    lib = G.lib()
+    rng = np.random.default_rng(31)
+    widths = np.sort(rng.integers(40, 20000, 40))[::-1]
+    h = 2
+    pts, off, wid = _chains(rng, widths, h, lim=512)
+    wtot = int(wid.sum())
+    tile = 3
+    items, nitems = _items(wid, tile)
+    dev = G.to_dev(G.key_major(pts, h))
+    off_d, w_d, it_d = G.to_dev(off), G.to_dev(wid.astype(np.int32)), G.to_dev(items)
+    queue = G.zeros(4, T.int32)
+    stats = G.zeros(4, T.int64)
+    c0 = sum(_gpu_cost(T, dev, h, wtot, off_d, w_d, len(wid)))
+    prev = c0
+    for p in range(24):
+        G.log(f"full-size knn pass {p}")
+        G.check(lib.amh_refine_knn(G.dptr(dev), h, wtot, G.dptr(off_d), G.dptr(w_d), G.dptr(it_d), nitems, tile,
+                                   p, 16, 5, G.dptr(queue), G.dptr(stats), G.stream()), "knn")
+        if p % 8 == 7:
+            cur = sum(_gpu_cost(T, dev, h, wtot, off_d, w_d, len(wid)))
+            assert cur <= prev
+            prev = cur
+    c1 = sum(_gpu_cost(T, dev, h, wtot, off_d, w_d, len(wid)))
+    st = G.to_host(stats, np.int64)
+    assert c0 - c1 == st[2] and st[2] > 0 and st[1] <= st[0]
+    got = G.atom_major(G.to_host(dev, np.uint64), h).reshape(-1, h)
+    src = pts.reshape(-1, h)
+    for o, w in zip(off, wid):
+        for j in range(h):
+            assert (np.sort(got[o:o + w, j]) == np.sort(src[o:o + w, j])).all()
+
+
 def test_refine_batched_full_size_properties():
     """The 512^2 two-key-frame workload's shape (40 chains, ~94k atoms) and a
     long run: rows stay permutations, the cost never rises, gain is exact."""
