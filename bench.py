@@ -257,7 +257,14 @@ def cpu_baseline(frames, gpu_attempts, match_steps, fluid_steps):
     r.ref_morph_free(m)
     total = t_detect + t_match_init + gpu_attempts / rate + FRAMES_OUT * t_frame
     total_thr = t_detect + t_match_init + gpu_attempts / rate_thr + FRAMES_OUT * t_frame
+    cpu_model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), None)
+    except OSError:
+        pass
     return {"value": round(FRAMES_OUT / total, 5), "unit": "frames/s", "cores": 1, "kind": "reference",
+            "host": {"cpu_model": cpu_model, "nproc": os.cpu_count()},
             "sample": (f"compiled reference on this host, threads=0: full blob detection ({t_detect:.1f} s), "
                        f"{match_steps} matching steps + chain init ({t_match_init:.1f} s), {n} chain visits of 2^16 "
                        f"attempts ({rate / 1e6:.2f} M attempts/s, scaled to the GPU job's {gpu_attempts} attempts), "
