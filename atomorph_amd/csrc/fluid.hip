@@ -1440,14 +1440,14 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, Frame F, Part
 // k_splat_tile gathers a tile's taps in LDS (FP64 sums: weight, weight *
 // channel) and resolves its pixels (morph.cpp:1169-1209) straight into the
 // image.
-__global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, double t, int W, int H, double sp, int bx1,
-                                                  int by1, int bx2, int by2, uint32_t *__restrict__ xy,
-                                                  double *__restrict__ taps, uint32_t *__restrict__ cols) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n) return;
+// Particle i's splat: its colour update, top-left tap pixel (xy[i], kNoTap
+// when nothing is drawn), four tap weights and packed colour.  Returns xy[i].
+__device__ __forceinline__ uint32_t splat_one(Particles P, int i, double t, int W, int H, double sp, int bx1, int by1,
+                                              int bx2, int by2, uint32_t *__restrict__ xy,
+                                              double *__restrict__ taps, uint32_t *__restrict__ cols) {
     if (!P.active[i]) {
         xy[i] = kNoTap;
-        return;
+        return kNoTap;
     }
     const double s = P.strength[i];
     const double w = (1.0 / (100.0 * pow((t - (1.0 * (1.0 - s))), 2.0) + 1.0)) * pow(s, 2.0);
@@ -1468,7 +1468,7 @@ __global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, double t,
     const double xf = modf(px, &ix), yf = modf(py, &iy);
     if ((ix >= W || iy >= H) && (ix > bx2 || ix < bx1 || iy > by2 || iy < by1)) {
         xy[i] = kNoTap;
-        return;
+        return kNoTap;
     }
     const double w11 = ((1.0 - xf) * (1.0 - yf)), w21 = (xf * (1.0 - yf));
     const double w12 = ((1.0 - xf) * yf), w22 = (xf * yf);
@@ -1479,10 +1479,25 @@ __global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, double t,
     if ((X < bx2 || X + 1 < W) && w21 > 0.0 && X + 1 < W) wt[1] = w21;
     if ((Y < by2 || Y + 1 < H) && w12 > 0.0 && Y + 1 < H) wt[2] = w12;
     if (w22 > 0.0 && ((Y < by2 && X < bx2) || (Y + 1 < H && X + 1 < W)) && X + 1 < W && Y + 1 < H) wt[3] = w22;
-    xy[i] = uint32_t(X) | (uint32_t(Y) << 16);
+    const uint32_t v = uint32_t(X) | (uint32_t(Y) << 16);
+    xy[i] = v;
     for (int q = 0; q < 4; ++q) taps[int64_t(i) * 4 + q] = wt[q];
     cols[i] = rgba_pack(narrow8(round(r * 255.0)), narrow8(round(g * 255.0)), narrow8(round(b * 255.0)),
                         narrow8(round(al * 255.0)));
+    return v;
+}
+
+// The splat of every particle, which also counts the pixel-tile bins of
+// its taps into B.count[par] (saving the binning's counting pass).
+__global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, double t, int W, int H, double sp, int bx1,
+                                                  int by1, int bx2, int by2, uint32_t *__restrict__ xy,
+                                                  double *__restrict__ taps, uint32_t *__restrict__ cols, Bins B,
+                                                  int par) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    const uint32_t v = i < n ? splat_one(P, i, t, W, H, sp, bx1, by1, bx2, by2, xy, taps, cols) : kNoTap;
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    const bool act = v != kNoTap && SplatTiles{xy, W, H}(i, tx0, tx1, ty0, ty1);
+    count_entries(act, tx0, tx1, ty0, ty1, i, n, B, par);  // every lane of the wave
 }
 
 struct SplatItem {
@@ -1748,13 +1763,13 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
         }
         {
             amh::ProfScope ps_("k_fluid_splat", s);
-            hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, F.tl, f->W, f->H,
-                               fa->sparseness, R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps,
-                               f->scol);
             const int ptiles = f->pbins.ntx * f->pbins.nty;
             const int par = f->pbin_par;
             f->pbin_par ^= 1;
-            bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s);
+            hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, F.tl, f->W, f->H,
+                               fa->sparseness, R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps,
+                               f->scol, f->pbins, par);
+            bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true);
             hipLaunchKernelGGL(k_splat_tile, dim3(uint32_t(ptiles)), dim3(kT), 0, s, f->pbins, f->sxy, f->staps,
                                f->scol, f->W, f->H, F.tl, R.keep_background, f->img, f->mask);
         }
