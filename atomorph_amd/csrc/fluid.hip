@@ -1115,8 +1115,12 @@ void carve_particles(Particles &Q, size_t n, const std::function<char *(size_t)>
 
 extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t count, int32_t width, int32_t height,
                                 int32_t nkeys, void *stream) {
-    if (!fluid || gsx < 8 || gsy < 8 || count < 0 || width <= 0 || height <= 0 || nkeys <= 0) {
-        set_error("amh_fluid_create: invalid argument");
+    // The splat packs a tap pixel as x | y << 16 with 0xFFFFFFFF as "no tap",
+    // and cell keys pack (cx << 16) | cy: both need coordinates below 65535.
+    if (!fluid || gsx < 8 || gsy < 8 || gsx >= 65535 || gsy >= 65535 || count < 0 || width <= 0 ||
+        height <= 0 || width > 65535 || height > 65535 || nkeys <= 0) {
+        set_error("amh_fluid_create: invalid argument (grid %dx%d, frame %dx%d, %d particles, %d keys)", gsx, gsy,
+                  width, height, count, nkeys);
         return AMH_E_ARG;
     }
     amh_fluid_s *f = new amh_fluid_s;

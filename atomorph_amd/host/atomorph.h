@@ -9,10 +9,20 @@
 #ifndef _ATOMORPH_H_
 #define _ATOMORPH_H_
 
+// The standard headers a caller of the reference receives through
+// atomorph.h (atomorph.h:9-17) and its spline.h -> vec3d.h (vec3d.h:15-16):
+// programs written against the reference (its demo/main.cpp uses std::cerr
+// and std::thread without including them) compile unchanged.
+#include <assert.h>
+#include <math.h>
+#include <stdlib.h>
+
 #include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <cstdlib>
+#include <future>
+#include <iostream>
 #include <limits>
 #include <map>
 #include <random>

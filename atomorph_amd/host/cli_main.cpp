@@ -65,126 +65,109 @@ struct Options {  // demo/options.h:27-62 (defaults kept)
 
 Options opt;
 
-void usage(FILE *s) {
-    std::fprintf(s, "Usage:   %s [options] <input files...>\n", opt.name.c_str());
-    std::fprintf(s, "Example: %s data/otu_1.png data/otu_2.png data/otu_3.png --verbose\n", opt.name.c_str());
-    std::fprintf(s,
-                 "Options:\n"
-                 "      --blend-blobs             Blob intersections are blended together.\n"
-                 "  -E  --blob-feather      INT   Number of gradually transparent outer layers.\n"
-                 "  -c  --blob-rgba-weight[0-255] Importance of a blob's color when matching.\n"
-                 "  -z  --blob-size-weight[0-255] Importance of a blob's size when matching.\n"
-                 "  -p  --blob-xy-weight  [0-255] Importance of a blob's location when matching.\n"
-                 "  -b  --blobs             INT   Preferred number of blobs to keep.\n"
-                 "      --blobs-as-texture        A blob is textured by its pixels (default).\n"
-                 "      --blobs-as-average        A blob has the average color of its pixels.\n"
-                 "      --blobs-as-distinct       A blob has a random color.\n"
-                 "      --blobs-by-rgb            Differentiate blobs by RGB distance.\n"
-                 "      --blobs-by-hsp            Differentiate blobs by HSP distance (default).\n"
-                 "  -B  --blobs-max-size    INT   Maximum size of a single blob in pixels.\n"
-                 "  -m  --blobs-min-size    INT   Minimum size of a single blob in pixels.\n"
-                 "  -g  --blobs-box-grip    INT   Unifying grip bounds for undersized blobs.\n"
-                 "  -S  --blobs-box-samples INT   Number of samples to take for each dust box.\n"
-                 "  -t  --blobs-threshold [0-255] Maximum color difference for merging blobs.\n"
-                 "      --brief                   Print brief messages (default).\n"
-                 "  -C  --cycle-length      INT   Number of iterations per morph cycle.\n"
-                 "  -d  --degenerate        INT   Degeneration period for energy minimzation.\n"
-                 "  -D  --density           INT   Number of atoms per pixel at minimum.\n"
-                 "      --fading-none             Colours are not interpolated.\n"
-                 "      --fading-linear           Colours are interpolated linearly.\n"
-                 "      --fading-cosine           Colours are cosine interpolated.\n"
-                 "      --fading-perlin           Use Perlin noise colour transition (default).\n"
-                 "  -f  --file              STR   Output files with prefix.\n"
-                 "      --finite                  Morph will not repeat itself seamlessly.\n"
-                 "  -L  --fluid             INT   Number of fluid simulation steps per frame.\n"
-                 "  -F  --frames            INT   Number of frames to generate.\n"
-                 "  -h  --help                    Display this usage information.\n"
-                 "  -i  --indir             STR   Read input from this directory.\n"
-                 "      --keep-background         Morph background is cross-dissolved.\n"
-                 "  -M  --match-time        INT   Time in seconds given for blob matching.\n"
-                 "  -O  --morph-time        INT   Time in seconds given for atom morphing.\n"
-                 "      --motion-none             Positions are not interpolated.\n"
-                 "      --motion-linear           Positions are linearly interpolated.\n"
-                 "      --motion-spline           Uses Catmull-Rom splines (default).\n"
-                 "  -o  --outdir            STR   Write output to this directory.\n"
-                 "  -s  --seed              INT   Seed for the random number generator.\n"
-                 "  -T  --threads           INT   Number of worker threads to spawn.\n"
-                 "      --verbose                 Print verbose messages.\n"
-                 "  -v  --version                 Show version information.\n"
-                 "GPU engine extensions:\n"
-                 "      --match-steps       INT   Matching steps instead of --match-time seconds.\n"
-                 "      --morph-steps       INT   Refinement steps instead of --morph-time seconds.\n"
-                 "      --batched                 Refine all chains per step (persistent, autotuned).\n"
-                 "      --tile              INT   Batched kernel variant (default: autotuned).\n"
-                 "      --rounds            INT   Batched rounds per pass (default: autotuned).\n"
-                 "      --knn                     Batched, with nearest-neighbour swap partners.\n"
-                 "      --learned           FILE  Learned warm start with these MLP parameters (.npz).\n"
-                 "      --fluid-sparseness  REAL  Fluid grid cells per pixel (default 1).\n"
-                 "      --load-chains       FILE  Resume the refinement from a chain checkpoint.\n"
-                 "      --save-chains       FILE  Write the refined chains to a checkpoint.\n");
-}
-
 enum { kMatchSteps = 1000, kMorphSteps, kBatched, kLearned, kSparseness, kTile, kRounds, kLoadChains, kSaveChains,
        kKnn };
 
+// One command-line flag: getopt_long's table and the help text are both
+// generated from this list.  Names, short letters and defaults are the
+// reference program's (demo/options.h:27-224), so its scripts run unchanged;
+// `group` 1 marks this engine's additions.
+struct Flag {
+    const char *name;
+    int has_arg;      // no_argument / required_argument
+    int *target;      // set to `val` when the flag has no argument
+    int val;          // short letter, extension code, or value for target
+    const char *arg;  // argument kind shown in the help ("" for none)
+    const char *help;
+    int group;
+};
+
+const Flag kFlags[] = {
+    {"blend-blobs", no_argument, &opt.blend_blobs, 1, "", "blend overlapping blobs instead of layering them", 0},
+    {"blob-feather", required_argument, nullptr, 'E', "N", "soften blob edges over N layers of fading alpha", 0},
+    {"blob-rgba-weight", required_argument, nullptr, 'c', "0..255", "weight of colour in blob matching", 0},
+    {"blob-size-weight", required_argument, nullptr, 'z', "0..255", "weight of area in blob matching", 0},
+    {"blob-xy-weight", required_argument, nullptr, 'p', "0..255", "weight of position in blob matching", 0},
+    {"blobs", required_argument, nullptr, 'b', "N", "number of blobs to aim for per key frame", 0},
+    {"blobs-as-texture", no_argument, &opt.show_blobs, am::TEXTURE, "", "draw blobs with their own pixels [default]",
+     0},
+    {"blobs-as-average", no_argument, &opt.show_blobs, am::AVERAGE, "", "draw each blob in its mean colour", 0},
+    {"blobs-as-distinct", no_argument, &opt.show_blobs, am::DISTINCT, "", "draw each blob in a random colour", 0},
+    {"blobs-by-rgb", no_argument, &opt.differ_blobs, am::RGB, "", "compare pixel colours in RGB space", 0},
+    {"blobs-by-hsp", no_argument, &opt.differ_blobs, am::HSP, "", "compare pixel colours in HSP space [default]", 0},
+    {"blobs-max-size", required_argument, nullptr, 'B', "N", "cap on the pixels of one blob", 0},
+    {"blobs-min-size", required_argument, nullptr, 'm', "N", "blobs below N pixels are merged into neighbours", 0},
+    {"blobs-box-grip", required_argument, nullptr, 'g', "N", "reach of the box that merges small blobs", 0},
+    {"blobs-box-samples", required_argument, nullptr, 'S', "N", "pixels sampled per merge box", 0},
+    {"blobs-threshold", required_argument, nullptr, 't', "0..255", "largest colour step inside one blob", 0},
+    {"brief", no_argument, &opt.verbose, 0, "", "short progress output [default]", 0},
+    {"cycle-length", required_argument, nullptr, 'C', "N", "swap attempts per refinement step", 0},
+    {"degenerate", required_argument, nullptr, 'd', "N", "accept a worse blob swap every N steps", 0},
+    {"density", required_argument, nullptr, 'D', "N", "atoms per pixel of the largest blob", 0},
+    {"fading-none", no_argument, &opt.fading, am::NONE, "", "keep the source colours", 0},
+    {"fading-linear", no_argument, &opt.fading, am::LINEAR, "", "blend colours linearly", 0},
+    {"fading-cosine", no_argument, &opt.fading, am::COSINE, "", "blend colours along a cosine ramp", 0},
+    {"fading-perlin", no_argument, &opt.fading, am::PERLIN, "", "blend colours with Perlin-noise timing [default]",
+     0},
+    {"file", required_argument, nullptr, 'f', "NAME", "prefix of the written frames", 0},
+    {"finite", no_argument, &opt.finite, 1, "", "do not loop back to the first key frame", 0},
+    {"fluid", required_argument, nullptr, 'L', "N", "fluid steps per frame (0: no fluid)", 0},
+    {"frames", required_argument, nullptr, 'F', "N", "frames to write", 0},
+    {"help", no_argument, nullptr, 'h', "", "print this list and exit", 0},
+    {"indir", required_argument, nullptr, 'i', "DIR", "where the key frames are read", 0},
+    {"keep-background", no_argument, &opt.keep_background, 1, "", "cross-fade the whole key frames behind the blobs",
+     0},
+    {"match-time", required_argument, nullptr, 'M', "SEC", "seconds of blob matching", 0},
+    {"morph-time", required_argument, nullptr, 'O', "SEC", "seconds of atom refinement", 0},
+    {"motion-none", no_argument, &opt.motion, am::NONE, "", "atoms stay at their source", 0},
+    {"motion-linear", no_argument, &opt.motion, am::LINEAR, "", "atoms move on straight lines", 0},
+    {"motion-spline", no_argument, &opt.motion, am::SPLINE, "", "atoms move on Catmull-Rom curves [default]", 0},
+    {"outdir", required_argument, nullptr, 'o', "DIR", "where the frames are written", 0},
+    {"seed", required_argument, nullptr, 's', "N", "random seed", 0},
+    {"threads", required_argument, nullptr, 'T', "N", "refinement threads (kept for compatibility)", 0},
+    {"verbose", no_argument, &opt.verbose, 1, "", "detailed progress output", 0},
+    {"version", no_argument, nullptr, 'v', "", "print the version and exit", 0},
+    {"match-steps", required_argument, nullptr, kMatchSteps, "N", "N matching steps instead of --match-time", 1},
+    {"morph-steps", required_argument, nullptr, kMorphSteps, "N", "N refinement steps instead of --morph-time", 1},
+    {"batched", no_argument, nullptr, kBatched, "", "refine every chain in each step (one persistent launch)", 1},
+    {"tile", required_argument, nullptr, kTile, "N", "batched kernel variant (default: autotuned)", 1},
+    {"rounds", required_argument, nullptr, kRounds, "N", "batched rounds per pass (default: autotuned)", 1},
+    {"knn", no_argument, nullptr, kKnn, "", "batched, with nearest-neighbour swap partners", 1},
+    {"learned", required_argument, nullptr, kLearned, "FILE", "warm-start the chains with this MLP (.npz)", 1},
+    {"fluid-sparseness", required_argument, nullptr, kSparseness, "X", "fluid grid cells per pixel (default 1)", 1},
+    {"load-chains", required_argument, nullptr, kLoadChains, "FILE", "resume refinement from a checkpoint", 1},
+    {"save-chains", required_argument, nullptr, kSaveChains, "FILE", "write a checkpoint of the refined chains", 1},
+};
+
+void usage(FILE *s) {
+    std::fprintf(s, "usage: %s [options] KEYFRAME.png KEYFRAME.png ...\n", opt.name.c_str());
+    for (int group = 0; group < 2; ++group) {
+        std::fprintf(s, group == 0 ? "options:\n" : "GPU engine options:\n");
+        for (const Flag &f : kFlags) {
+            if (f.group != group) continue;
+            char lhs[64], letter[8] = "   ";
+            if (f.target == nullptr && f.val > 0 && f.val < 128) std::snprintf(letter, sizeof(letter), "-%c,", f.val);
+            std::snprintf(lhs, sizeof(lhs), "  %s --%s %s", letter, f.name, f.arg);
+            std::fprintf(s, "%-34s %s\n", lhs, f.help);
+        }
+    }
+}
+
 bool parse(int argc, char **argv) {  // demo/options.h:126-224
     opt.name = argv[0];
-    static struct option longs[] = {
-        {"verbose", no_argument, &opt.verbose, 1},
-        {"blend-blobs", no_argument, &opt.blend_blobs, 1},
-        {"finite", no_argument, &opt.finite, 1},
-        {"blobs-as-texture", no_argument, &opt.show_blobs, am::TEXTURE},
-        {"blobs-as-average", no_argument, &opt.show_blobs, am::AVERAGE},
-        {"blobs-as-distinct", no_argument, &opt.show_blobs, am::DISTINCT},
-        {"blobs-by-rgb", no_argument, &opt.differ_blobs, am::RGB},
-        {"blobs-by-hsp", no_argument, &opt.differ_blobs, am::HSP},
-        {"brief", no_argument, &opt.verbose, 0},
-        {"keep-background", no_argument, &opt.keep_background, 1},
-        {"blob-feather", required_argument, nullptr, 'E'},
-        {"blob-rgba-weight", required_argument, nullptr, 'c'},
-        {"blob-size-weight", required_argument, nullptr, 'z'},
-        {"blob-xy-weight", required_argument, nullptr, 'p'},
-        {"blobs", required_argument, nullptr, 'b'},
-        {"blobs-max-size", required_argument, nullptr, 'B'},
-        {"blobs-min-size", required_argument, nullptr, 'm'},
-        {"blobs-box-grip", required_argument, nullptr, 'g'},
-        {"blobs-box-samples", required_argument, nullptr, 'S'},
-        {"blobs-threshold", required_argument, nullptr, 't'},
-        {"cycle-length", required_argument, nullptr, 'C'},
-        {"degenerate", required_argument, nullptr, 'd'},
-        {"density", required_argument, nullptr, 'D'},
-        {"fading-none", no_argument, &opt.fading, am::NONE},
-        {"fading-linear", no_argument, &opt.fading, am::LINEAR},
-        {"fading-cosine", no_argument, &opt.fading, am::COSINE},
-        {"fading-perlin", no_argument, &opt.fading, am::PERLIN},
-        {"file", required_argument, nullptr, 'f'},
-        {"fluid", required_argument, nullptr, 'L'},
-        {"frames", required_argument, nullptr, 'F'},
-        {"help", no_argument, nullptr, 'h'},
-        {"indir", required_argument, nullptr, 'i'},
-        {"match-time", required_argument, nullptr, 'M'},
-        {"morph-time", required_argument, nullptr, 'O'},
-        {"motion-none", no_argument, &opt.motion, am::NONE},
-        {"motion-linear", no_argument, &opt.motion, am::LINEAR},
-        {"motion-spline", no_argument, &opt.motion, am::SPLINE},
-        {"outdir", required_argument, nullptr, 'o'},
-        {"seed", required_argument, nullptr, 's'},
-        {"threads", required_argument, nullptr, 'T'},
-        {"version", no_argument, nullptr, 'v'},
-        {"match-steps", required_argument, nullptr, kMatchSteps},
-        {"morph-steps", required_argument, nullptr, kMorphSteps},
-        {"batched", no_argument, nullptr, kBatched},
-        {"knn", no_argument, nullptr, kKnn},
-        {"learned", required_argument, nullptr, kLearned},
-        {"fluid-sparseness", required_argument, nullptr, kSparseness},
-        {"tile", required_argument, nullptr, kTile},
-        {"rounds", required_argument, nullptr, kRounds},
-        {"load-chains", required_argument, nullptr, kLoadChains},
-        {"save-chains", required_argument, nullptr, kSaveChains},
-        {nullptr, 0, nullptr, 0}};
+    std::vector<struct option> longs;
+    std::string shorts;
+    for (const Flag &f : kFlags) {
+        longs.push_back({f.name, f.has_arg, f.target, f.val});
+        if (f.target == nullptr && f.val > 0 && f.val < 128) {
+            shorts += char(f.val);
+            if (f.has_arg == required_argument) shorts += ':';
+        }
+    }
+    longs.push_back({nullptr, 0, nullptr, 0});
     for (;;) {
         int index = 0;
-        const int c = getopt_long(argc, argv, "b:B:c:C:d:D:E:m:M:O:g:p:S:t:T:f:L:F:hi:o:s:vz:", longs, &index);
+        const int c = getopt_long(argc, argv, shorts.c_str(), longs.data(), &index);
         if (c == -1) break;
         switch (c) {
             case 0: break;

@@ -290,7 +290,8 @@ int amh_render_frame(const amh_render *r, double t, void *ws_dev, size_t ws_byte
  * sizes it by the chains' max_surface, which equals the atoms at density 1)
  * on a gsx x gsy node grid (the reference uses width*s+20 x height*s+20 with
  * s = 1), plus width x height render buffers.  State persists from frame to
- * frame: rendering advances the simulation, as in the reference. */
+ * frame: rendering advances the simulation, as in the reference.  Sizes:
+ * 8 <= gsx, gsy < 65535 and 1 <= width, height <= 65535 (AMH_E_ARG beyond). */
 int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t count, int32_t width, int32_t height,
                      int32_t nkeys, void *stream);
 int amh_fluid_destroy(void *fluid);

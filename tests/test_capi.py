@@ -57,6 +57,19 @@ def test_error_path_without_device_work():
     assert rc == -1
 
 
+@pytest.mark.parametrize("gsx,gsy,w,h", [(65535, 64, 64, 64), (64, 70000, 64, 64), (64, 64, 65536, 8),
+                                         (64, 64, 8, 100000), (7, 64, 64, 64)])
+def test_fluid_create_rejects_sizes_the_packing_cannot_hold(gsx, gsy, w, h):
+    """Tap pixels pack x | y << 16 (0xFFFFFFFF = no tap) and cell keys
+    (cx << 16) | cy, so sizes of 65535 and more are refused before any
+    device allocation."""
+    lib = A.load()
+    f = C.c_void_p()
+    rc = lib.amh_fluid_create(C.byref(f), gsx, gsy, 100, w, h, 2, None)
+    assert rc == -1 and b"invalid" in lib.amh_last_error()
+    assert f.value is None
+
+
 def test_host_colour_matches_oracle():
     lib = A.load()
     rng = np.random.default_rng(0)
