@@ -342,11 +342,10 @@ bool needed(const options &o) {
     return o.threshold < 1.0 || o.max_size != SIZE_MAX || o.min_size > 1 || o.number > 1;
 }
 
-bool segment(const std::vector<const uint32_t *> &stored, int W, int H, const options &o, unsigned seed,
-             std::vector<frame_result> &out) {
+bool segment(const std::vector<const uint32_t *> &stored, int W, int H, const options &o,
+             std::default_random_engine &e1, std::vector<frame_result> &out) {
     out.clear();
     if (W <= 0 || H <= 0 || int64_t(W) * int64_t(H) > int64_t(INT32_MAX)) return false;
-    std::default_random_engine e1(seed);
     std::vector<Frame> F(stored.size());
     for (size_t k = 0; k < F.size(); ++k) {
         F[k].W = W;
@@ -401,7 +400,8 @@ extern "C" int am_blobify_frames(const uint32_t *stored, int32_t nframes, int32_
     std::vector<const uint32_t *> frames(nframes);
     for (int32_t k = 0; k < nframes; ++k) frames[k] = stored + size_t(k) * npx;
     std::vector<am_blobify::frame_result> res;
-    if (!am_blobify::segment(frames, W, H, o, seed, res)) return -1;
+    std::default_random_engine e1(seed);
+    if (!am_blobify::segment(frames, W, H, o, e1, res)) return -1;
     for (int32_t k = 0; k < nframes; ++k) {
         const auto &r = res[k];
         std::copy(r.bop.begin(), r.bop.end(), bop + size_t(k) * npx);

@@ -11,6 +11,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <random>
 #include <vector>
 
 namespace am_blobify {
@@ -40,11 +41,13 @@ bool needed(const options &o);
 
 // stored[k]: the W x H RGBA words (red in the low byte) of key frame k as the
 // morph stores them (after the colour conversion); a pixel is present when
-// its alpha is non-zero.  seed: the morph's seed.  Frames are in key order.
-// Returns false (and leaves out empty) when W*H exceeds the 31-bit pixel
-// indices.
-bool segment(const std::vector<const uint32_t *> &stored, int W, int H, const options &o, unsigned seed,
-             std::vector<frame_result> &out);
+// its alpha is non-zero.  e1: the worker's random engine, seeded with the
+// morph's seed (thread::set_seed, thread.cpp:86-90); detection draws from it
+// and leaves it where the reference's blob matching continues.  Frames are
+// in key order.  Returns false (and leaves out empty) when W*H exceeds the
+// 31-bit pixel indices.
+bool segment(const std::vector<const uint32_t *> &stored, int W, int H, const options &o,
+             std::default_random_engine &e1, std::vector<frame_result> &out);
 
 }  // namespace am_blobify
 

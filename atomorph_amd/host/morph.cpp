@@ -151,6 +151,9 @@ struct BlobInfo {  // one blob of a key frame after detection or padding
     size_t size = 0;
     double x = 0, y = 0, r = 0, g = 0, b = 0, a = 0;
     int first = -1;    // smallest pixel index (canvas)
+    size_t group0 = 0; // group before matching: y*65536 + x of the pixel the
+                       // greedy merge grew the blob from (thread.cpp:243),
+                       // of its first pixel for device-detected blobs
     pixel px{};        // the blob as one pixel (blob_distance), set by match_init
 };
 
@@ -184,7 +187,10 @@ struct morph::engine {
     unsigned wstate = STATE_BLOB_DETECTION;
     bool wskip = false;
     size_t counter = 0;
-    std::mt19937_64 rng{0};
+    // The worker's random engine, as the reference's (thread.h:120): greedy
+    // blob detection and blob matching draw from it in the reference's
+    // order, so matching continues where detection left it.
+    std::default_random_engine rng;
     void *cstream = nullptr, *rstream = nullptr;
     // canvas and key frames
     int32_t CW = 0, CH = 0, h = 0;
@@ -199,6 +205,7 @@ struct morph::engine {
     std::vector<int32_t> bmap, best_map;          // [c*h + y] -> index in fb[y]
     size_t bw = 0;                                // blob map width
     double map_e = 0, best_e = std::numeric_limits<double>::infinity();
+    double best_report = 0;                       // get_energy() while matching (best_blob_map_e)
     bool matched = false;
     size_t match_steps = 0;
     uint32_t bbox_d = 0;
@@ -327,6 +334,7 @@ void morph::engine::reset_worker() {
     bmap.clear();
     best_map.clear();
     best_e = std::numeric_limits<double>::infinity();
+    best_report = 0;
     map_e = 0;
     wtot = 0;
     off.clear();
@@ -489,7 +497,7 @@ bool morph::engine::detect() {
             ptrs[y] = gstored[y].data();
         }
         if (amh_stream_sync(cstream)) return fail("ingest sync");
-        if (!am_blobify::segment(ptrs, CW, CH, bo, wopt.seed, gres))
+        if (!am_blobify::segment(ptrs, CW, CH, bo, rng, gres))
             return fail("greedy blob detection: canvas too large for 31-bit pixel indices");
         for (int y = 0; y < h; ++y) {
             nblobs[y] = int32_t(gres[y].blobs.size());
@@ -560,6 +568,7 @@ bool morph::engine::detect() {
                 bi.x = g.x; bi.y = g.y;
                 bi.r = g.r; bi.g = g.g; bi.b = g.b; bi.a = g.a;
                 bi.first = first[b];
+                bi.group0 = size_t(g.group);
                 fb[y].push_back(bi);
             }
         }
@@ -588,6 +597,7 @@ bool morph::engine::detect() {
             bi.b = double(s.sum_c[2]) / (255.0 * s.count);
             bi.a = double(s.sum_c[3]) / (255.0 * s.count);
             bi.first = int(s.first);
+            bi.group0 = xy2pos(uint16_t(s.first % CW), uint16_t(s.first / CW));
             fb[y].push_back(bi);
         }
     }
@@ -652,8 +662,11 @@ bool morph::engine::match_init() {
         w_size = double(wopt.blob_size_weight * wopt.blob_size_weight) / sum;
         w_xy = 1.0 - (w_rgba + w_size);
     }
+    // best_e stays infinite until the first accepted swap (thread.cpp:51),
+    // so that swap always becomes the best map, even at zero gain.
     map_e = bw > 0 && h > 0 ? map_energy(bmap) : 0.0;
-    best_e = map_e;
+    best_e = std::numeric_limits<double>::infinity();
+    best_report = 0.0;
     matched = true;
     return false;
 }
@@ -673,16 +686,18 @@ bool morph::engine::match_step() {
     const BlobInfo &a_c = fb[y][bmap[x1 * h + y]], &b_c = fb[y][bmap[x2 * h + y]];
     const BlobInfo &a_n = fb[yn][bmap[x1 * h + yn]], &b_n = fb[yn][bmap[x2 * h + yn]];
     if (a_c.empty && b_c.empty) return wopt.match_budget && match_steps >= wopt.match_budget;
-    const double c1 = blob_distance(a_p, a_c) + blob_distance(a_c, a_n) + blob_distance(b_p, b_c) +
-                      blob_distance(b_c, b_n);
-    const double c2 = blob_distance(b_p, a_c) + blob_distance(a_c, b_n) + blob_distance(a_p, b_c) +
-                      blob_distance(b_c, a_n);
+    // summed per blob first, as thread.cpp:690-696 does
+    const double c1 = (blob_distance(a_p, a_c) + blob_distance(a_c, a_n)) +
+                      (blob_distance(b_p, b_c) + blob_distance(b_c, b_n));
+    const double c2 = (blob_distance(b_p, a_c) + blob_distance(a_c, b_n)) +
+                      (blob_distance(a_p, b_c) + blob_distance(b_c, a_n));
     if (c1 >= c2 || (wopt.degeneration && (counter % wopt.degeneration) == 0)) {
         std::swap(bmap[x1 * h + y], bmap[x2 * h + y]);
         map_e -= c1 - c2;
         if (map_e < 0.0) map_e = 0.0;
         if (map_e < best_e) {  // the groups follow the best map seen
             best_e = map_e;
+            best_report = best_e;
             best_map = bmap;
             counter = 0;
         }
@@ -1466,7 +1481,7 @@ bool morph::synchronize() {  // morph.cpp:102-269
     }
     e.state = e.wstate;
     e.energy = -1.0;
-    if (e.wstate == STATE_BLOB_MATCHING && e.matched) e.energy = e.best_e;
+    if (e.wstate == STATE_BLOB_MATCHING) e.energy = e.best_report;  // best_blob_map_e (thread.cpp:1178)
     if (e.wstate >= STATE_ATOM_MORPHING && e.wtot > 0) {
         if (!e.snapshot()) return false;
         uint64_t lo, hi;
@@ -1519,7 +1534,13 @@ const blob *morph::get_blob(size_t f, size_t b) {
             bl->index = i;
             bl->x = bi.x; bl->y = bi.y; bl->r = bi.r; bl->g = bi.g; bl->b = bi.b; bl->a = bi.a;
             bl->unified = true;
-            bl->group = i;
+            // the reference's blob->group: the start pixel before matching,
+            // the column of the best blob map while matching (thread.cpp:
+            // 655, 713-730), the chain afterwards
+            bl->group = bi.group0;
+            if (e.matched && e.best_map.size() == e.bw * size_t(e.h))
+                for (size_t c = 0; c < e.bw; ++c)
+                    if (e.best_map[c * e.h + y] == int32_t(i)) bl->group = c;
             for (size_t c = 0; c < nch; ++c)
                 if (e.rank.size() == size_t(e.h) * nch && e.rank[y * nch + c] == int32_t(i)) bl->group = c;
             if (!bi.empty && bi.id >= 0) bl->surface.insert(surf[bi.id].begin(), surf[bi.id].end());

@@ -29,6 +29,11 @@
  *                                       + brute-force flood fill
  *   or_render_atoms                   - vs reference rendered-frame fixtures
  *   or_fluid_step                     - vs compiled reference FluidModel
+ *   or_match                          - vs the compiled reference's blob
+ *                                       groups, list order, energies and
+ *                                       fix_volatiles positions (match.npz)
+ *   or_match_brute                    - exhaustive enumeration (the check
+ *                                       of the descent's optimum)
  *   or_lap_min_cost                   - vs scipy linear_sum_assignment and
  *                                       brute force permutations (n <= 8)
  *   or_mlp_forward, or_mlp_grad       - central finite differences of the
@@ -172,6 +177,35 @@ double or_get_time(size_t frame_index, size_t total_frames, int finite,
 void or_fluid_step(double *particles, int32_t count, int32_t gsx, int32_t gsy,
                    int32_t steps_left, double freedom_radius,
                    unsigned jitter_seed);
+
+/* ---- blob matching (thread.cpp:598-738, 1087-1107, 1141-1174, 1187-1233) */
+/* thread::match for `steps` worker steps in BLOB_MATCHING (the first pads
+ * every frame with blobs at the frame means and shuffles it, the others try
+ * one random swap each), driven like thread::step drives it (its counter
+ * sets the degeneration period), from std::default_random_engine state
+ * rng_state.  Input per key frame y: nblobs[y] blobs in list order with
+ * sizes[] and means[][6] = x, y, r, g, b, a (concatenated over frames);
+ * fmeans[y][6] the frame's running means (morph::add_pixel); bbox x1 y1 x2
+ * y2; weights rgba, size, xy (0..255).  Output, bw = max nblobs:
+ * order[y][bw] = caller's index of each listed blob (-1: padding blob),
+ * group[y][bw] = its group (the column it held at the last improvement),
+ * energy = {current map energy, best energy (infinity before any accepted
+ * swap), what get_energy() reports while matching}; with fix != 0,
+ * fixed[y][bw][2] = each blob's x, y after init_morph's fix_volatiles;
+ * *rng_out = the engine state afterwards.  Returns bw (-1: bad input). */
+int32_t or_match(int32_t h, const int32_t *nblobs, const int64_t *sizes, const double *means,
+                 const double *fmeans, const int32_t *bbox, const uint8_t *weights, uint64_t degeneration,
+                 uint64_t rng_state, uint64_t steps, int fix, int32_t *order, int32_t *group, double *energy,
+                 double *fixed, uint64_t *rng_out);
+/* get_energy(blob***) of padded lists sizes[h][bw], means[h][bw][6] under
+ * group[h][bw] (the column of each list entry). */
+double or_match_energy(int32_t h, int32_t bw, const int64_t *sizes, const double *means, const int32_t *bbox,
+                       const uint8_t *weights, const int32_t *group);
+/* Exhaustive minimum of that energy over every assignment of key frames
+ * 1..h-1 to the columns (h <= 3, bw <= 8); best_group receives one
+ * minimiser.  Returns the minimum (-1: sizes out of range). */
+double or_match_brute(int32_t h, int32_t bw, const int64_t *sizes, const double *means, const int32_t *bbox,
+                      const uint8_t *weights, int32_t *best_group);
 
 /* ---- exact assignment (for the OT lower bound) -------------------------- */
 /* Minimum-cost perfect matching of an n x n int64 cost matrix (row-major),

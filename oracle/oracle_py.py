@@ -49,6 +49,9 @@ _ORACLE = {
     "or_distinct_color": (u32, [sz]),
     "or_fluid_step": (None, [vp, i32, i32, i32, i32, f64, C.c_uint]),
     "or_lap_min_cost": (i64, [vp, i32, vp]),
+    "or_match": (i32, [i32, vp, vp, vp, vp, vp, vp, u64, u64, u64, C.c_int, vp, vp, vp, vp, C.POINTER(u64)]),
+    "or_match_energy": (f64, [i32, i32, vp, vp, vp, vp, vp]),
+    "or_match_brute": (f64, [i32, i32, vp, vp, vp, vp, vp]),
     "or_mlp_forward": (None, [vp, vp, i32, C.c_int, vp]),
     "or_mlp_grad": (f64, [vp, vp, vp, i32, C.c_int, vp]),
 }
@@ -85,6 +88,9 @@ _REF = {
     "ref_morph_render": (None, [vp, f64, vp]),
     "ref_morph_bbox": (None, [vp, vp]),
     "ref_morph_get_pixel": (u32, [vp, sz, C.c_uint, C.c_uint]),
+    "ref_morph_rng_state": (u64, [vp]),
+    "ref_morph_frame_means": (None, [vp, sz, vp]),
+    "ref_morph_match_energy": (None, [vp, vp]),
 }
 
 

@@ -162,6 +162,9 @@ void ref_morph_set_option(void *mp, const char *name, double v) {
     else if (n == "blob_box_grip") m->set_blob_box_grip(uint16_t(v));
     else if (n == "blob_box_samples") m->set_blob_box_samples(size_t(v));
     else if (n == "blob_number") m->set_blob_number(size_t(v));
+    else if (n == "blob_rgba_weight") m->set_blob_rgba_weight((unsigned char)v);
+    else if (n == "blob_size_weight") m->set_blob_size_weight((unsigned char)v);
+    else if (n == "blob_xy_weight") m->set_blob_xy_weight((unsigned char)v);
 }
 
 /* demo/main.cpp:96-127 (fill_morph): skips alpha == 0 pixels. */
@@ -224,6 +227,30 @@ size_t ref_morph_blob(void *mp, size_t f, size_t b, size_t *group, double *means
         for (size_t p : bl->surface) surf[i++] = p;
     }
     return bl->surface.size();
+}
+/* The worker's random engine (std::default_random_engine, one state word),
+ * e.g. where blob matching starts drawing from it. */
+uint64_t ref_morph_rng_state(void *mp) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    std::ostringstream os;
+    os << m->worker.e1;
+    return std::stoull(os.str());
+}
+/* Frame f's running means x, y, r, g, b, a as morph::add_pixel keeps them
+ * (morph.cpp:317-334). */
+void ref_morph_frame_means(void *mp, size_t f, double *m6) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    const am::frame &fr = m->frames[f];
+    m6[0] = fr.x; m6[1] = fr.y; m6[2] = fr.r; m6[3] = fr.g; m6[4] = fr.b; m6[5] = fr.a;
+}
+/* Blob matching energies of the worker: the current map's running value,
+ * the best seen (infinity before the first accepted swap) and the value
+ * get_energy() reports while matching. */
+void ref_morph_match_energy(void *mp, double *e3) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    e3[0] = m->worker.blob_map_e;
+    e3[1] = m->worker.best_e;
+    e3[2] = m->worker.best_blob_map_e;
 }
 size_t ref_morph_chain_count(void *mp) { return static_cast<am::morph *>(mp)->chains.size(); }
 /* Chain with key `key`: returns width, writes height and max_surface; if pts

@@ -29,3 +29,14 @@ def golden_cfg1():
 def orc():
     from oracle import oracle_py
     return oracle_py
+
+
+@pytest.fixture(scope="session")
+def golden_density():
+    """density.npz split by density: golden_density(2) looks like cfg1.npz."""
+    d = dict(np.load(os.path.join(GOLDEN, "density.npz")))
+
+    def part(density):
+        pre = f"d{density}_"
+        return {k[len(pre):]: v for k, v in d.items() if k.startswith(pre)}
+    return part

@@ -20,7 +20,14 @@ Fixtures
                  the reference reaches BLOB_MATCHING (order, running means,
                  starting pixel, membership).
 
-    python tests/golden/make_golden.py [functions] [cfg1] [blobify]
+  match.npz      blob matching with greedy blob options: the blob lists and
+                 random engine state where matching starts, list order,
+                 groups and energies after 1 .. 3000 steps, fix_volatiles
+                 positions and the chains init_morph builds.
+  density.npz    the cfg1 pipeline at density 2 and 3 (the reference CLI's
+                 default is 2): chains and rendered frames.
+
+    python tests/golden/make_golden.py [functions] [cfg1] [blobify] [match] [density]
 """
 import ctypes as C
 import os
@@ -112,11 +119,23 @@ def functions_fixture():
     return d
 
 
-def run_cfg1():
+CFG1_VARIANTS = [
+    ("default", {}),
+    ("linear_cosine", {"motion": 3, "fading": 5}),
+    ("none_linear_bg", {"motion": 2, "fading": 3, "keep_background": 1}),
+    ("spline_perlin_bg_feather2", {"keep_background": 1, "feather": 2}),
+    ("average_blobs", {"show_blobs": 1}),
+    ("distinct_blobs", {"show_blobs": 2, "finite": 1}),
+]
+
+
+def run_cfg1(density=1, variants=None):
     r = op.ref()
     frames = synth.workload("ellipses64")
     m = r.ref_morph_new()
     opts = {"seed": 1, "threads": 0, "cycle_length": 1 << 16}
+    if density != 1:
+        opts["density"] = density
     for k, v in opts.items():
         r.ref_morph_set_option(m, k.encode(), float(v))
     for f, img in enumerate(frames):
@@ -203,14 +222,7 @@ def run_cfg1():
     assert state == 4, STATE.get(state)
 
     # rendered frames from the refined chains, several option sets
-    variants = [
-        ("default", {}),
-        ("linear_cosine", {"motion": 3, "fading": 5}),
-        ("none_linear_bg", {"motion": 2, "fading": 3, "keep_background": 1}),
-        ("spline_perlin_bg_feather2", {"keep_background": 1, "feather": 2}),
-        ("average_blobs", {"show_blobs": 1}),
-        ("distinct_blobs", {"show_blobs": 2, "finite": 1}),
-    ]
+    variants = CFG1_VARIANTS if variants is None else variants
     F = 16
     renders = {}
     base = {"motion": 4, "fading": 6, "keep_background": 0, "feather": 0, "show_blobs": 0, "finite": 0}
@@ -323,10 +335,136 @@ def blobify_fixture():
     return d
 
 
+# Blob matching cases (greedy detection, so the blob lists and the random
+# engine at the start of matching are the ones the engine also has):
+# threshold, min size, blob number, delimiter, seed, key frames, key frame
+# left empty (-1: none), degeneration, (rgba, size, xy) weights, frame size.
+MATCH_CASES = [
+    (0.1, 1, 1, 1, 11, 3, -1, 0, (1, 1, 1), (32, 28)),
+    (0.05, 8, 1, 1, 12, 2, -1, 0, (1, 1, 1), (32, 28)),
+    (0.2, 1, 1, 0, 13, 4, -1, 7, (1, 1, 1), (32, 28)),
+    (0.15, 1, 20, 1, 14, 3, 1, 0, (3, 0, 2), (32, 28)),
+    (0.1, 6, 1, 1, 15, 3, -1, 3, (0, 1, 1), (40, 24)),
+    (0.3, 1, 1, 1, 16, 2, -1, 0, (1, 1, 1), (24, 24)),
+]
+MATCH_STEPS = [1, 2, 5, 40, 300, 3000]
+
+
+def _blob_list(r, m, f):
+    """(sizes, first pixel index y*W+x or -1, means[6], group) of frame f's
+    blob list in list order."""
+    W = None
+    out = []
+    for b in range(r.ref_morph_blob_count(m, f)):
+        g = C.c_size_t()
+        mm = np.zeros(6)
+        cnt = r.ref_morph_blob(m, f, b, C.byref(g), op.ptr(mm), None)
+        first = -1
+        if cnt:
+            surf = np.zeros(cnt, np.uint64)
+            r.ref_morph_blob(m, f, b, C.byref(g), None, op.ptr(surf))
+            first = int(surf.min())
+        out.append((cnt, first, mm, g.value))
+    return out
+
+
+def match_fixture():
+    """Per case: the blob lists, frame means and random engine state where
+    matching starts; after each cumulative step count of MATCH_STEPS the
+    list order (index into the starting list, -1 for a padding blob), each
+    blob's group and the matching energies; after init_morph the blob
+    positions (fix_volatiles) and the chains (widths, points, and how many
+    leading points of each row are the blob's own pixels or its volatile
+    point: the part of a row that involves no random draw)."""
+    r = op.ref()
+    rng = np.random.default_rng(77)
+    d = {"match_steps": np.array(MATCH_STEPS, np.int64), "match_ncases": np.int64(len(MATCH_CASES))}
+    for c, (thr, mn, num, delim, seed, nf, empty, degen, wts, (W, H)) in enumerate(MATCH_CASES):
+        fr = noisy_frames(rng, nf, W, H, nell=6)
+        if empty >= 0:
+            fr[empty] = np.zeros_like(fr[empty])
+        m = r.ref_morph_new()
+        for k, v in (("seed", seed), ("threads", 0), ("blob_delimiter", delim), ("blob_threshold", thr),
+                     ("blob_min_size", mn), ("blob_number", num), ("degeneration", degen),
+                     ("blob_rgba_weight", wts[0]), ("blob_size_weight", wts[1]), ("blob_xy_weight", wts[2])):
+            r.ref_morph_set_option(m, k.encode(), float(v))
+        for f, img in enumerate(fr):
+            a = np.ascontiguousarray(img)
+            r.ref_morph_add_rgba(m, f, op.ptr(a), W, H)
+        r.ref_morph_set_resolution(m, W, H)
+        state = r.ref_morph_run_until(m, 2, 1 << 40)
+        assert state == 2, STATE.get(state)
+        p = f"m{c}_"
+        d[p + "frames"] = np.stack(fr)
+        d[p + "opts"] = np.array([thr, mn, num, delim, seed, degen, *wts], np.float64)
+        d[p + "rng0"] = np.uint64(r.ref_morph_rng_state(m))
+        lists = [_blob_list(r, m, f) for f in range(nf)]
+        d[p + "nblobs0"] = np.array([len(l) for l in lists], np.int32)
+        d[p + "sizes0"] = np.array([b[0] for l in lists for b in l], np.int64)
+        d[p + "first0"] = np.array([b[1] for l in lists for b in l], np.int64)
+        d[p + "means0"] = np.array([b[2] for l in lists for b in l]).reshape(-1, 6)
+        fm = np.zeros((nf, 6))
+        for f in range(nf):
+            r.ref_morph_frame_means(m, f, op.ptr(fm[f]))
+        d[p + "fmeans"] = fm
+        bbox = np.zeros(4, np.int32)
+        r.ref_morph_bbox(m, op.ptr(bbox))
+        d[p + "bbox"] = bbox
+        bw = max(len(l) for l in lists)
+        orders, groups, energies = [], [], []
+        done = 0
+        for n in MATCH_STEPS:
+            state = r.ref_morph_run(m, n - done)
+            done = n
+            assert state == 2, STATE.get(state)
+            order = np.full((nf, bw), -2, np.int32)
+            group = np.full((nf, bw), -1, np.int32)
+            for f in range(nf):
+                firsts = [b[1] for b in lists[f]]
+                for b, (cnt, first, mm, g) in enumerate(_blob_list(r, m, f)):
+                    order[f, b] = firsts.index(first) if cnt else -1
+                    group[f, b] = g
+            e3 = np.zeros(3)
+            r.ref_morph_match_energy(m, op.ptr(e3))
+            orders.append(order)
+            groups.append(group)
+            energies.append(e3)
+        d[p + "order"] = np.stack(orders)
+        d[p + "group"] = np.stack(groups)
+        d[p + "energy"] = np.stack(energies)
+        r.ref_morph_next_state(m)
+        state = r.ref_morph_run(m, 1)
+        assert state == 3, STATE.get(state)
+        fixed = np.zeros((nf, bw, 2))
+        for f in range(nf):
+            for b, (cnt, first, mm, g) in enumerate(_blob_list(r, m, f)):
+                fixed[f, b] = mm[:2]
+        d[p + "fixed_xy"] = fixed
+        nch = r.ref_morph_chain_count(m)
+        widths, pts, det = [], [], []
+        gsize = {}
+        for f in range(nf):
+            for (cnt, first, mm, g) in _blob_list(r, m, f):
+                gsize[(g, f)] = cnt
+        for key in range(nch):
+            hh, ms = C.c_size_t(), C.c_size_t()
+            w = r.ref_morph_chain(m, key, C.byref(hh), C.byref(ms), None)
+            q = np.zeros(w * hh.value, np.uint64)
+            r.ref_morph_chain(m, key, C.byref(hh), C.byref(ms), op.ptr(q))
+            widths.append(w)
+            pts.append(q)
+            det.append([max(gsize[(key, f)], 1) for f in range(nf)])
+        d[p + "chain_w"] = np.array(widths, np.int64)
+        d[p + "chain_pts"] = np.concatenate(pts)
+        d[p + "chain_det"] = np.array(det, np.int64)
+        r.ref_morph_free(m)
+    return d
+
+
 def main():
     if not op.have_ref():
         sys.exit("oracle/_ref/libamref.so is missing: run `make -C oracle ref` where the reference exists")
-    which = set(sys.argv[1:]) or {"functions", "cfg1", "blobify"}
+    which = set(sys.argv[1:]) or {"functions", "cfg1", "blobify", "match", "density"}
     if "functions" in which:
         f = functions_fixture()
         np.savez_compressed(os.path.join(OUT, "functions.npz"), **f)
@@ -339,6 +477,21 @@ def main():
         b = blobify_fixture()
         np.savez_compressed(os.path.join(OUT, "blobify.npz"), **b)
         print("blobify: blobs per case and frame", b["bf_nblobs"].tolist())
+    if "density" in which:
+        # the cfg1 pipeline at the reference CLI's default density 2 (and 3):
+        # chains max_surface * density wide, alpha min(1, taps/density)
+        dd = {}
+        for dens in (2, 3):
+            picks = [v for v in CFG1_VARIANTS if v[0] in ("default", "spline_perlin_bg_feather2", "average_blobs")]
+            c = run_cfg1(density=dens, variants=picks if dens == 2 else picks[:1])
+            for k, v in c.items():
+                dd[f"d{dens}_{k}"] = v
+        np.savez_compressed(os.path.join(OUT, "density.npz"), **dd)
+        print("density: chain widths", dd["d2_chain_w"].tolist(), dd["d3_chain_w"].tolist())
+    if "match" in which:
+        mt = match_fixture()
+        np.savez_compressed(os.path.join(OUT, "match.npz"), **mt)
+        print("match: blobs per case", [mt[f"m{c}_nblobs0"].tolist() for c in range(len(MATCH_CASES))])
     for name in sorted(which):
         print(name + ".npz", os.path.getsize(os.path.join(OUT, name + ".npz")), "bytes")
 

@@ -56,8 +56,8 @@ class GpuRenderer:
         return G.to_host(self.out, np.uint32).view(np.uint8).reshape(self.H, self.W, 4).copy()
 
 
-def _renderer_for(g, variant):
-    cfg, keys, off, widths, order, nb, blob_rgba, (lag, slope) = render_inputs(g, variant)
+def _renderer_for(g, variant, density=1):
+    cfg, keys, off, widths, order, nb, blob_rgba, (lag, slope) = render_inputs(g, variant, density)
     vals = {"bbox": [cfg.bbox_x1, cfg.bbox_y1, cfg.bbox_x2, cfg.bbox_y2], "motion": cfg.motion,
             "fading": cfg.fading, "show": cfg.show_blobs, "density": cfg.density, "feather": cfg.feather,
             "keep_bg": cfg.keep_background}
@@ -95,3 +95,27 @@ def test_render_empty_and_negative_time(golden_cfg1):
     # normalize_time maps -0.75 to 0.25
     assert (rend.frame(-0.75) == rend.frame(0.25)).all()
     assert (rend.frame(3.25) == rend.frame(0.25)).all()
+
+
+DENSITY_CASES = [(2, "default"), (2, "spline_perlin_bg_feather2"), (2, "average_blobs"), (3, "default")]
+
+
+@pytest.mark.parametrize("density,variant", DENSITY_CASES)
+def test_render_at_density_within_one_lsb_of_reference(golden_density, density, variant):
+    """Chains max_surface * density wide (the reference CLI's default
+    density is 2) and alpha round(min(1, taps / density) * mean alpha)
+    (morph.cpp:606-612): GPU frames within 1 LSB of the reference's."""
+    g = golden_density(density)
+    rend, cfg = _renderer_for(g, variant, density)
+    assert cfg.density == density
+    worst, halved = 0, 0
+    for i, t in enumerate(g[f"render_{variant}_t"]):
+        G.log(f"render density {density} {variant} frame {i} t={t}")
+        got = rend.frame(t)
+        want = g[f"render_{variant}"][i]
+        worst = max(worst, int(np.abs(got.astype(np.int32) - want.astype(np.int32)).max()))
+        halved += int(((want[..., 3] > 0) & (want[..., 3] < 255)).sum())
+    G.log(f"density {density} {variant}: worst channel difference {worst}, partial-alpha pixels {halved}")
+    if cfg.show_blobs == 0:  # AVERAGE / DISTINCT draw the blob's own alpha (morph.cpp:1321-1334)
+        assert halved > 0  # pixels with fewer taps than the density exist
+    assert worst <= 1

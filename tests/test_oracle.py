@@ -409,8 +409,8 @@ def test_lap_matches_scipy_and_brute_force():
 
 
 # --------------------------------------------------------------- render
-def render_inputs(g, variant):
-    """Oracle render inputs from the cfg1 fixture."""
+def render_inputs(g, variant, density=1):
+    """Oracle render inputs from the cfg1 fixture (or a density.npz part)."""
     h = 2
     opts = g[f"render_{variant}_opts"]
     motion, fading, keep_bg, feather, show, finite = [int(v) for v in opts]
@@ -422,7 +422,7 @@ def render_inputs(g, variant):
     lag, slope = op.perlin_perm(1), op.perlin_perm(2)
     bbox = g["bbox"]
     cfg = op.RenderCfg(64, 64, h, int(bbox[0]), int(bbox[1]), int(bbox[2]), int(bbox[3]), motion, fading,
-                       show, 1, feather, keep_bg, 0, 0, op.ptr(lag), op.ptr(slope))
+                       show, density, feather, keep_bg, 0, 0, op.ptr(lag), op.ptr(slope))
     blob_rgba = np.zeros((h, nb), np.uint32)
     for f in range(h):
         for b in range(nb):
@@ -446,6 +446,29 @@ def test_render_matches_reference_frames(golden_cfg1, variant):
                           op.ptr(order), nb, op.ptr(blob_rgba), float(t), op.ptr(out))
         want = g[f"render_{variant}"][i]
         assert (out == want).all(), (variant, i, int((out != want).sum()))
+
+
+@pytest.mark.parametrize("density,variant", [(2, "default"), (2, "spline_perlin_bg_feather2"),
+                                             (2, "average_blobs"), (3, "default")])
+def test_render_at_density_matches_reference_frames(golden_density, density, variant):
+    """Density 2 and 3: wider chains (max_surface * density) and the
+    min(1, taps / density) alpha factor (morph.cpp:606-612)."""
+    g = golden_density(density)
+    cfg, keys, off, widths, order, nb, blob_rgba, keep = render_inputs(g, variant, density)
+    # chain c is max over key frames of its blob's size, times the density
+    # (thread.cpp:771-778)
+    for c in range(len(widths)):
+        biggest = max(int(g["blob_sizes"][f][g["order"][f] == c].max(initial=0)) for f in range(2))
+        assert widths[c] == density * biggest, (c, widths[c], biggest)
+    pts = np.ascontiguousarray(g["chain_after"])
+    out = np.zeros((64, 64, 4), np.uint8)
+    worst = 0
+    for i, t in enumerate(g[f"render_{variant}_t"]):
+        o.or_render_atoms(C.byref(cfg), op.ptr(keys), op.ptr(pts), op.ptr(off), op.ptr(widths), len(widths),
+                          op.ptr(order), nb, op.ptr(blob_rgba), float(t), op.ptr(out))
+        want = g[f"render_{variant}"][i]
+        worst = max(worst, int(np.abs(out.astype(int) - want.astype(int)).max()))
+    assert worst == 0, (density, variant, worst)
 
 
 def test_get_time_closed_form():
