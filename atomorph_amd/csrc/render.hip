@@ -17,6 +17,8 @@
 //            background blend, RGBA8 out.
 // Composite order equals the reference's (blob order); sums are exact, so
 // nothing depends on the order the atoms land in.
+#include <mutex>
+
 #include "colour.h"
 #include "common.h"
 #include "../../include/atomorph_hip.h"
@@ -405,12 +407,24 @@ extern "C" int amh_render_workspace_init(void *ws, size_t ws_bytes, int32_t W, i
     return AMH_OK;
 }
 
+namespace {
+
+bool render_args_ok(const amh_render *R) {
+    return R && R->width > 0 && R->height > 0 && R->nkeys > 0 && R->nchains >= 0 && R->keys_dev &&
+           !(R->wtot > 0 && (!R->pts_dev || !R->chain_off_dev || !R->chain_w_dev || !R->rank_dev)) &&
+           !(R->fading == kPerlin && (!R->lag_perm_dev || !R->slope_perm_dev)) && R->width <= 65535 &&
+           R->height <= 65535 && R->wtot >= 0;
+}
+
+// The frame itself, on a workspace laid out by `L` whose accumulators,
+// overflow counts and smallest-blob map are clean; leaves them clean.
+int render_frame_on(const amh_render *R, double t, char *base, const Layout &L, uint32_t *out, hipStream_t s);
+
+}  // namespace
+
 extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t ws_bytes, uint32_t *out,
                                 void *stream) {
-    if (!R || !ws || !out || R->width <= 0 || R->height <= 0 || R->nkeys <= 0 || R->nchains < 0 ||
-        !R->keys_dev || (R->wtot > 0 && (!R->pts_dev || !R->chain_off_dev || !R->chain_w_dev || !R->rank_dev)) ||
-        (R->fading == kPerlin && (!R->lag_perm_dev || !R->slope_perm_dev)) || R->width > 65535 ||
-        R->height > 65535) {
+    if (!render_args_ok(R) || !ws || !out) {
         set_error("amh_render_frame: invalid argument");
         return AMH_E_ARG;
     }
@@ -419,9 +433,87 @@ extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t 
         set_error("amh_render_frame: workspace %zu < %zu bytes", ws_bytes, L.total);
         return AMH_E_ARG;
     }
+    return render_frame_on(R, t, static_cast<char *>(ws), L, out, static_cast<hipStream_t>(stream));
+}
+
+// ------------------------------------------- allocation-free frame render
+// The same frame render for callers that cannot allocate (code under HIP
+// graph capture, serving loops, harnesses that forbid hipMalloc): the
+// workspace is a fixed array in device memory, one per GPU, held until the
+// process ends.  Every call on a GPU shares it, so calls that overlap on one
+// GPU race; each call clears the parts that must start clean with a fill
+// kernel (no memset node, which graph capture would record).
+namespace {
+
+constexpr size_t kScratchBytes = size_t(96) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device.  The array's name
+// is only a host-side stand-in, so kernels and copies get this pointer.
+int scratch_ptr(char **out) {
+    int dev = -1;
+    if (int r = check_hip(hipGetDevice(&dev), "amh_render_frame_noalloc: hipGetDevice")) return r;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("amh_render_frame_noalloc: device %d beyond %d", dev, kMaxDevices);
+        return AMH_E_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (int r = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                              "amh_render_frame_noalloc: hipGetSymbolAddress"))
+            return r;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<char *>(g_scratch_ptr[dev]);
+    return AMH_OK;
+}
+
+// bmin := INT_MAX, overflow counts and accumulators := 0.
+__global__ __launch_bounds__(256) void k_ws_reset(int *__restrict__ bmin, int *__restrict__ ovcount,
+                                                  unsigned long long *__restrict__ acc, int64_t npx) {
+    const int64_t stride = int64_t(gridDim.x) * 256;
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < npx; i += stride) {
+        bmin[i] = 0x7FFFFFFF;
+        ovcount[i] = 0;
+        for (int k = 0; k < 5; ++k) acc[int64_t(k) * npx + i] = 0ull;
+    }
+}
+
+}  // namespace
+
+extern "C" size_t amh_render_noalloc_max_bytes(void) { return kScratchBytes; }
+
+extern "C" int amh_render_frame_noalloc(const amh_render *R, double t, uint32_t *out, void *stream) {
+    if (!render_args_ok(R) || !out) {
+        set_error("amh_render_frame_noalloc: invalid argument");
+        return AMH_E_ARG;
+    }
+    const Layout L = layout(R->width, R->height, R->wtot);
+    if (L.total > kScratchBytes) {
+        set_error("amh_render_frame_noalloc: %dx%d with %lld atoms needs %zu bytes of workspace, more than the "
+                  "%zu this entry point holds; use amh_render_frame",
+                  R->width, R->height, (long long)R->wtot, L.total, kScratchBytes);
+        return AMH_E_ARG;
+    }
+    char *base = nullptr;
+    if (int r = scratch_ptr(&base)) return r;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint32_t grid = uint32_t(std::min<int64_t>((L.npx + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_ws_reset, dim3(grid), dim3(256), 0, s, reinterpret_cast<int *>(base + L.bmin),
+                       reinterpret_cast<int *>(base + L.ovcount),
+                       reinterpret_cast<unsigned long long *>(base + L.acc), L.npx);
+    AMH_LAUNCH_CHECK("amh_render_frame_noalloc");
+    return render_frame_on(R, t, base, L, out, s);
+}
+
+namespace {
+int render_frame_on(const amh_render *R, double t, char *base, const Layout &L, uint32_t *out, hipStream_t s) {
     amh::ProfScope whole_("render_frame", s);
-    char *base = static_cast<char *>(ws);
     int *bmin = reinterpret_cast<int *>(base + L.bmin);
     int *ovcount = reinterpret_cast<int *>(base + L.ovcount);
     int *ovstart = reinterpret_cast<int *>(base + L.ovstart);
@@ -465,3 +557,4 @@ extern "C" int amh_render_frame(const amh_render *R, double t, void *ws, size_t 
     AMH_LAUNCH_CHECK("amh_render_frame");
     return AMH_OK;
 }
+}  // namespace

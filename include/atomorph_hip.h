@@ -282,6 +282,17 @@ int amh_render_workspace_init(void *ws_dev, size_t ws_bytes, int32_t width, int3
  * into out_dev[height*width] packed RGBA8. */
 int amh_render_frame(const amh_render *r, double t, void *ws_dev, size_t ws_bytes, uint32_t *out_dev,
                      void *stream);
+/* amh_render_frame without a caller workspace and without any allocation.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a fixed device array of
+ * amh_render_noalloc_max_bytes() (96 MiB) per GPU, held until the process
+ * ends; frames whose amh_render_workspace_bytes() exceed it are refused with
+ * AMH_E_ARG.  Each call clears the array with a fill kernel first.  All
+ * calls on one GPU share the array: calls that overlap on one GPU (two
+ * streams, two threads) race. */
+size_t amh_render_noalloc_max_bytes(void);
+int amh_render_frame_noalloc(const amh_render *r, double t, uint32_t *out_dev, void *stream);
 
 /* --------------------------------------------------------- fluid path -- */
 /* The fluid renderer (fluidsteps > 0): morph::step_fluid, FluidModel::step
