@@ -1720,6 +1720,37 @@ extern "C" int amh_fluid_ranks(const uint64_t *pts_dev, int32_t nkeys, int64_t w
     return AMH_OK;
 }
 
+namespace {
+// draw_fluid after step_fluid (morph.cpp:1082-1209) on the cell-ordered
+// copy f->S: colour relax and splat, resolve into f->img / f->mask, then the
+// particles (colours relaxed, mature set) back to f->P.
+void draw_splat(amh_fluid_s *f, const amh_render &R, const Frame &F, double sparseness, hipStream_t s) {
+    amh::ProfScope ps_("k_fluid_splat", s);
+    const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
+    const int ptiles = f->pbins.ntx * f->pbins.nty;
+    const int par = f->pbin_par;
+    f->pbin_par ^= 1;
+    hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, F.tl, f->W, f->H, sparseness,
+                       R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps, f->scol, f->pbins, par);
+    bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true);
+    hipLaunchKernelGGL(k_splat_tile, dim3(uint32_t(ptiles)), dim3(kT), 0, s, f->pbins, f->sxy, f->staps, f->scol,
+                       f->W, f->H, F.tl, R.keep_background, f->img, f->mask);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->pos, f->P);
+}
+
+// The outward feather layers and the background blend (morph.cpp:1211-1298).
+void draw_finish(amh_fluid_s *f, const amh_render &R, const Frame &F, uint32_t *out, hipStream_t s) {
+    const int64_t npx = int64_t(f->W) * f->H;
+    const uint32_t gpx = uint32_t((npx + kT - 1) / kT);
+    for (int layer = 0; layer < R.feather; ++layer) {
+        const double alpha = 1.0 - (double(layer + 1) / double(R.feather + 1));
+        hipLaunchKernelGGL(k_fluid_feather, dim3(gpx), dim3(kT), 0, s, f->W, f->H, alpha, f->img, f->mask);
+        hipLaunchKernelGGL(k_fluid_commit, dim3(gpx), dim3(kT), 0, s, f->mask, npx);
+    }
+    hipLaunchKernelGGL(k_fluid_compose, dim3(gpx), dim3(kT), 0, s, R, F, f->img, f->mask, out);
+}
+}  // namespace
+
 extern "C" int amh_fluid_reset(void *fluid, double sparseness, void *stream) {
     amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);
     if (!f || !(sparseness > 0.0)) { set_error("amh_fluid_reset: invalid argument"); return AMH_E_ARG; }
@@ -1765,29 +1796,39 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
                                         mix64(f->frame * 131 + k), s, k > 0, k + 1 < fa->fluidsteps);
             if (r) return r;
         }
-        {
-            amh::ProfScope ps_("k_fluid_splat", s);
-            const int ptiles = f->pbins.ntx * f->pbins.nty;
-            const int par = f->pbin_par;
-            f->pbin_par ^= 1;
-            hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, F.tl, f->W, f->H,
-                               fa->sparseness, R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps,
-                               f->scol, f->pbins, par);
-            bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true);
-            hipLaunchKernelGGL(k_splat_tile, dim3(uint32_t(ptiles)), dim3(kT), 0, s, f->pbins, f->sxy, f->staps,
-                               f->scol, f->W, f->H, F.tl, R.keep_background, f->img, f->mask);
-        }
-        hipLaunchKernelGGL(k_sort_scatter, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->pos, f->P);
+        draw_splat(f, R, F, fa->sparseness, s);
     } else {  // nothing drawn: every pixel empty
         hipLaunchKernelGGL(k_fill_i32, dim3(gpx), dim3(kT), 0, s, f->mask, npx);
     }
-    for (int layer = 0; layer < R.feather; ++layer) {
-        const double alpha = 1.0 - (double(layer + 1) / double(R.feather + 1));
-        hipLaunchKernelGGL(k_fluid_feather, dim3(gpx), dim3(kT), 0, s, f->W, f->H, alpha, f->img, f->mask);
-        hipLaunchKernelGGL(k_fluid_commit, dim3(gpx), dim3(kT), 0, s, f->mask, npx);
-    }
-    hipLaunchKernelGGL(k_fluid_compose, dim3(gpx), dim3(kT), 0, s, R, F, f->img, f->mask, out);
+    draw_finish(f, R, F, out, s);
     ++f->frame;
     AMH_LAUNCH_CHECK("amh_fluid_render");
+    return AMH_OK;
+}
+
+// The drawing half on its own, on the particles as they are.
+extern "C" int amh_fluid_draw(void *fluid, const amh_fluid_frame *fa, double t, uint32_t *out, void *stream) {
+    amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);
+    if (!f || !fa || !fa->r || !out || !(fa->sparseness > 0.0)) {
+        set_error("amh_fluid_draw: invalid argument");
+        return AMH_E_ARG;
+    }
+    const amh_render &R = *fa->r;
+    if (R.width != f->W || R.height != f->H || R.nkeys <= 0 || !R.keys_dev ||
+        (R.keep_background && R.fading == kPerlin && (!R.lag_perm_dev || !R.slope_perm_dev))) {
+        set_error("amh_fluid_draw: render arguments do not match the model (%dx%d)", f->W, f->H);
+        return AMH_E_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const Frame F = frame_of(t, R.nkeys);
+    if (F.fk < R.nkeys && f->count > 0) {
+        if (int r = sort_particles(f, s)) return r;
+        draw_splat(f, R, F, fa->sparseness, s);
+    } else {
+        const int64_t npx = int64_t(f->W) * f->H;
+        hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((npx + kT - 1) / kT)), dim3(kT), 0, s, f->mask, npx);
+    }
+    draw_finish(f, R, F, out, s);
+    AMH_LAUNCH_CHECK("amh_fluid_draw");
     return AMH_OK;
 }

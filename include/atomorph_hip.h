@@ -339,6 +339,14 @@ typedef struct amh_fluid_frame {
 /* One frame at global time t: particle update, fluidsteps model steps,
  * splat, resolve, outward feather, background; out_dev[width*height]. */
 int amh_fluid_render(void *fluid, const amh_fluid_frame *frame, double t, uint32_t *out_dev, void *stream);
+/* The drawing half of morph::draw_fluid (morph.cpp:1082-1298) alone, on the
+ * model's particles as they are (after amh_fluid_set_particles, say): each
+ * active particle's colour relaxes towards its target (kept in the model)
+ * and is splatted bilinearly; pixels take the weighted means, the feather
+ * grows the drawn region outwards, the background is blended.  Uses only
+ * frame->r (size, key frames, bbox, feather, keep_background, fading and
+ * Perlin tables) and frame->sparseness. */
+int amh_fluid_draw(void *fluid, const amh_fluid_frame *frame, double t, uint32_t *out_dev, void *stream);
 
 /* ---- learned proposal: MLP on the matrix cores -------------------------- */
 /* No counterpart in the reference (its swap descent starts from row-major

@@ -29,6 +29,8 @@
  *                                       + brute-force flood fill
  *   or_render_atoms                   - vs reference rendered-frame fixtures
  *   or_fluid_step                     - vs compiled reference FluidModel
+ *   or_draw_fluid                     - vs the compiled reference's fluid
+ *                                       frames drawn from dumped particles
  *   or_match                          - vs the compiled reference's blob
  *                                       groups, list order, energies and
  *                                       fix_volatiles positions (match.npz)
@@ -177,6 +179,15 @@ double or_get_time(size_t frame_index, size_t total_frames, int finite,
 void or_fluid_step(double *particles, int32_t count, int32_t gsx, int32_t gsy,
                    int32_t steps_left, double freedom_radius,
                    unsigned jitter_seed);
+
+/* The drawing half of morph::draw_fluid (morph.cpp:1082-1298), after
+ * step_fluid: particles (layout above; colours relaxed and mature set in
+ * place) at local time t of the key segment on a width x height image,
+ * fluid grid sparseness, bbox x1 y1 x2 y2, outward feather layers,
+ * keep_background (bg: width*height RGBA8 get_background colours, used only
+ * then).  out: width*height RGBA8. */
+void or_draw_fluid(double *particles, int32_t count, double t, int32_t width, int32_t height, double sparseness,
+                   const int32_t *bbox, int32_t feather, int32_t keep_background, const uint8_t *bg, uint8_t *out);
 
 /* ---- blob matching (thread.cpp:598-738, 1087-1107, 1141-1174, 1187-1233) */
 /* thread::match for `steps` worker steps in BLOB_MATCHING (the first pads

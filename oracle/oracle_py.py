@@ -49,6 +49,7 @@ _ORACLE = {
     "or_distinct_color": (u32, [sz]),
     "or_fluid_step": (None, [vp, i32, i32, i32, i32, f64, C.c_uint]),
     "or_lap_min_cost": (i64, [vp, i32, vp]),
+    "or_draw_fluid": (None, [vp, i32, f64, i32, i32, f64, vp, i32, i32, vp, vp]),
     "or_match": (i32, [i32, vp, vp, vp, vp, vp, vp, u64, u64, u64, C.c_int, vp, vp, vp, vp, C.POINTER(u64)]),
     "or_match_energy": (f64, [i32, i32, vp, vp, vp, vp, vp]),
     "or_match_brute": (f64, [i32, i32, vp, vp, vp, vp, vp]),
@@ -91,6 +92,9 @@ _REF = {
     "ref_morph_rng_state": (u64, [vp]),
     "ref_morph_frame_means": (None, [vp, sz, vp]),
     "ref_morph_match_energy": (None, [vp, vp]),
+    "ref_morph_particle_count": (sz, [vp]),
+    "ref_fluid_frame": (C.c_int, [vp, f64, C.c_uint, vp, vp, vp]),
+    "ref_morph_background": (None, [vp, f64, vp]),
 }
 
 

@@ -252,6 +252,79 @@ void ref_morph_match_energy(void *mp, double *e3) {
     e3[1] = m->worker.best_e;
     e3[2] = m->worker.best_blob_map_e;
 }
+namespace {
+void dump_particles(const Particle *ps, size_t n, double *out) {
+    for (size_t i = 0; i < n; ++i) {
+        const Particle *p = &ps[i];
+        double *q = out + 20 * i;
+        q[0] = p->x; q[1] = p->y; q[2] = p->u; q[3] = p->v;
+        q[4] = p->gravity_x; q[5] = p->gravity_y; q[6] = p->freedom_r;
+        q[7] = p->R; q[8] = p->G; q[9] = p->B; q[10] = p->A;
+        q[11] = p->r; q[12] = p->g; q[13] = p->b; q[14] = p->a;
+        q[15] = p->strength; q[16] = p->active ? 1.0 : 0.0; q[17] = p->mature ? 1.0 : 0.0;
+        q[18] = 0.0; q[19] = 0.0;
+    }
+}
+}  // namespace
+
+/* Number of particles of the morph's fluid model (0: none yet). */
+size_t ref_morph_particle_count(void *mp) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    return m->fluid ? m->fluid->get_particle_count() : 0;
+}
+/* One fluid frame at global time `time`, split at the boundary between
+ * step_fluid and the drawing half of morph::draw_fluid (morph.cpp:1057-1300):
+ * from the current state step_fluid runs and its particles are written to
+ * P_pre; the particles and the random engine are then put back and
+ * get_pixels(time) renders the frame the normal way (step_fluid + drawing)
+ * into img; P_post receives the particles after it.  rand() is reseeded
+ * with srand_seed before each step_fluid.  The drawn frame was therefore
+ * made from P_pre when the second step_fluid reproduced it: returns 1 when
+ * the positions after get_pixels equal P_pre's, 0 if not, -1 without a
+ * fluid model. */
+int ref_fluid_frame(void *mp, double time, unsigned srand_seed, double *P_pre, uint8_t *img, double *P_post) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    if (!m->fluid) return -1;
+    FluidModel *fm = m->fluid;
+    const size_t n = fm->get_particle_count();
+    Particle *ps = fm->getParticles();
+    std::vector<Particle> saved(ps, ps + n);
+    const std::default_random_engine e_saved = m->e1;
+    const double tn = m->normalize_time(time);
+    const size_t fk = m->get_frame_key(tn);
+    const double dt = 1.0 / double(m->frames.size());
+    const double tl = std::max(0.0, (tn - (fk * dt)) / dt);
+    srand(srand_seed);
+    m->step_fluid(fk, tl, tn);
+    dump_particles(ps, n, P_pre);
+    std::copy(saved.begin(), saved.end(), ps);
+    m->e1 = e_saved;
+    srand(srand_seed);
+    std::vector<am::pixel> px;
+    m->get_pixels(time, &px);
+    const unsigned W = m->get_width(), H = m->get_height();
+    std::memset(img, 0, size_t(W) * H * 4);
+    for (const am::pixel &q : px) {
+        const size_t pos = (size_t(q.y) * W + q.x) * 4;
+        if (pos >= size_t(W) * H * 4) continue;
+        img[pos + 0] = q.c.r; img[pos + 1] = q.c.g; img[pos + 2] = q.c.b; img[pos + 3] = q.c.a;
+    }
+    dump_particles(ps, n, P_post);
+    for (size_t i = 0; i < n; ++i)
+        if (P_post[20 * i] != P_pre[20 * i] || P_post[20 * i + 1] != P_pre[20 * i + 1]) return 0;
+    return 1;
+}
+/* get_background(x, y, time) for every pixel (morph.cpp:1431-1465). */
+void ref_morph_background(void *mp, double time, uint8_t *img) {
+    am::morph *m = static_cast<am::morph *>(mp);
+    const unsigned W = m->get_width(), H = m->get_height();
+    for (unsigned y = 0; y < H; ++y)
+        for (unsigned x = 0; x < W; ++x) {
+            const am::color c = m->get_background(uint16_t(x), uint16_t(y), time);
+            uint8_t *o = img + (size_t(y) * W + x) * 4;
+            o[0] = c.r; o[1] = c.g; o[2] = c.b; o[3] = c.a;
+        }
+}
 size_t ref_morph_chain_count(void *mp) { return static_cast<am::morph *>(mp)->chains.size(); }
 /* Chain with key `key`: returns width, writes height and max_surface; if pts
  * != NULL copies atom-major points (padding byte cleared). */

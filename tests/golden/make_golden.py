@@ -26,8 +26,12 @@ Fixtures
                  positions and the chains init_morph builds.
   density.npz    the cfg1 pipeline at density 2 and 3 (the reference CLI's
                  default is 2): chains and rendered frames.
+  fluid.npz      the cfg1 pipeline with the fluid on: at six of 16 frames
+                 the particles between step_fluid and the drawing half of
+                 draw_fluid, the background, the drawn frame and the
+                 particles after drawing (feather and keep_background vary).
 
-    python tests/golden/make_golden.py [functions] [cfg1] [blobify] [match] [density]
+    python tests/golden/make_golden.py [functions] [cfg1] [blobify] [match] [density] [fluid]
 """
 import ctypes as C
 import os
@@ -461,10 +465,91 @@ def match_fixture():
     return d
 
 
+# Fluid frames of the cfg1 morph (fluid on, 3 steps per frame): frame index
+# of the 16 -> (feather, keep_background) where the drawing half is dumped.
+FLUID_DUMPS = {2: (0, 0), 5: (2, 0), 7: (0, 1), 9: (3, 1), 12: (1, 0), 14: (0, 0)}
+
+
+def fluid_draw_fixture():
+    """The cfg1 pipeline with the fluid on, rendered frame after frame; at
+    the FLUID_DUMPS frames the particles between step_fluid and the drawing
+    half of draw_fluid, the background, the drawn frame and the particles
+    after drawing."""
+    r = op.ref()
+    frames = synth.workload("ellipses64")
+    m = r.ref_morph_new()
+    for k, v in (("seed", 1), ("threads", 0), ("cycle_length", 1 << 16), ("fluid", 3)):
+        r.ref_morph_set_option(m, k.encode(), float(v))
+    for f, img in enumerate(frames):
+        a = np.ascontiguousarray(img)
+        r.ref_morph_add_rgba(m, f, op.ptr(a), 64, 64)
+    r.ref_morph_set_resolution(m, 64, 64)
+    state = r.ref_morph_run_until(m, 2, 1 << 40)
+    r.ref_morph_run(m, 2000)
+    r.ref_morph_next_state(m)
+    assert r.ref_morph_run(m, 1) == 3
+    r.ref_morph_run(m, r.ref_morph_chain_count(m))
+    r.ref_morph_next_state(m)
+    assert r.ref_morph_run(m, 1) == 4
+    n = r.ref_morph_particle_count(m)
+    assert n > 0
+    F = 16
+    d = {"fl_frames": np.stack(frames)}
+    pre, post, imgs, bgs, ts, opts, idx = [], [], [], [], [], [], []
+    for i in range(F):
+        t = r.ref_morph_get_time(m, i, F)
+        feather, keep = FLUID_DUMPS.get(i, (0, 0))
+        r.ref_morph_set_option(m, b"feather", float(feather))
+        r.ref_morph_set_option(m, b"keep_background", float(keep))
+        if i not in FLUID_DUMPS:
+            out = np.zeros((64, 64, 4), np.uint8)
+            r.ref_morph_render(m, t, op.ptr(out))
+            continue
+        bg = np.zeros((64, 64, 4), np.uint8)
+        r.ref_morph_background(m, t, op.ptr(bg))
+        a, b = np.zeros((n, 20)), np.zeros((n, 20))
+        img = np.zeros((64, 64, 4), np.uint8)
+        ok = r.ref_fluid_frame(m, t, 1000 + i, op.ptr(a), op.ptr(img), op.ptr(b))
+        assert ok == 1, (i, ok)
+        pre.append(a); post.append(b); imgs.append(img); bgs.append(bg); ts.append(t)
+        opts.append((feather, keep)); idx.append(i)
+    d["fl_index"] = np.array(idx, np.int32)
+    d["fl_t"] = np.array(ts)
+    d["fl_opts"] = np.array(opts, np.int32)
+    d["fl_pre"] = np.stack(pre)
+    d["fl_post"] = np.stack(post)[:, :, [11, 12, 13, 14, 17]]  # colours and mature after drawing
+    d["fl_img"] = np.stack(imgs)
+    d["fl_bg"] = np.stack(bgs)
+    bbox = np.zeros(4, np.int32)
+    r.ref_morph_bbox(m, op.ptr(bbox))
+    d["fl_bbox"] = bbox
+    shown = np.zeros((2, 64 * 64), np.uint32)
+    for f in range(2):
+        for i in range(64 * 64):
+            shown[f, i] = r.ref_morph_get_pixel(m, f, i % 64, i // 64)
+    d["fl_shown"] = shown
+    # chains (particles are laid out chain after chain, max_surface each at
+    # density 1) and blob sizes per key frame and chain (blob group)
+    widths = []
+    for key in range(r.ref_morph_chain_count(m)):
+        hh, ms = C.c_size_t(), C.c_size_t()
+        widths.append(r.ref_morph_chain(m, key, C.byref(hh), C.byref(ms), None))
+    d["fl_chain_w"] = np.array(widths, np.int64)
+    surf = np.zeros((2, len(widths)), np.int64)
+    for f in range(2):
+        for b in range(r.ref_morph_blob_count(m, f)):
+            g = C.c_size_t()
+            cnt = r.ref_morph_blob(m, f, b, C.byref(g), None, None)
+            surf[f, g.value] = cnt
+    d["fl_surface"] = surf
+    r.ref_morph_free(m)
+    return d
+
+
 def main():
     if not op.have_ref():
         sys.exit("oracle/_ref/libamref.so is missing: run `make -C oracle ref` where the reference exists")
-    which = set(sys.argv[1:]) or {"functions", "cfg1", "blobify", "match", "density"}
+    which = set(sys.argv[1:]) or {"functions", "cfg1", "blobify", "match", "density", "fluid"}
     if "functions" in which:
         f = functions_fixture()
         np.savez_compressed(os.path.join(OUT, "functions.npz"), **f)
@@ -488,6 +573,11 @@ def main():
                 dd[f"d{dens}_{k}"] = v
         np.savez_compressed(os.path.join(OUT, "density.npz"), **dd)
         print("density: chain widths", dd["d2_chain_w"].tolist(), dd["d3_chain_w"].tolist())
+    if "fluid" in which:
+        fl = fluid_draw_fixture()
+        np.savez_compressed(os.path.join(OUT, "fluid.npz"), **fl)
+        print("fluid: particles", fl["fl_pre"].shape[1], "active per dump",
+              [int(a[:, 16].sum()) for a in fl["fl_pre"]])
     if "match" in which:
         mt = match_fixture()
         np.savez_compressed(os.path.join(OUT, "match.npz"), **mt)

@@ -251,3 +251,52 @@ def test_fluid_activation_ranks():
             seg = got[y, o:o + w]
             assert np.array_equal(np.sort(seg), np.arange(w)), (y, c)
             assert np.array_equal(seg, want), (y, c)
+
+
+# ------------------------------------------------------------ fluid frames
+def _fluid_fixture():
+    import os
+    return dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fluid.npz")))
+
+
+def test_fluid_draw_within_one_lsb_of_reference():
+    """The drawing half of draw_fluid (colour relax, bilinear splat, FP64
+    resolve, outward feather, background) on the particles the compiled
+    reference had after step_fluid, at six frames of the cfg1 fluid morph
+    with feather 0-3 and keep_background on and off: frames within 1 LSB of
+    the reference's, relaxed colours and maturity equal to its particles."""
+    fx = _fluid_fixture()
+    lib = G.lib()
+    T = G.torch()
+    n = fx["fl_pre"].shape[1]
+    keys = G.to_dev(np.ascontiguousarray(fx["fl_shown"]).ravel())
+    lag, slope = G.to_dev(op.perlin_perm(1)), G.to_dev(op.perlin_perm(2))
+    out = G.zeros(64 * 64, T.int32)
+    bb = [int(v) for v in fx["fl_bbox"]]
+    fluid = C.c_void_p()
+    G.check(lib.amh_fluid_create(C.byref(fluid), 84, 84, n, 64, 64, 2, G.stream()), "create")
+    try:
+        for k, i in enumerate(fx["fl_index"]):
+            feather, keep = (int(v) for v in fx["fl_opts"][k])
+            t = float(fx["fl_t"][k])
+            G.log(f"fluid draw: frame {i} t={t} feather {feather} keep_background {keep}")
+            args = A.RenderArgs(64, 64, 2, 0, *bb, 4, 6, 0, 1, feather, keep, 0, 0, 0, keys.data_ptr(), None, None,
+                                None, None, None, lag.data_ptr(), slope.data_ptr())
+            fa = A.FluidFrame(C.pointer(args), None, None, None, 1.0, 3, 0)
+            P = np.ascontiguousarray(fx["fl_pre"][k])
+            G.check(lib.amh_fluid_set_particles(fluid, op.ptr(P), G.stream()), "set")
+            G.check(lib.amh_fluid_draw(fluid, C.byref(fa), t, G.dptr(out), G.stream()), "amh_fluid_draw")
+            img = G.to_host(out, np.uint32).view(np.uint8).reshape(64, 64, 4)
+            want = fx["fl_img"][k]
+            d = np.abs(img.astype(np.int32) - want.astype(np.int32))
+            G.log(f"  worst channel difference {int(d.max())}, differing channels {int((d > 0).sum())}, "
+                  f"drawn pixels {int((want[..., 3] > 0).sum())}")
+            assert d.max() <= 1, (i, int(d.max()))
+            Q = np.zeros((n, 20))
+            G.check(lib.amh_fluid_get_particles(fluid, op.ptr(Q), G.stream()), "get")
+            post = fx["fl_post"][k]
+            assert np.allclose(Q[:, 11:15], post[:, :4], rtol=0, atol=1e-12), i
+            assert (Q[:, 17] == post[:, 4]).all(), i
+            assert (Q[:, :2] == P[:, :2]).all()  # drawing moves nothing
+    finally:
+        lib.amh_fluid_destroy(fluid)

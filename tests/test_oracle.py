@@ -471,6 +471,49 @@ def test_render_at_density_matches_reference_frames(golden_density, density, var
     assert worst == 0, (density, variant, worst)
 
 
+def test_draw_fluid_matches_reference_frames():
+    """or_draw_fluid on the particles the compiled reference had after
+    step_fluid: the reference's frames and its particles' relaxed colours,
+    bit for bit (six frames, feather 0-3, keep_background on and off)."""
+    import os
+    fx = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fluid.npz")))
+    bbox = fx["fl_bbox"].astype(np.int32)
+    for k in range(len(fx["fl_index"])):
+        P = np.ascontiguousarray(fx["fl_pre"][k]).copy()
+        feather, keep = (int(v) for v in fx["fl_opts"][k])
+        tn = float(fx["fl_t"][k]) % 1.0
+        fk = int(tn * 2)
+        tl = max(0.0, (tn - fk * 0.5) / 0.5)
+        out = np.zeros((64, 64, 4), np.uint8)
+        bg = np.ascontiguousarray(fx["fl_bg"][k])
+        o.or_draw_fluid(op.ptr(P), P.shape[0], tl, 64, 64, 1.0, op.ptr(bbox), feather, keep, op.ptr(bg), op.ptr(out))
+        assert (out == fx["fl_img"][k]).all(), k
+        assert np.array_equal(P[:, [11, 12, 13, 14, 17]], fx["fl_post"][k]), k
+    # the fixture covers feathered, background-blended and partly drawn frames
+    assert {tuple(v) for v in fx["fl_opts"]} >= {(0, 0), (2, 0), (0, 1), (3, 1)}
+    assert all(0 < (img[..., 3] > 0).sum() < 64 * 64 for img in fx["fl_img"])
+
+
+def test_reference_fluid_active_counts_follow_interpolated_areas():
+    """step_fluid keeps round((1 - t)|A| + t|B|) particles of each chain
+    active (morph.cpp:829-832, 880-1025); the engine's activation by rank
+    keeps the same count per chain and frame.  Checked on the reference's
+    own particles after step_fluid (chain c owns the particles of its
+    max_surface slots, chain after chain, at density 1)."""
+    import os
+    fx = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fluid.npz")))
+    off = np.concatenate([[0], np.cumsum(fx["fl_chain_w"])])
+    surf = fx["fl_surface"]
+    for k in range(len(fx["fl_index"])):
+        tn = float(fx["fl_t"][k]) % 1.0
+        fk = int(tn * 2)
+        tl = max(0.0, (tn - fk * 0.5) / 0.5)
+        for c in range(len(off) - 1):
+            got = int(fx["fl_pre"][k][off[c]:off[c + 1], 16].sum())
+            want = int(np.floor((1 - tl) * surf[fk, c] + tl * surf[(fk + 1) % 2, c] + 0.5))
+            assert got == want, (k, c, got, want)
+
+
 def test_get_time_closed_form():
     assert o.or_get_time(3, 16, 0, 2) == 3 / 16
     assert o.or_get_time(0, 1, 1, 4) == (1 - 1 / 4) / 2
