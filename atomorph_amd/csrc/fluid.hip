@@ -877,7 +877,9 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
 }
 
 __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
-    // replaces rand() / RAND_MAX * 0.01 of the wall bounce (fluidmodel.cpp:554-564)
+    // replaces rand() / RAND_MAX * 0.01 of the wall bounce (fluidmodel.cpp:554-564);
+    // seed 0 bounces without jitter
+    if (seed == 0) return 0.0;
     return double(mix64(seed ^ (uint64_t(i) << 3) ^ uint64_t(k)) >> 11) * (1.0 / 9007199254740992.0) * 0.01;
 }
 
@@ -1793,7 +1795,7 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
         if (int r = sort_particles(f, s)) return r;
         for (int k = 0; k < fa->fluidsteps; ++k) {
             const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), radius,
-                                        mix64(f->frame * 131 + k), s, k > 0, k + 1 < fa->fluidsteps);
+                                        mix64(f->frame * 131 + k) | 1u, s, k > 0, k + 1 < fa->fluidsteps);
             if (r) return r;
         }
         draw_splat(f, R, F, fa->sparseness, s);

@@ -9,6 +9,7 @@
 #include "colour.h"
 #include "common.h"
 #include "../../include/atomorph_hip.h"
+#include "scan.h"
 
 using namespace amh;
 
@@ -425,6 +426,184 @@ __global__ __launch_bounds__(64) void k_group_scatter(const int *__restrict__ bo
     }
 }
 
+// Grouping with many blobs: a stable LSD radix sort of the pixels by blob
+// number (8-bit digits, as many passes as the blob numbers need), so the
+// memory is O(pixels + blobs) whatever the blob count.  Pixels without a
+// blob get key nblobs and sort last.  Per pass: a digit histogram per block
+// of kGrpTile pixels, an exclusive scan of the digit-major histograms, and a
+// scatter that ranks equal digits in pixel order with ballots (stable).
+constexpr int kGrpTile = 256 * 8;
+
+__device__ __forceinline__ uint32_t grp_key(bool from_bop, const int *__restrict__ bop,
+                                            const uint32_t *__restrict__ kin, int64_t e, int nblobs) {
+    if (!from_bop) return kin[e];
+    const int b = bop[e];
+    return b >= 0 ? uint32_t(b) : uint32_t(nblobs);
+}
+
+__global__ __launch_bounds__(256) void k_grp_hist(bool from_bop, const int *__restrict__ bop,
+                                                  const uint32_t *__restrict__ kin, int64_t n, int nblobs, int shift,
+                                                  int nblk, int *__restrict__ hist) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t b0 = int64_t(blockIdx.x) * kGrpTile;
+    for (int j = 0; j < kGrpTile / 256; ++j) {
+        const int64_t e = b0 + j * 256 + threadIdx.x;
+        if (e < n) atomicAdd(&h[(grp_key(from_bop, bop, kin, e, nblobs) >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[int64_t(threadIdx.x) * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one pass.  `last`: values go to `list` and only those of
+// real blobs (their places are below the number of blob pixels).
+__global__ __launch_bounds__(256) void k_grp_scatter(bool from_bop, const int *__restrict__ bop,
+                                                     const uint32_t *__restrict__ kin,
+                                                     const int32_t *__restrict__ vin, int64_t n, int nblobs,
+                                                     int shift, int nblk, const int *__restrict__ off, bool last,
+                                                     uint32_t *__restrict__ kout, int32_t *__restrict__ vout) {
+    __shared__ int base[256];
+    __shared__ int cnt[4][256];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    base[t] = off[int64_t(t) * nblk + blockIdx.x];
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int64_t b0 = int64_t(blockIdx.x) * kGrpTile;
+    for (int j = 0; j < kGrpTile / 256; ++j) {
+        for (int w = 0; w < 4; ++w) cnt[w][t] = 0;
+        __syncthreads();
+        const int64_t e = b0 + j * 256 + t;
+        const bool ok = e < n;
+        const uint32_t k = ok ? grp_key(from_bop, bop, kin, e, nblobs) : 0u;
+        const int32_t v = ok ? (from_bop ? int32_t(e) : vin[e]) : 0;
+        const int d = int((k >> shift) & 255u);
+        uint64_t same = __ballot(ok);
+        for (int bit = 0; bit < 8; ++bit) {
+            const uint64_t bb = __ballot((d >> bit) & 1);
+            same &= ((d >> bit) & 1) ? bb : ~bb;
+        }
+        const int rank = int(__popcll(same & below));
+        if (ok && rank == 0) cnt[wave][d] = int(__popcll(same));
+        __syncthreads();
+        int before = 0;
+        for (int w = 0; w < wave; ++w) before += cnt[w][d];
+        if (ok) {
+            const int pos = base[d] + before + rank;
+            if (!last) {
+                kout[pos] = k;
+                vout[pos] = v;
+            } else if (k < uint32_t(nblobs)) {
+                kout[pos] = k;
+                vout[pos] = v;
+            }
+        }
+        __syncthreads();
+        int tot = 0;
+        for (int w = 0; w < 4; ++w) tot += cnt[w][t];
+        base[t] += tot;
+        __syncthreads();
+    }
+}
+
+// Per blob, from the sorted keys of its pixels (places [0, *total)): the
+// start and end of its run; absent blobs keep start = end = 0.
+__global__ void k_grp_bounds_total(const uint32_t *__restrict__ key, const unsigned long long *__restrict__ ptotal,
+                                   int *__restrict__ start, int *__restrict__ end) {
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t total = int64_t(*ptotal);
+    if (p >= total) return;
+    const uint32_t k = key[p];
+    if (p == 0 || key[p - 1] != k) start[k] = int(p);
+    if (p == total - 1 || key[p + 1] != k) end[k] = int(p + 1);
+}
+
+__global__ void k_grp_counts(const int *__restrict__ start, const int *__restrict__ end, int nblobs,
+                             int64_t *__restrict__ count) {
+    const int b = int(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b < nblobs) count[b] = end[b] - start[b];
+}
+
+// Counts the pixels that have a blob.
+__global__ void k_grp_present(const int *__restrict__ bop, int64_t n, unsigned long long *__restrict__ total) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool has = i < n && bop[i] >= 0;
+    const uint64_t m = __ballot(has);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(total, (unsigned long long)__popcll(m));
+}
+
+struct GroupLayout {
+    bool radix;
+    int nblk;                   // table: pixel blocks; radix: sort tiles
+    int passes;
+    size_t hist, k0, v0, k1, v1, hoff, sums, offs, start, end, total, bytes;
+};
+
+GroupLayout group_layout(int64_t n, int64_t nblobs) {
+    GroupLayout L{};
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const int64_t tblk = (n + kScanBlock - 1) / kScanBlock;
+    const int64_t nb = nblobs > 0 ? nblobs : 1;
+    // the per-(block, blob) table when it is at most 2 ints per pixel
+    if (tblk * nb <= 2 * std::max<int64_t>(n, 1)) {
+        L.radix = false;
+        L.nblk = int(tblk);
+        L.bytes = size_t(tblk) * size_t(nb) * sizeof(int);
+        return L;
+    }
+    L.radix = true;
+    L.nblk = int((n + kGrpTile - 1) / kGrpTile);
+    int bits = 1;
+    while ((int64_t(1) << bits) <= nblobs) ++bits;  // keys 0..nblobs
+    L.passes = (bits + 7) / 8;
+    const size_t hn = size_t(256) * size_t(std::max(L.nblk, 1));
+    size_t o = 0;
+    L.k0 = o; o = al(o + size_t(n) * 4);
+    L.v0 = o; o = al(o + size_t(n) * 4);
+    L.k1 = o; o = al(o + size_t(n) * 4);
+    L.v1 = o; o = al(o + size_t(n) * 4);
+    L.hist = o; o = al(o + hn * 4);
+    L.hoff = o; o = al(o + (hn + 1) * 4);
+    L.sums = o; o = al(o + size_t(scan::blocks(int64_t(hn))) * 4);
+    L.offs = o; o = al(o + (size_t(scan::blocks(int64_t(hn))) + 1) * 4);
+    L.start = o; o = al(o + size_t(nb) * 4);
+    L.end = o; o = al(o + size_t(nb) * 4);
+    L.total = o; o = al(o + 8);
+    L.bytes = o;
+    return L;
+}
+
+int group_radix(const GroupLayout &L, const int32_t *bop, int64_t n, int nblobs, char *ws, int64_t *first,
+                int32_t *list, hipStream_t s) {
+    uint32_t *k[2] = {reinterpret_cast<uint32_t *>(ws + L.k0), reinterpret_cast<uint32_t *>(ws + L.k1)};
+    int32_t *v[2] = {reinterpret_cast<int32_t *>(ws + L.v0), reinterpret_cast<int32_t *>(ws + L.v1)};
+    int *hist = reinterpret_cast<int *>(ws + L.hist), *hoff = reinterpret_cast<int *>(ws + L.hoff);
+    int *sums = reinterpret_cast<int *>(ws + L.sums), *offs = reinterpret_cast<int *>(ws + L.offs);
+    int *start = reinterpret_cast<int *>(ws + L.start), *end = reinterpret_cast<int *>(ws + L.end);
+    unsigned long long *total = reinterpret_cast<unsigned long long *>(ws + L.total);
+    const int64_t hn = int64_t(256) * L.nblk;
+    int cur = 0;
+    for (int p = 0; p < L.passes; ++p) {
+        const bool from_bop = p == 0, last = p + 1 == L.passes;
+        hipLaunchKernelGGL(k_grp_hist, dim3(uint32_t(L.nblk)), dim3(256), 0, s, from_bop, bop, k[cur], n, nblobs,
+                           8 * p, L.nblk, hist);
+        scan::exclusive(hist, hn, sums, offs, hoff, s);
+        hipLaunchKernelGGL(k_grp_scatter, dim3(uint32_t(L.nblk)), dim3(256), 0, s, from_bop, bop, k[cur], v[cur], n,
+                           nblobs, 8 * p, L.nblk, hoff, last, k[cur ^ 1], last ? list : v[cur ^ 1]);
+        cur ^= 1;
+    }
+    // runs of the sorted keys -> counts -> first[] (exclusive scan)
+    if (int r = check_hip(hipMemsetAsync(ws + L.start, 0, L.total + 8 - L.start, s), "amh_group_pixels memset"))
+        return r;
+    const uint32_t gn = uint32_t((n + 255) / 256);
+    hipLaunchKernelGGL(k_grp_present, dim3(gn), dim3(256), 0, s, bop, n, total);
+    // the blob pixels are the first `total` places; bounds over all n places
+    // would read the keys of blobless pixels, which the last pass skipped
+    hipLaunchKernelGGL(k_grp_bounds_total, dim3(gn), dim3(256), 0, s, k[cur], total, start, end);
+    hipLaunchKernelGGL(k_grp_counts, dim3(uint32_t((nblobs + 255) / 256)), dim3(256), 0, s, start, end, nblobs,
+                       first);
+    return AMH_OK;
+}
+
 // ------------------------------------------------------------ chain build
 // Row y of every chain (thread.cpp:803-848).
 __global__ void k_build_row(int y, int h, int64_t wtot, int nchains, const int64_t *__restrict__ off,
@@ -569,8 +748,7 @@ extern "C" int amh_blob_stats(const int32_t *labels, const uint32_t *stored, int
 }
 
 extern "C" size_t amh_group_workspace_bytes(int32_t npixels, int32_t nblobs) {
-    const int64_t nblk = (int64_t(npixels) + kScanBlock - 1) / kScanBlock;
-    return size_t(nblk) * size_t(nblobs > 0 ? nblobs : 1) * sizeof(int);
+    return group_layout(npixels > 0 ? npixels : 0, nblobs > 0 ? nblobs : 0).bytes;
 }
 
 extern "C" int amh_group_pixels(const int32_t *bop, int32_t npixels, int32_t nblobs, void *ws, size_t ws_bytes,
@@ -583,7 +761,14 @@ extern "C" int amh_group_pixels(const int32_t *bop, int32_t npixels, int32_t nbl
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (nblobs == 0) return check_hip(hipMemsetAsync(first, 0, sizeof(int64_t), s), "amh_group_pixels");
     const int64_t n = npixels;
-    const int nblk = int((n + kScanBlock - 1) / kScanBlock);
+    const GroupLayout L = group_layout(n, nblobs);
+    if (L.radix) {
+        if (int r = group_radix(L, bop, n, int(nblobs), static_cast<char *>(ws), first, list, s)) return r;
+        hipLaunchKernelGGL(k_first_scan, dim3(1), dim3(1024), 0, s, int(nblobs), first);
+        AMH_LAUNCH_CHECK("amh_group_pixels");
+        return AMH_OK;
+    }
+    const int nblk = L.nblk;
     int *hist = static_cast<int *>(ws);
     if (int r = check_hip(hipMemsetAsync(hist, 0, size_t(nblk) * nblobs * sizeof(int), s), "amh_group_pixels memset"))
         return r;

@@ -314,7 +314,8 @@ int amh_fluid_reset(void *fluid, double sparseness, void *stream);
 int amh_fluid_set_particles(void *fluid, const double *host, void *stream);
 int amh_fluid_get_particles(void *fluid, double *host, void *stream);
 /* One FluidModel::step (fluidmodel.cpp:165-580); the wall bounce jitter
- * (rand() in the reference) comes from `seed`. */
+ * (rand() in the reference) comes from `seed`, and seed 0 bounces without
+ * jitter. */
 int amh_fluid_step(void *fluid, int32_t steps_left, double freedom_radius, uint64_t seed, void *stream);
 /* Activation order of the fluid particles (no counterpart in the reference,
  * whose std::map bookkeeping births and kills particles, morph.cpp:915-1025):

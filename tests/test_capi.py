@@ -149,3 +149,12 @@ def test_density_and_fluid_coupling(morph):
     morph.set(fluid=10)
     morph.set(density=2)  # turns fluid off
     morph.set(fluid=5)    # turns density back to 1
+
+
+def test_group_workspace_is_linear_in_pixels_and_blobs():
+    """Grouping pixels by blob needs O(pixels + blobs) workspace for any blob
+    count (a dithered 2048^2 frame has about two million blobs)."""
+    lib = A.load()
+    for npx, nb in ((1024 * 1024, 150), (1024 * 1024, 524288), (2048 * 2048, 2097152), (2048 * 2048, 1200),
+                    (64 * 64, 1), (1000, 999)):
+        assert lib.amh_group_workspace_bytes(npx, nb) <= 24 * npx + 16 * nb + (1 << 20), (npx, nb)

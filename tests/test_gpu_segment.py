@@ -141,3 +141,77 @@ def test_blob_stats_and_grouping(which, golden_cfg1):
             assert (plist[first[b]:first[b + 1]] == idx).all()
         assert first[nb] == int((lab >= 0).sum())
         assert (bop[lab < 0] == -1).all()
+
+
+def _group(bop_host, nb):
+    T = G.torch()
+    lib = G.lib()
+    n = bop_host.size
+    gws_b = lib.amh_group_workspace_bytes(n, nb)
+    gws = T.zeros(max(gws_b, 4), dtype=T.uint8, device="cuda")
+    bop = G.to_dev(np.ascontiguousarray(bop_host, np.int32))
+    first = G.zeros(nb + 1, T.int64)
+    plist = G.zeros(n, T.int32)
+    G.check(lib.amh_group_pixels(G.dptr(bop), n, nb, G.dptr(gws), gws_b, G.dptr(first), G.dptr(plist), G.stream()),
+            "group")
+    return G.to_host(first, np.int64), G.to_host(plist, np.int32), gws_b
+
+
+@pytest.mark.parametrize("case", ["checkerboard512", "few_blobs", "many_blobs", "sparse_many", "one_blob"])
+def test_group_pixels_any_blob_count(case):
+    """Stable grouping of pixels by blob for blob counts from 1 to a
+    checkerboard's 131,072 single-pixel blobs: the per-(block, blob) table
+    for few blobs, the radix sort beyond; both give the pixels of each blob
+    in row-major order, and the workspace stays O(pixels + blobs)."""
+    rng = np.random.default_rng(9)
+    if case == "checkerboard512":
+        yy, xx = np.mgrid[0:512, 0:512]
+        on = ((xx + yy) % 2 == 0).ravel()
+        bop = -np.ones(512 * 512, np.int32)
+        bop[on] = np.arange(int(on.sum()))
+        nb = int(on.sum())
+    elif case == "few_blobs":
+        nb = 300
+        bop = rng.integers(-1, nb, 256 * 256).astype(np.int32)
+    elif case == "many_blobs":
+        nb = 5000
+        bop = rng.integers(-1, nb, 256 * 256).astype(np.int32)
+    elif case == "sparse_many":
+        nb = 70000
+        bop = np.where(rng.uniform(size=1024 * 512) < 0.6, rng.integers(0, nb, 1024 * 512), -1).astype(np.int32)
+    else:
+        nb = 1
+        bop = np.where(rng.uniform(size=333 * 77) < 0.5, 0, -1).astype(np.int32)
+    G.log(f"group pixels {case}: {bop.size} pixels, {nb} blobs")
+    first, plist, ws = _group(bop, nb)
+    counts = np.bincount(bop[bop >= 0], minlength=nb)
+    assert np.array_equal(first, np.concatenate([[0], np.cumsum(counts)]))
+    idx = np.flatnonzero(bop >= 0)
+    want = idx[np.argsort(bop[idx], kind="stable")]
+    assert np.array_equal(plist[:idx.size], want)
+    assert ws <= 24 * bop.size + 16 * nb + (1 << 20)
+
+
+def test_engine_detects_checkerboard_alpha():
+    """A dithered (checkerboard) alpha channel: every present pixel is its
+    own blob, 131,072 per 512^2 key frame, through am::morph's detection."""
+    from atomorph_amd import morph as M
+    yy, xx = np.mgrid[0:512, 0:512]
+    frames = []
+    for k in range(2):
+        f = np.zeros((512, 512, 4), np.uint8)
+        on = (xx + yy + k) % 2 == 0
+        f[on] = (200, 40 + 100 * k, 90, 255)
+        frames.append(f)
+    m = M.Morph()
+    m.set(seed=1)
+    for i, f in enumerate(frames):
+        m.add_frame_rgba(i, f)
+    m.set_resolution(512, 512)
+    G.log("checkerboard alpha: detection")
+    assert m.run(1) == M.STATE_BLOB_UNIFICATION, m.error()
+    for k in range(2):
+        assert m.blob_count(k) == 512 * 512 // 2
+        for b in (0, 1, 77777, 131071):
+            s = m.blob(k, b)["surface"]
+            assert len(s) == 1
