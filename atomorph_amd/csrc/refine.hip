@@ -137,6 +137,56 @@ __global__ __launch_bounds__(kRoundThreads) void k_refine_round(uint64_t *__rest
     flush_stats<kRoundThreads>(st, att, acc, gain);
 }
 
+// ---------------------------- plain: a whole visit of one chain, one launch
+// All `rounds` rounds of one visit in one workgroup, for chains of up to
+// kVisitCap slots: row y and the slots' neighbour sums are staged in LDS,
+// the rounds run there (they are sequential: each reads the swaps of the
+// one before, hence a barrier between rounds), and the row is written back
+// once.  The same pairs in the same order as one k_refine_round launch per
+// round.
+constexpr int kVisitThreads = 1024;
+constexpr int kVisitCap = 8192;
+
+__global__ __launch_bounds__(kVisitThreads) void k_refine_visit(uint64_t *__restrict__ pts, int h, int64_t wtot,
+                                                                int64_t off, int32_t w, int32_t y, uint64_t seed,
+                                                                uint64_t stream_id, uint64_t rounds,
+                                                                amh_refine_stats *st) {
+    __shared__ uint64_t C[kVisitCap];
+    __shared__ int2 S[kVisitCap];  // neighbour sums (x, y) of each slot
+    const int yp = y > 0 ? y - 1 : h - 1, yn = (y + 1) % h;
+    uint64_t *row = pts + int64_t(y) * wtot + off;
+    const uint64_t *rprev = pts + int64_t(yp) * wtot + off;
+    const uint64_t *rnext = pts + int64_t(yn) * wtot + off;
+    for (int l = threadIdx.x; l < w; l += kVisitThreads) {
+        const uint64_t p = rprev[l], q = rnext[l];
+        C[l] = row[l];
+        S[l] = make_int2(pt_fx(p) + pt_fx(q), pt_fy(p) + pt_fy(q));
+    }
+    __syncthreads();
+    uint64_t att = 0, acc = 0;
+    int64_t gain = 0;
+    for (uint64_t r = 0; r < rounds; ++r) {
+        const RoundPar rp = round_params(seed, stream_id, r, uint32_t(w));
+        for (uint32_t t = threadIdx.x; t < rp.npairs; t += kVisitThreads) {
+            uint32_t i, j;
+            round_pair(rp, uint32_t(w), t, i, j);
+            const uint64_t ci = C[i], cj = C[j];
+            const int2 si = S[i], sj = S[j];
+            const int64_t d = swap_test(ci, cj, si.x, si.y, sj.x, sj.y);
+            ++att;
+            if (d <= 0) {
+                C[i] = cj;
+                C[j] = ci;
+                ++acc;
+                gain += -2 * d;
+            }
+        }
+        __syncthreads();
+    }
+    for (int l = threadIdx.x; l < w; l += kVisitThreads) row[l] = C[l];
+    flush_stats<kVisitThreads>(st, att, acc, gain);
+}
+
 // ------------------------------------- batched: persistent, LDS resident
 // queue[0] = next item, queue[1] = workgroups finished, queue[2] = pass
 // counter.  The last workgroup to finish resets the first two and advances
@@ -523,6 +573,13 @@ extern "C" int amh_refine_chain(uint64_t *pts_dev, int32_t nkeys, int64_t wtot, 
     // cost; nothing to refine.  Chains of width < 2 have no pairs.
     if (nkeys < 2 || chain_w < 2) return AMH_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (rounds > 1 && chain_w <= kVisitCap) {
+        // the whole visit in one launch
+        amh::launch_timed("k_refine_visit", k_refine_visit, dim3(1), dim3(kVisitThreads), 0, s, pts_dev, int(nkeys),
+                          wtot, chain_off, chain_w, y, seed, stream_id, rounds, stats_dev);
+        AMH_LAUNCH_CHECK("amh_refine_chain");
+        return AMH_OK;
+    }
     for (uint64_t r = 0; r < rounds; ++r) {
         const RoundPar rp = round_params(seed, stream_id, r, uint32_t(chain_w));
         if (rp.npairs == 0) continue;
