@@ -112,6 +112,7 @@ PROTOTYPES = {
     "amh_fluid_step": (C.c_int, [vp, i32, f64, u64, vp]),
     "amh_fluid_render": (C.c_int, [vp, C.POINTER(FluidFrame), f64, vp, vp]),
     "amh_fluid_draw": (C.c_int, [vp, C.POINTER(FluidFrame), f64, vp, vp]),
+    "amh_fluid_use_graphs": (C.c_int, [vp, i32]),
     "amh_mlp_create": (C.c_int, [C.POINTER(vp), i32, vp]),
     "amh_mlp_destroy": (C.c_int, [vp]),
     "amh_mlp_set_params": (C.c_int, [vp, vp, vp]),

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <functional>
 #include <vector>
 
@@ -876,6 +877,20 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
     }
 }
 
+// The values that change from frame to frame, read by the frame's kernels
+// from device memory, so that one captured graph replays every frame.
+struct FrameParams {
+    Frame F;           // time of the frame
+    double radius;     // freedom radius of its steps
+    uint64_t frame;    // frames rendered before (the wall jitter's seed)
+    uint32_t *out;     // the frame's image
+};
+
+__global__ void k_set_params(FrameParams *dst, FrameParams v) { *dst = v; }
+
+// Wall jitter seed of step k of a frame (never 0, which means none).
+__device__ __forceinline__ uint64_t step_seed(uint64_t frame, int k) { return mix64(frame * 131 + uint64_t(k)) | 1u; }
+
 __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
     // replaces rand() / RAND_MAX * 0.01 of the wall bounce (fluidmodel.cpp:554-564);
     // seed 0 bounces without jitter
@@ -957,7 +972,12 @@ __device__ __forceinline__ double2 advect_one(Particles P, int i, const Grid &Gd
 // step's bins (B.count[count_par]), which saves that step's counting pass.
 __global__ __launch_bounds__(kT) void k_advect(Particles P, int n, Grid Gd, int gsx, int gsy, int steps_left,
                                                double freedom_radius, uint64_t seed,
-                                               const int32_t *__restrict__ atom, Bins B, int count_par) {
+                                               const int32_t *__restrict__ atom, Bins B, int count_par,
+                                               const FrameParams *__restrict__ fp, int step_k) {
+    if (fp) {  // a frame's step: radius and jitter seed of the frame
+        freedom_radius = fp->radius;
+        seed = step_seed(fp->frame, step_k);
+    }
     const int i = blockIdx.x * kT + threadIdx.x;
     const bool act = i < n && P.active[i];
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
@@ -1087,6 +1107,17 @@ struct amh_fluid_s {
     int32_t W = 0, H = 0;
     uint64_t frame = 0;        // frames rendered (jitter seed)
     int32_t bin_par = 0;       // parity of the next step's bin counts
+    FrameParams *fp = nullptr; // the current frame's values (device)
+    // Captured frames: the launch sequence of one frame as a hipGraph, keyed
+    // by everything its launches bake in (render arguments, steps, parities).
+    struct Captured {
+        std::vector<unsigned char> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    Captured graphs[4];
+    uint64_t graph_clock = 0;
+    bool use_graphs = true;
 };
 
 namespace {
@@ -1103,7 +1134,7 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
            (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) +
            max_chunks(tiles, n) * 16 + 4 + 3 * 256 +
-           nn * 4 * kGridFields + 6 * 256 + npx * 4 * 2 + 64 * 256;
+           nn * 4 * kGridFields + 6 * 256 + npx * 4 * 2 + 64 * 256 + 256;
 }
 
 void carve_particles(Particles &Q, size_t n, const std::function<char *(size_t)> &take) {
@@ -1176,6 +1207,7 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     for (float **q : g) *q = reinterpret_cast<float *>(take(nn * 4));
     f->mask = reinterpret_cast<int32_t *>(take(npx * 4));
     f->img = reinterpret_cast<uint32_t *>(take(npx * 4));
+    f->fp = reinterpret_cast<FrameParams *>(take(sizeof(FrameParams)));
     // cells ranked in Morton (Z) order of (x, y): a run of consecutive ranks
     // is a compact patch of the grid
     std::vector<int32_t> by_code(nn), rank(nn);
@@ -1211,6 +1243,8 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
 extern "C" int amh_fluid_destroy(void *fluid) {
     amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);
     if (!f) return AMH_OK;
+    for (auto &g : f->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (f->buf) (void)hipFree(f->buf);
     delete f;
     return AMH_OK;
@@ -1290,7 +1324,7 @@ void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int p
 // particles).
 int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32_t steps_left,
                   double freedom_radius, uint64_t seed, hipStream_t s, bool counted = false,
-                  bool count_next = false) {
+                  bool count_next = false, const FrameParams *fp = nullptr, int step_k = 0) {
     amh::ProfScope step_("fluid_step", s);
     if (f->count == 0) return AMH_OK;
     const int tiles = f->bins.ntx * f->bins.nty;
@@ -1316,7 +1350,7 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
                       f->gsy, B);
     amh::launch_timed("k_advect", k_advect, dim3(uint32_t((f->count + kT - 1) / kT)), dim3(kT), 0, s, Q, f->count,
                       f->G, f->gsx, f->gsy, int(steps_left), freedom_radius, seed, atom, B,
-                      count_next ? f->bin_par : -1);
+                      count_next ? f->bin_par : -1, fp, step_k);
     AMH_LAUNCH_CHECK("amh_fluid_step");
     return AMH_OK;
 }
@@ -1363,10 +1397,11 @@ __global__ void k_fluid_reset(Particles P, int n, double x0, double y0) {
 }
 
 // step_fluid's bookkeeping and update_particle (morph.cpp:680-1050).
-__global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, Frame F, Particles P,
+__global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FrameParams *__restrict__ fp, Particles P,
                                                     const int32_t *__restrict__ rank,
                                                     const int32_t *__restrict__ surf,
                                                     const double *__restrict__ bcol, double sp) {
+    const Frame F = fp->F;
     __shared__ int lagp[512], slopep[512];
     if (R.fading == kPerlin) {
         for (int i = threadIdx.x; i < 512; i += kT) {
@@ -1495,10 +1530,11 @@ __device__ __forceinline__ uint32_t splat_one(Particles P, int i, double t, int 
 
 // The splat of every particle, which also counts the pixel-tile bins of
 // its taps into B.count[par] (saving the binning's counting pass).
-__global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, double t, int W, int H, double sp, int bx1,
-                                                  int by1, int bx2, int by2, uint32_t *__restrict__ xy,
-                                                  double *__restrict__ taps, uint32_t *__restrict__ cols, Bins B,
-                                                  int par) {
+__global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, const FrameParams *__restrict__ fp, int W,
+                                                  int H, double sp, int bx1, int by1, int bx2, int by2,
+                                                  uint32_t *__restrict__ xy, double *__restrict__ taps,
+                                                  uint32_t *__restrict__ cols, Bins B, int par) {
+    const double t = fp->F.tl;
     const int i = blockIdx.x * kT + threadIdx.x;
     const uint32_t v = i < n ? splat_one(P, i, t, W, H, sp, bx1, by1, bx2, by2, xy, taps, cols) : kNoTap;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
@@ -1514,8 +1550,9 @@ struct SplatItem {
 
 __global__ __launch_bounds__(kT) void k_splat_tile(Bins B, const uint32_t *__restrict__ xy,
                                                   const double *__restrict__ taps, const uint32_t *__restrict__ cols,
-                                                  int W, int H, double t, int keep_bg, uint32_t *__restrict__ img,
-                                                  int32_t *__restrict__ mask) {
+                                                  int W, int H, const FrameParams *__restrict__ fp, int keep_bg,
+                                                  uint32_t *__restrict__ img, int32_t *__restrict__ mask) {
+    const double t = fp->F.tl;
     __shared__ double sums[5][kPxTile * kPxTile];  // 40 KB
     const int tx = int(blockIdx.x) / B.nty, ty = int(blockIdx.x) % B.nty;
     const int x0 = tx * kPxTile, y0 = ty * kPxTile;
@@ -1594,8 +1631,10 @@ __global__ void k_fluid_commit(int32_t *mask, int64_t npx) {
 }
 
 // background and the final blend (morph.cpp:1264-1298, 1405-1417)
-__global__ __launch_bounds__(kT) void k_fluid_compose(amh_render R, Frame F, const uint32_t *img, const int32_t *mask,
-                                                      uint32_t *out) {
+__global__ __launch_bounds__(kT) void k_fluid_compose(amh_render R, const FrameParams *__restrict__ fp,
+                                                      const uint32_t *img, const int32_t *mask) {
+    const Frame F = fp->F;
+    uint32_t *out = fp->out;
     __shared__ int lagp[512], slopep[512];
     if (R.keep_background && R.fading == kPerlin) {
         for (int i = threadIdx.x; i < 512; i += kT) {
@@ -1725,23 +1764,25 @@ extern "C" int amh_fluid_ranks(const uint64_t *pts_dev, int32_t nkeys, int64_t w
 namespace {
 // draw_fluid after step_fluid (morph.cpp:1082-1209) on the cell-ordered
 // copy f->S: colour relax and splat, resolve into f->img / f->mask, then the
-// particles (colours relaxed, mature set) back to f->P.
-void draw_splat(amh_fluid_s *f, const amh_render &R, const Frame &F, double sparseness, hipStream_t s) {
+// particles (colours relaxed, mature set) back to f->P.  The frame's time
+// comes from f->fp.
+void draw_splat(amh_fluid_s *f, const amh_render &R, double sparseness, hipStream_t s) {
     amh::ProfScope ps_("k_fluid_splat", s);
     const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
     const int ptiles = f->pbins.ntx * f->pbins.nty;
     const int par = f->pbin_par;
     f->pbin_par ^= 1;
-    hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, F.tl, f->W, f->H, sparseness,
+    hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->fp, f->W, f->H, sparseness,
                        R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps, f->scol, f->pbins, par);
     bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true);
     hipLaunchKernelGGL(k_splat_tile, dim3(uint32_t(ptiles)), dim3(kT), 0, s, f->pbins, f->sxy, f->staps, f->scol,
-                       f->W, f->H, F.tl, R.keep_background, f->img, f->mask);
+                       f->W, f->H, f->fp, R.keep_background, f->img, f->mask);
     hipLaunchKernelGGL(k_sort_scatter, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->pos, f->P);
 }
 
-// The outward feather layers and the background blend (morph.cpp:1211-1298).
-void draw_finish(amh_fluid_s *f, const amh_render &R, const Frame &F, uint32_t *out, hipStream_t s) {
+// The outward feather layers and the background blend (morph.cpp:1211-1298)
+// into the frame's image f->fp->out.
+void draw_finish(amh_fluid_s *f, const amh_render &R, hipStream_t s) {
     const int64_t npx = int64_t(f->W) * f->H;
     const uint32_t gpx = uint32_t((npx + kT - 1) / kT);
     for (int layer = 0; layer < R.feather; ++layer) {
@@ -1749,7 +1790,91 @@ void draw_finish(amh_fluid_s *f, const amh_render &R, const Frame &F, uint32_t *
         hipLaunchKernelGGL(k_fluid_feather, dim3(gpx), dim3(kT), 0, s, f->W, f->H, alpha, f->img, f->mask);
         hipLaunchKernelGGL(k_fluid_commit, dim3(gpx), dim3(kT), 0, s, f->mask, npx);
     }
-    hipLaunchKernelGGL(k_fluid_compose, dim3(gpx), dim3(kT), 0, s, R, F, f->img, f->mask, out);
+    hipLaunchKernelGGL(k_fluid_compose, dim3(gpx), dim3(kT), 0, s, R, f->fp, f->img, f->mask);
+}
+
+// One frame's launches after its values are in f->fp: step_fluid's
+// bookkeeping, the model steps, the drawing.  `drawn`: the frame has a key
+// segment and particles (otherwise every pixel is empty).
+int enqueue_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStream_t s) {
+    const amh_render &R = *fa->r;
+    if (drawn) {
+        const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
+        amh::launch_timed("k_fluid_update", k_fluid_update, dim3(gp), dim3(kT), 0, s, R, f->fp, f->P, fa->rank_dev,
+                          fa->surface_dev, fa->blob_col_dev, fa->sparseness);
+        // the steps and the splat run on the cell-ordered copy
+        if (int r = sort_particles(f, s)) return r;
+        for (int k = 0; k < fa->fluidsteps; ++k) {
+            const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), 0.0, 0, s, k > 0,
+                                        k + 1 < fa->fluidsteps, f->fp, k);
+            if (r) return r;
+        }
+        draw_splat(f, R, fa->sparseness, s);
+    } else {
+        const int64_t npx = int64_t(f->W) * f->H;
+        hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((npx + kT - 1) / kT)), dim3(kT), 0, s, f->mask, npx);
+    }
+    draw_finish(f, R, s);
+    AMH_LAUNCH_CHECK("fluid frame");
+    return AMH_OK;
+}
+
+// Everything the launches of enqueue_frame bake into a captured graph.
+std::vector<unsigned char> frame_key(const amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn) {
+    amh_fluid_frame fc = *fa;
+    fc.r = nullptr;
+    const int32_t state[4] = {drawn ? 1 : 0, f->pbin_par, f->bin_par, f->count};
+    std::vector<unsigned char> k(sizeof(amh_render) + sizeof(fc) + sizeof(state));
+    std::memcpy(k.data(), fa->r, sizeof(amh_render));
+    std::memcpy(k.data() + sizeof(amh_render), &fc, sizeof(fc));
+    std::memcpy(k.data() + sizeof(amh_render) + sizeof(fc), state, sizeof(state));
+    return k;
+}
+
+// The frame from its captured graph (captured on first use for its key);
+// false when the stream cannot be captured (then nothing was enqueued).
+bool replay_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStream_t s, int *rc) {
+    *rc = AMH_OK;
+    if (!f->use_graphs || !s || amh::profiling()) return false;
+    const std::vector<unsigned char> key = frame_key(f, fa, drawn);
+    const int32_t pbin_after = f->pbin_par ^ (drawn ? 1 : 0);
+    const int32_t bin_after = f->bin_par ^ (drawn ? (fa->fluidsteps & 1) : 0);
+    amh_fluid_s::Captured *hit = nullptr, *victim = &f->graphs[0];
+    for (auto &g : f->graphs) {
+        if (g.exec && g.key == key) hit = &g;
+        if (!g.exec || g.used < victim->used) victim = &g;
+    }
+    if (!hit) {
+        hipStreamCaptureStatus st;
+        if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return false;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        const int32_t p0 = f->pbin_par, b0 = f->bin_par;
+        const int r = enqueue_frame(f, fa, drawn, s);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        f->pbin_par = p0;  // the replay below advances them
+        f->bin_par = b0;
+        hipGraphExec_t exec = nullptr;
+        if (r || e != hipSuccess || !graph || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            f->use_graphs = false;  // run this handle's frames eagerly from now on
+            return false;
+        }
+        (void)hipGraphDestroy(graph);
+        if (victim->exec) (void)hipGraphExecDestroy(victim->exec);
+        victim->exec = exec;
+        victim->key = key;
+        hit = victim;
+    }
+    hit->used = ++f->graph_clock;
+    *rc = check_hip(hipGraphLaunch(hit->exec, s), "amh_fluid_render graph");
+    f->pbin_par = pbin_after;
+    f->bin_par = bin_after;
+    return true;
 }
 }  // namespace
 
@@ -1781,30 +1906,30 @@ extern "C" int amh_fluid_render(void *fluid, const amh_fluid_frame *fa, double t
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     amh::ProfScope whole_("fluid_render", s);
-    const Frame F = frame_of(t, R.nkeys);
-    const int64_t npx = int64_t(f->W) * f->H;
-    const uint32_t gpx = uint32_t((npx + kT - 1) / kT);
-    if (F.fk < R.nkeys && f->count > 0) {
-        const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
-        amh::launch_timed("k_fluid_update", k_fluid_update, dim3(gp), dim3(kT), 0, s, R, F, f->P, fa->rank_dev,
-                          fa->surface_dev, fa->blob_col_dev, fa->sparseness);
-        // morph.cpp:1050-1054
-        const double freedom = sqrt(double(f->W * f->W + f->H * f->H)) * fa->sparseness / 4.0;
-        const double radius = freedom * (1.0 - F.tl);
-        // the steps and the splat run on the cell-ordered copy
-        if (int r = sort_particles(f, s)) return r;
-        for (int k = 0; k < fa->fluidsteps; ++k) {
-            const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), radius,
-                                        mix64(f->frame * 131 + k) | 1u, s, k > 0, k + 1 < fa->fluidsteps);
-            if (r) return r;
-        }
-        draw_splat(f, R, F, fa->sparseness, s);
-    } else {  // nothing drawn: every pixel empty
-        hipLaunchKernelGGL(k_fill_i32, dim3(gpx), dim3(kT), 0, s, f->mask, npx);
-    }
-    draw_finish(f, R, F, out, s);
+    FrameParams v;
+    v.F = frame_of(t, R.nkeys);
+    // morph.cpp:1050-1054
+    const double freedom = sqrt(double(f->W * f->W + f->H * f->H)) * fa->sparseness / 4.0;
+    v.radius = freedom * (1.0 - v.F.tl);
+    v.frame = f->frame;
+    v.out = out;
+    const bool drawn = v.F.fk < R.nkeys && f->count > 0;
+    hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, s, f->fp, v);
+    int r = AMH_OK;
+    if (!replay_frame(f, fa, drawn, s, &r)) r = enqueue_frame(f, fa, drawn, s);
+    if (r) return r;
     ++f->frame;
     AMH_LAUNCH_CHECK("amh_fluid_render");
+    return AMH_OK;
+}
+
+extern "C" int amh_fluid_use_graphs(void *fluid, int32_t on) {
+    amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);
+    if (!f) {
+        set_error("amh_fluid_use_graphs: NULL");
+        return AMH_E_ARG;
+    }
+    f->use_graphs = on != 0;
     return AMH_OK;
 }
 
@@ -1822,15 +1947,20 @@ extern "C" int amh_fluid_draw(void *fluid, const amh_fluid_frame *fa, double t, 
         return AMH_E_ARG;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const Frame F = frame_of(t, R.nkeys);
-    if (F.fk < R.nkeys && f->count > 0) {
+    FrameParams v;
+    v.F = frame_of(t, R.nkeys);
+    v.radius = 0.0;
+    v.frame = f->frame;
+    v.out = out;
+    hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, s, f->fp, v);
+    if (v.F.fk < R.nkeys && f->count > 0) {
         if (int r = sort_particles(f, s)) return r;
-        draw_splat(f, R, F, fa->sparseness, s);
+        draw_splat(f, R, fa->sparseness, s);
     } else {
         const int64_t npx = int64_t(f->W) * f->H;
         hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((npx + kT - 1) / kT)), dim3(kT), 0, s, f->mask, npx);
     }
-    draw_finish(f, R, F, out, s);
+    draw_finish(f, R, s);
     AMH_LAUNCH_CHECK("amh_fluid_draw");
     return AMH_OK;
 }

@@ -233,6 +233,8 @@ struct morph::engine {
     void *fluid = nullptr;
     double fluid_sp = 0.0;
     bool fluid_inputs = false;
+    bool fluid_stale = false;        // restarted: particles back to a new model's state on next use
+    int32_t fluid_dims[5] = {0, 0, 0, 0, 0};  // gsx, gsy, particles, width, height of the handle
     DevBuf d_frank, d_fsurf, d_fbcol;
     // learned warm start: MLP parameters (caller side), the handle and its
     // workspace (worker side)
@@ -350,10 +352,10 @@ void morph::engine::reset_worker() {
     }
     have_snap = false;
     fluid_inputs = false;
-    if (fluid) {  // a restart deletes the fluid model (morph.cpp:131-132)
-        amh_fluid_destroy(fluid);
-        fluid = nullptr;
-    }
+    // a restart deletes the fluid model (morph.cpp:131-132); the handle is
+    // kept and reset on its next use when the sizes allow, with its
+    // captured frame graphs
+    fluid_stale = fluid != nullptr;
     rng.seed(wopt.seed);
 }
 
@@ -379,14 +381,20 @@ bool morph::engine::prepare_fluid() {
     const size_t nch = cw.size();
     if (!have_snap || nch == 0 || wtot == 0) return false;
     const double sp = opt.fluid_sparseness > 0.0 ? opt.fluid_sparseness : 1.0;
-    if (!fluid || fluid_sp != sp) {
+    const int32_t gsx = int32_t(CW * sp) + 20, gsy = int32_t(CH * sp) + 20;
+    const int32_t dims[5] = {gsx, gsy, int32_t(wtot), CW, CH};
+    const bool same = fluid && std::equal(dims, dims + 5, fluid_dims);
+    if (!same || fluid_sp != sp) {
         if (fluid) amh_fluid_destroy(fluid);
         fluid = nullptr;
-        const int32_t gsx = int32_t(CW * sp) + 20, gsy = int32_t(CH * sp) + 20;
         if (amh_fluid_create(&fluid, gsx, gsy, int32_t(wtot), CW, CH, h, rstream)) return fail("fluid create");
         if (amh_fluid_reset(fluid, sp, rstream)) return fail("fluid reset");
+        std::copy(dims, dims + 5, fluid_dims);
         fluid_sp = sp;
+    } else if (fluid_stale) {  // a new model's particles on the same handle
+        if (amh_fluid_reset(fluid, sp, rstream)) return fail("fluid reset");
     }
+    fluid_stale = false;
     if (fluid_inputs) return true;
     // per key segment and chain: blob size and its AVERAGE / DISTINCT colour
     std::vector<int32_t> surf(size_t(h) * nch);

@@ -338,8 +338,16 @@ typedef struct amh_fluid_frame {
     int32_t pad_;
 } amh_fluid_frame;
 /* One frame at global time t: particle update, fluidsteps model steps,
- * splat, resolve, outward feather, background; out_dev[width*height]. */
+ * splat, resolve, outward feather, background; out_dev[width*height].  On a
+ * stream made by amh_stream_create the frame's launches (about 5 per model
+ * step) are captured once into a hipGraph and replayed for every later
+ * frame with the same arguments; the values that change per frame (time,
+ * freedom radius, jitter seed, out_dev) reach the kernels through a small
+ * parameter block written before each replay.  NULL stream: launched one by
+ * one. */
 int amh_fluid_render(void *fluid, const amh_fluid_frame *frame, double t, uint32_t *out_dev, void *stream);
+/* Graph capture of amh_fluid_render on (1, the default) or off (0). */
+int amh_fluid_use_graphs(void *fluid, int32_t on);
 /* The drawing half of morph::draw_fluid (morph.cpp:1082-1298) alone, on the
  * model's particles as they are (after amh_fluid_set_particles, say): each
  * active particle's colour relaxes towards its target (kept in the model)
