@@ -524,13 +524,19 @@ __device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fet
         }
     };
     if constexpr (!kPrefetch) {
-        if (s0 >= s1) prologue();
+        // one call site of the prologue, after the first entries' loads are
+        // issued (a prologue on two paths has its common part hoisted above
+        // the bin list's load)
+        int slot;
+        bool home;
+        entry(s0, slot, home);
+        auto item = fetch(slot);
+        prologue();
         for (int base = s0; base < s1; base += kT) {
-            int slot;
-            bool home;
-            entry(base, slot, home);
-            const auto item = fetch(slot);
-            if (base == s0) prologue();
+            if (base > s0) {
+                entry(base, slot, home);
+                item = fetch(slot);
+            }
             body(slot, home, item);
         }
         return;
@@ -569,8 +575,10 @@ __device__ __forceinline__ bool block_work(const Bins &B, int nchunks, int b, in
         single = true;
         return true;
     }
+    // loaded before the count is compared, so the two loads overlap (the
+    // grid never exceeds max_chunks; later items clamp their index)
+    const int4 c = B.chunk[min(b, B.max_chunks - 1)];
     if (b >= nchunks) return false;
-    const int4 c = B.chunk[b];
     t = c.x;
     s0 = c.y;
     s1 = c.z;
@@ -598,20 +606,31 @@ __device__ __forceinline__ int halo_index(int i, int j, int x0, int y0) {
     return (i - (x0 - 2)) * kHalo + (j - (y0 - 2));
 }
 
-// stage(f, k) = value of field f at grid index k; F fields into sm.
-template <int F, class Stage>
-__device__ __forceinline__ void stage_halo(float (&sm)[F][kHalo * kHalo], int x0, int y0, int gsx, int gsy,
-                                           const Stage &stage) {
-    for (int e = threadIdx.x; e < kHalo * kHalo; e += kT) {
+// A node halo staged in LDS in two parts: load() issues this thread's loads
+// of R raw fields (thread e < kHalo^2 takes halo entry e, the others repeat
+// the last entry, so no load or LDS write sits in a branch and the loads
+// overlap the bin list's), store() writes F fields computed from them once
+// they arrive.  raw(f, k) = raw field f at grid index k; value(f, r) = staged
+// field f from the raw values r.  Nodes outside the grid stage 0.
+static_assert(kHalo * kHalo <= kT, "one halo entry per thread");
+template <int R>
+struct Halo {
+    float r[R];
+    int e;
+    bool ok;
+    template <class Raw>
+    __device__ __forceinline__ void load(int x0, int y0, int gsx, int gsy, const Raw &raw) {
+        e = min(int(threadIdx.x), kHalo * kHalo - 1);
         const int i = x0 - 2 + e / kHalo, j = y0 - 2 + e % kHalo;
-        if (node_ok(i, j, gsx, gsy)) {
-            const int64_t k = int64_t(i) * gsy + j;
-            for (int f = 0; f < F; ++f) sm[f][e] = stage(f, k);
-        } else {
-            for (int f = 0; f < F; ++f) sm[f][e] = 0.0f;
-        }
+        ok = node_ok(i, j, gsx, gsy);
+        const int64_t k = ok ? int64_t(i) * gsy + j : 0;
+        for (int f = 0; f < R; ++f) r[f] = raw(f, k);
     }
-}
+    template <int F, class Value>
+    __device__ __forceinline__ void store(float (&sm)[F][kHalo * kHalo], const Value &value) const {
+        for (int f = 0; f < F; ++f) sm[f][e] = ok ? value(f, r) : 0.0f;
+    }
+};
 
 struct P2gItem {
     double x, y, s, r, g, b, a;
@@ -630,41 +649,34 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
         if (!block_work(B, nchunks, item, tile, s0, s1, single)) break;
         const int tx = tile / B.nty, ty = tile % B.nty;
         const int x0 = tx * kTile, y0 = ty * kTile;
-        for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
-        __syncthreads();
+        // every field load is unconditional (slot -1, past the end of the
+        // bin, reads particle 0 and is masked), so the eight loads are issued
+        // together, and the sums are cleared while they are in flight
         auto fetch = [&](int slot) {
-            P2gItem it = {0, 0, 0, 0, 0, 0, 0, 0};
-            if (slot >= 0) {
-                it.x = P.x[slot]; it.y = P.y[slot]; it.s = P.strength[slot];
-                it.r = P.r[slot]; it.g = P.g[slot]; it.b = P.b[slot]; it.a = P.a[slot];
-                it.mature = P.mature[slot];
-            }
-            return it;
+            const int q = slot >= 0 ? slot : 0;
+            return P2gItem{P.x[q], P.y[q], P.strength[q], P.r[q], P.g[q], P.b[q], P.a[q], P.mature[q]};
+        };
+        const auto clear = [&] {
+            for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
+            __syncthreads();
         };
         for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool, const P2gItem &it) {
-            int cx = 0, cy = 0;
-            bool valid = false;
+            // branch-free: selects instead of conditional blocks, so no load
+            // is deferred into a branch
+            const int cx = static_cast<int>(it.x - 0.5), cy = static_cast<int>(it.y - 0.5);
+            const bool valid = slot >= 0 && in_reach(cx, cy, x0, y0);
             float c[kCellSums];
-            for (int k = 0; k < kCellSums; ++k) c[k] = 0.0f;
-            if (slot >= 0) {
-                cx = static_cast<int>(it.x - 0.5);
-                cy = static_cast<int>(it.y - 0.5);
-                valid = in_reach(cx, cy, x0, y0);
-                if (valid) {
-                    const double e = it.x - cx - 1.0, h = it.y - cy - 1.0;
-                    const double ep[3] = {1.0, e, e * e}, hp[3] = {1.0, h, h * h};
-                    for (int p = 0; p < 3; ++p)
-                        for (int q = 0; q < 3; ++q) c[p * 3 + q] = float(ep[p] * hp[q]);
-                    if (it.mature && it.s > 0.0) {
-                        // strength-weighted running mean == sum(s c) / sum(s)
-                        c[9] = float(it.s * it.r);
-                        c[10] = float(it.s * it.g);
-                        c[11] = float(it.s * it.b);
-                        c[12] = float(it.s * it.a);
-                        c[13] = float(it.s);
-                    }
-                }
-            }
+            const double e = it.x - cx - 1.0, h = it.y - cy - 1.0;
+            const double ep[3] = {1.0, e, e * e}, hp[3] = {1.0, h, h * h};
+            for (int p = 0; p < 3; ++p)
+                for (int q = 0; q < 3; ++q) c[p * 3 + q] = valid ? float(ep[p] * hp[q]) : 0.0f;
+            // strength-weighted running mean == sum(s c) / sum(s)
+            const bool tint = valid && it.mature && it.s > 0.0;
+            c[9] = tint ? float(it.s * it.r) : 0.0f;
+            c[10] = tint ? float(it.s * it.g) : 0.0f;
+            c[11] = tint ? float(it.s * it.b) : 0.0f;
+            c[12] = tint ? float(it.s * it.a) : 0.0f;
+            c[13] = tint ? float(it.s) : 0.0f;
             const Seg S = segments(valid, cx, cy);
             for (int k = 0; k < kCellSums; ++k) c[k] = seg_sum(c[k], S);
             if (valid && S.tail) {
@@ -672,7 +684,7 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
                 for (int k = 0; k < kCellSums; ++k)
                     if (c[k] != 0.0f) atomicAdd(&cs[k][idx], c[k]);
             }
-        });
+        }, clear);
         __syncthreads();
         // Nodes from the cells: thread (a, node) sums the three cells
         // (node - (a, b)), then one thread per node adds the three partials.
@@ -752,28 +764,28 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
         const int tx = tile / B.nty, ty = tile % B.nty;
         const int x0 = tx * kTile, y0 = ty * kTile;
         for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
+        // the halo's loads are issued before the bin list's and the
+        // particles', which are unconditional (slot -1 reads particle 0)
+        Halo<3> halo;
+        halo.load(x0, y0, gsx, gsy,
+                  [&](int f, int64_t k) { return f == 0 ? Gd.muvw[4 * k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
         const auto stage = [&] {
-            stage_halo(nodes, x0, y0, gsx, gsy,
-                       [&](int f, int64_t k) { return f == 0 ? Gd.muvw[4 * k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
+            halo.store(nodes, [](int f, const float *r) { return r[f]; });
             __syncthreads();
         };
         const auto node = [&](int f, int i, int j) { return nodes[f][halo_index(i, j, x0, y0)]; };
         auto fetch = [&](int slot) {
-            double2 it = {0.0, 0.0};
-            if (slot >= 0) it = double2{P.x[slot], P.y[slot]};
-            return it;
+            const int q = slot >= 0 ? slot : 0;
+            return double2{P.x[q], P.y[q]};
         };
         for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool, const double2 &it) {
-            int cx = 0, cy = 0;
-            bool valid = false;
-            double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3] = {0, 0, 0}, gy[3] = {0, 0, 0};
+            int cx, cy;
+            double px[3], py[3], gx[3], gy[3];
             double pressure = 0.0, fx = 0.0, fy = 0.0;
-            if (slot >= 0) {
-                const double x = it.x, y = it.y;
-                weights(x, y, cx, cy, px, py, gx, gy);
-                valid = in_reach(cx, cy, x0, y0);
-                if (valid) particle_forces(x, y, node, gsx, gsy, pressure, fx, fy);
-            }
+            const double x = it.x, y = it.y;
+            weights(x, y, cx, cy, px, py, gx, gy);
+            const bool valid = slot >= 0 && in_reach(cx, cy, x0, y0);
+            if (valid) particle_forces(x, y, node, gsx, gsy, pressure, fx, fy);  // LDS reads only
             const Seg S = segments(valid, cx, cy);
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) {
@@ -809,59 +821,49 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
         const int tx = tile / B.nty, ty = tile % B.nty;
         const int x0 = tx * kTile, y0 = ty * kTile;
         for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
+        // as in k_forces: halo loads first, unconditional particle loads
+        Halo<3> halo;
+        halo.load(x0, y0, gsx, gsy,
+                  [&](int f, int64_t k) { return f == 0 ? Gd.ax[k] : f == 1 ? Gd.ay[k] : Gd.muvw[4 * k]; });
         const auto stage = [&] {
-            stage_halo(accel, x0, y0, gsx, gsy,
-                       [&](int f, int64_t k) { return per_mass(f == 0 ? Gd.ax[k] : Gd.ay[k], Gd.muvw[4 * k]); });
+            halo.store(accel, [](int f, const float *r) { return per_mass(r[f], r[2]); });
             __syncthreads();
         };
         auto fetch = [&](int slot) {
-            G2pItem it = {0, 0, 0, 0, 0, 0, 0};
-            if (slot >= 0) {
-                it.x = P.x[slot]; it.y = P.y[slot]; it.gx = P.gx[slot]; it.gy = P.gy[slot];
-                it.u = P.u[slot]; it.v = P.v[slot]; it.mature = P.mature[slot];
-            }
-            return it;
+            const int q = slot >= 0 ? slot : 0;
+            return G2pItem{P.x[q], P.y[q], P.gx[q], P.gy[q], P.u[q], P.v[q], P.mature[q]};
         };
         for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool home, const G2pItem &it) {
-            int cx = 0, cy = 0;
-            bool valid = false;
-            double px[3] = {0, 0, 0}, py[3] = {0, 0, 0}, gx[3], gy[3];
-            double mu = 0.0, mv = 0.0;
-            if (slot >= 0) {
-                const double x = it.x, y = it.y;
-                weights(x, y, cx, cy, px, py, gx, gy);
-                valid = in_reach(cx, cy, x0, y0);
-                if (valid || home) {
-                    double cax, cay;  // attractor, 0.03 per step (fluidmodel.cpp:383-410)
-                    toward(x, y, it.gx, it.gy, 0.03, cax, cay);
-                    double u = it.u, v = it.v;
-                    for (int a = 0; a < 3; ++a)
-                        for (int b = 0; b < 3; ++b) {
-                            const double phi = px[a] * py[b];
-                            double ax, ay;
-                            if (valid) {  // in reach: the node is staged
-                                const int e = halo_index(cx + a, cy + b, x0, y0);
-                                ax = accel[0][e];
-                                ay = accel[1][e];
-                            } else {
-                                ax = nd_m(Gd.ax, Gd.muvw, cx + a, cy + b, gsx, gsy);
-                                ay = nd_m(Gd.ay, Gd.muvw, cx + a, cy + b, gsx, gsy);
-                            }
-                            u += phi * (ax + cax);
-                            v += phi * (ay + cay);
-                        }
-                    if (home) {  // other tiles still read the old velocity
-                        P.u2[slot] = u;
-                        P.v2[slot] = v;
+            int cx, cy;
+            double px[3], py[3], gx[3], gy[3];
+            const double x = it.x, y = it.y;
+            weights(x, y, cx, cy, px, py, gx, gy);
+            const bool valid = slot >= 0 && in_reach(cx, cy, x0, y0);
+            double cax, cay;  // attractor, 0.03 per step (fluidmodel.cpp:383-410)
+            toward(x, y, it.gx, it.gy, 0.03, cax, cay);
+            double u = it.u, v = it.v;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double phi = px[a] * py[b];
+                    double ax = 0.0, ay = 0.0;
+                    if (valid) {  // in reach: the node is staged
+                        const int e = halo_index(cx + a, cy + b, x0, y0);
+                        ax = accel[0][e];
+                        ay = accel[1][e];
+                    } else if (home) {
+                        ax = nd_m(Gd.ax, Gd.muvw, cx + a, cy + b, gsx, gsy);
+                        ay = nd_m(Gd.ay, Gd.muvw, cx + a, cy + b, gsx, gsy);
                     }
-                    mu = u;
-                    mv = v;
-                    if (!it.mature) {
-                        mu *= 0.0;
-                        mv *= 0.0;
-                    }
+                    u += phi * (ax + cax);
+                    v += phi * (ay + cay);
                 }
+            if (home) {  // other tiles still read the old velocity
+                P.u2[slot] = u;
+                P.v2[slot] = v;
             }
+            const bool act = valid || home;
+            const double mu = act ? (it.mature ? u : u * 0.0) : 0.0;
+            const double mv = act ? (it.mature ? v : v * 0.0) : 0.0;
             const Seg S = segments(valid, cx, cy);
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) {
@@ -898,63 +900,82 @@ __device__ __forceinline__ double jitter(uint64_t seed, int i, int k) {
     return double(mix64(seed ^ (uint64_t(i) << 3) ^ uint64_t(k)) >> 11) * (1.0 / 9007199254740992.0) * 0.01;
 }
 
+// v where ok, else zeros: selects, not a branch, so the load of v is not
+// waited for before the next loads are issued.
+__device__ __forceinline__ float4 keep_if(bool ok, float4 v) {
+    return float4{ok ? v.x : 0.0f, ok ? v.y : 0.0f, ok ? v.z : 0.0f, ok ? v.w : 0.0f};
+}
+
 // One particle's advection (fluidmodel.cpp:443-564); returns its new
 // position.  atom: the particle held in each slot when P is the cell-ordered
 // copy (NULL: slot i holds particle i); the wall jitter is keyed by particle.
+// Every load is unconditional and issued up front (nodes outside the grid
+// read node 0 and count as zero), so the particle costs two memory round
+// trips -- its fields, then its nine nodes -- instead of one per node.
 __device__ __forceinline__ double2 advect_one(Particles P, int i, const Grid &Gd, int gsx, int gsy, int steps_left,
                                               double freedom_radius, uint64_t seed, const int32_t *__restrict__ atom) {
     const int id = atom ? atom[i] : i;
+    double x = P.x[i], y = P.y[i];
+    const int mature = P.mature[i];
+    const double R0 = P.R[i], G0 = P.G[i], B0 = P.B[i], A0 = P.A[i];
+    const double r0 = P.r[i], g0 = P.g[i], b0 = P.b[i], a0 = P.a[i];
+    const double tgx = P.gx[i], tgy = P.gy[i], fr = P.fr[i], u2 = P.u2[i], v2 = P.v2[i];
     int cx, cy;
     double px[3], py[3], gx[3], gy[3];
-    double x = P.x[i], y = P.y[i];
     weights(x, y, cx, cy, px, py, gx, gy);
+    float4 q[3][3], c[3][3];  // m, u, v, w and r, g, b, a of the stencil nodes
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            const bool ok = node_ok(cx + a, cy + b, gsx, gsy);
+            const int64_t k = ok ? int64_t(cx + a) * gsy + cy + b : 0;
+            q[a][b] = keep_if(ok, reinterpret_cast<const float4 *>(Gd.muvw)[k]);
+            c[a][b] = keep_if(ok, reinterpret_cast<const float4 *>(Gd.col)[k]);
+        }
+    __builtin_amdgcn_sched_barrier(0);  // all 18 node loads are in flight before the first use
     double gu = 0, gv = 0, nR = 0, nG = 0, nB = 0, nA = 0, wsum = 0;
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) {
-            if (!node_ok(cx + a, cy + b, gsx, gsy)) continue;
-            const int64_t k = int64_t(cx + a) * gsy + cy + b;
             const double phi = px[a] * py[b];
-            const float4 q = reinterpret_cast<const float4 *>(Gd.muvw)[k];  // m, u, v, w
-            const float4 c = reinterpret_cast<const float4 *>(Gd.col)[k];   // r, g, b, a
-            gu += phi * per_mass(q.y, q.x);
-            gv += phi * per_mass(q.z, q.x);
-            const double w = q.w;
-            if (w > 0.0) {
-                // node colour = sum(s c)/sum(s); weighted by sum(s)
-                wsum += w;
-                nR += c.x;
-                nG += c.y;
-                nB += c.z;
-                nA += c.w;
-            }
+            gu += phi * per_mass(q[a][b].y, q[a][b].x);
+            gv += phi * per_mass(q[a][b].z, q[a][b].x);
+            // node colour = sum(s c)/sum(s), weighted by sum(s), over the
+            // nodes with w > 0 (selects rather than a branch per node)
+            const double w = q[a][b].w;
+            const bool wpos = w > 0.0;
+            wsum += wpos ? w : 0.0;
+            const float4 cw = keep_if(wpos, c[a][b]);
+            nR += cw.x;
+            nG += cw.y;
+            nB += cw.z;
+            nA += cw.w;
         }
-    if (wsum > 0.0) {  // colour diffusion (fluidmodel.cpp:470-495)
-        nR /= wsum; nG /= wsum; nB /= wsum; nA /= wsum;
-        if (!P.mature[i]) {
-            P.r[i] = nR; P.g[i] = nG; P.b[i] = nB; P.a[i] = nA;
-        } else {
-            const double wr = fabs(nR - P.R[i]), wg = fabs(nG - P.G[i]), wb = fabs(nB - P.B[i]),
-                         wa = fabs(nA - P.A[i]);
-            P.r[i] = (1.0 - wr) * P.r[i] + wr * nR;
-            P.g[i] = (1.0 - wg) * P.g[i] + wg * nG;
-            P.b[i] = (1.0 - wb) * P.b[i] + wb * nB;
-            P.a[i] = (1.0 - wa) * P.a[i] + wa * nA;
-        }
+    {   // colour diffusion (fluidmodel.cpp:470-495) where wsum > 0; computed
+        // and stored unconditionally, so no load waits inside a branch
+        const bool diffuse = wsum > 0.0;
+        const double ws = diffuse ? wsum : 1.0;
+        nR /= ws; nG /= ws; nB /= ws; nA /= ws;
+        const double wr = fabs(nR - R0), wg = fabs(nG - G0), wb = fabs(nB - B0), wa = fabs(nA - A0);
+        const double r1 = mature ? (1.0 - wr) * r0 + wr * nR : nR, g1 = mature ? (1.0 - wg) * g0 + wg * nG : nG;
+        const double b1 = mature ? (1.0 - wb) * b0 + wb * nB : nB, a1 = mature ? (1.0 - wa) * a0 + wa * nA : nA;
+        P.r[i] = diffuse ? r1 : r0;
+        P.g[i] = diffuse ? g1 : g0;
+        P.b[i] = diffuse ? b1 : b0;
+        P.a[i] = diffuse ? a1 : a0;
     }
     x += gu;
     y += gv;
     {   // freedom radius (fluidmodel.cpp:501-539)
-        const double A = fabs(y - P.gy[i]), B = fabs(x - P.gx[i]), Cd = sqrt(A * A + B * B);
-        const double r = freedom_radius * P.fr[i];
+        const double A = fabs(y - tgy), B = fabs(x - tgx), Cd = sqrt(A * A + B * B);
+        const double r = freedom_radius * fr;
         if (Cd > r) {
             double mx, my;
-            toward(x, y, P.gx[i], P.gy[i], Cd - r, mx, my);
+            toward(x, y, tgx, tgy, Cd - r, mx, my);
             const double w = 1.0 / (steps_left + 1);
             x += mx * w;
             y += my * w;
         }
     }
-    double u = P.u2[i], v = P.v2[i];
+    double u = u2, v = v2;
     u += gu - u;
     v += gv - v;
     if (x < 1.0) { x = 1.0 + jitter(seed, id, 0); u = 0.0; }
