@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <vector>
@@ -1179,6 +1180,9 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     }
     amh_fluid_s *f = new amh_fluid_s;
     f->gsx = gsx; f->gsy = gsy; f->count = count; f->W = width; f->H = height;
+    // AMH_FLUID_GRAPHS=0 in the environment starts the handle with graph
+    // capture off (for tools that follow only eager launches)
+    if (const char *g = std::getenv("AMH_FLUID_GRAPHS")) f->use_graphs = std::strcmp(g, "0") != 0;
     const size_t bytes = fluid_bytes(gsx, gsy, count, width, height, nkeys);
     if (int r = check_hip(hipMalloc(&f->buf, bytes), "amh_fluid_create")) {
         delete f;

@@ -346,7 +346,8 @@ typedef struct amh_fluid_frame {
  * parameter block written before each replay.  NULL stream: launched one by
  * one. */
 int amh_fluid_render(void *fluid, const amh_fluid_frame *frame, double t, uint32_t *out_dev, void *stream);
-/* Graph capture of amh_fluid_render on (1, the default) or off (0). */
+/* Graph capture of amh_fluid_render on (1, the default) or off (0).  A handle
+ * starts with it off when the environment holds AMH_FLUID_GRAPHS=0. */
 int amh_fluid_use_graphs(void *fluid, int32_t on);
 /* The drawing half of morph::draw_fluid (morph.cpp:1082-1298) alone, on the
  * model's particles as they are (after amh_fluid_set_particles, say): each
