@@ -337,9 +337,10 @@ struct StencilTiles {  // the step: tiles holding the 3x3 stencil nodes
     int gsx, gsy;
     Grid G;
     __device__ __forceinline__ bool operator()(int i, int &tx0, int &tx1, int &ty0, int &ty1) const {
-        if (!P.active[i]) return false;
+        // the three loads together: the position is read whatever the flag
+        const int act = P.active[i];
         stencil_tiles(P.x[i], P.y[i], gsx, gsy, tx0, tx1, ty0, ty1);
-        return true;
+        return act != 0;
     }
     // Zeroes every field of tile t's nodes (the workgroup together).
     __device__ __forceinline__ void clear_tile(int t, int nty) const {
@@ -434,6 +435,14 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
     __shared__ int sstart[kLds ? kLdsTiles + 1 : 1];
     __shared__ int wsum[kT / 64];
     const int32_t *start = B.start;
+    // the particle's tiles and its entries' ranks, loaded before the counts
+    // are scanned so that the two waits overlap (past n: particle 0, unused)
+    const int i = blockIdx.x * kT + threadIdx.x;
+    const int ic = i < n ? i : 0;
+    int tx0, tx1, ty0, ty1;
+    const bool binned = tiles_of(ic, tx0, tx1, ty0, ty1) && i < n;
+    int rank[4];
+    for (int j = 0; j < 4; ++j) rank[j] = B.rank[int64_t(j) * n + ic];
     if (kLds) {
         const int per = (tiles + kT - 1) / kT;
         const int lo = min(int(threadIdx.x) * per, tiles), hi = min(lo + per, tiles);
@@ -488,12 +497,10 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
         }
     }
     for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) B.count[par ^ 1][k] = 0;
-    const int i = blockIdx.x * kT + threadIdx.x;
-    int tx0, tx1, ty0, ty1;
-    if (i >= n || !tiles_of(i, tx0, tx1, ty0, ty1)) return;
+    if (!binned) return;
     for (int j = 0; j < 4; ++j) {
         const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
-        if (t >= 0) B.list[start[t] + B.rank[int64_t(j) * n + i]] = i | (j == 0 ? kHome : 0);
+        if (t >= 0) B.list[start[t] + rank[j]] = i | (j == 0 ? kHome : 0);
     }
 }
 
@@ -1081,13 +1088,18 @@ __global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, Particle
     const int s = blockIdx.x * kT + threadIdx.x;
     if (s >= n) return;
     const int i = S.atom[s];
-    S.x[s] = P.x[i]; S.y[s] = P.y[i]; S.u[s] = P.u[i]; S.v[s] = P.v[i];
-    S.gx[s] = P.gx[i]; S.gy[s] = P.gy[i]; S.fr[s] = P.fr[i];
-    S.R[s] = P.R[i]; S.G[s] = P.G[i]; S.B[s] = P.B[i]; S.A[s] = P.A[i];
-    S.r[s] = P.r[i]; S.g[s] = P.g[i]; S.b[s] = P.b[i]; S.a[s] = P.a[i];
-    S.strength[s] = P.strength[i];
-    S.active[s] = P.active[i];
-    S.mature[s] = P.mature[i];
+    // every load before the first store (the compiler cannot move a load of
+    // P past a store to S, which it must assume may alias)
+    const double v[16] = {P.x[i], P.y[i], P.u[i], P.v[i], P.gx[i], P.gy[i], P.fr[i], P.R[i],
+                          P.G[i], P.B[i], P.A[i], P.r[i], P.g[i], P.b[i], P.a[i], P.strength[i]};
+    const int32_t act = P.active[i], mat = P.mature[i];
+    S.x[s] = v[0]; S.y[s] = v[1]; S.u[s] = v[2]; S.v[s] = v[3];
+    S.gx[s] = v[4]; S.gy[s] = v[5]; S.fr[s] = v[6];
+    S.R[s] = v[7]; S.G[s] = v[8]; S.B[s] = v[9]; S.A[s] = v[10];
+    S.r[s] = v[11]; S.g[s] = v[12]; S.b[s] = v[13]; S.a[s] = v[14];
+    S.strength[s] = v[15];
+    S.active[s] = act;
+    S.mature[s] = mat;
 }
 
 // What the steps and the splat change goes back to particle order, one
@@ -1097,10 +1109,13 @@ __global__ __launch_bounds__(kT) void k_sort_scatter(Particles S, int n, const i
     const int i = blockIdx.x * kT + threadIdx.x;
     if (i >= n) return;
     const int s = slot[i];
-    if (!S.active[s]) return;
-    P.x[i] = S.x[s]; P.y[i] = S.y[s]; P.u[i] = S.u[s]; P.v[i] = S.v[s];
-    P.r[i] = S.r[s]; P.g[i] = S.g[s]; P.b[i] = S.b[s]; P.a[i] = S.a[s];
-    P.mature[i] = S.mature[s];
+    // loads first (unconditional: an inactive slot is read and not stored)
+    const int32_t act = S.active[s], mat = S.mature[s];
+    const double v[8] = {S.x[s], S.y[s], S.u[s], S.v[s], S.r[s], S.g[s], S.b[s], S.a[s]};
+    if (!act) return;
+    P.x[i] = v[0]; P.y[i] = v[1]; P.u[i] = v[2]; P.v[i] = v[3];
+    P.r[i] = v[4]; P.g[i] = v[5]; P.b[i] = v[6]; P.a[i] = v[7];
+    P.mature[i] = mat;
 }
 
 }  // namespace
@@ -1438,42 +1453,38 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
     const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
     const int c = block_chain(R.chain_off_dev, R.nchains, a);
     if (a >= R.wtot) return;
+    // every load first: the compiler must assume the stores to P below may
+    // alias the other arrays, and would wait for each load in turn
     const int nA = surf[int64_t(F.fk) * R.nchains + c], nB = surf[int64_t(F.fnext) * R.nchains + c];
+    const int rk = rank[int64_t(F.fk) * R.wtot + a];
+    const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a], p2 = R.pts_dev[int64_t(F.fnext) * R.wtot + a];
+    const int act0 = P.active[a], key0 = P.key[a], mat0 = P.mature[a], own0 = P.owner[a];
+    const double r0 = P.r[a], g0 = P.g[a], b0 = P.b[a], a0 = P.a[a];
+    uint64_t tk[4] = {p1, p1, p2, p2};
+    traj_keys(R, F, a, tk);
     const int limit = int(round((1.0 - F.tl) * double(nA) + F.tl * double(nB)));
-    if (rank[int64_t(F.fk) * R.wtot + a] >= limit) {
+    if (rk >= limit) {
         P.active[a] = 0;
         return;
     }
-    if (!(P.active[a] && P.key[a] == F.fk)) {  // a new particle (Particle::clear, fluidmodel.cpp:80-95)
-        P.u[a] = 0.0; P.v[a] = 0.0;
-        P.strength[a] = 1.0;
-        P.mature[a] = 0;
-        P.owner[a] = 1;
-        P.key[a] = F.fk;
-    }
-    const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a], p2 = R.pts_dev[int64_t(F.fnext) * R.wtot + a];
+    // a new particle (Particle::clear, fluidmodel.cpp:80-95)
+    const bool fresh = !(act0 && key0 == F.fk);
+    const int mature = fresh ? 0 : mat0, owner = fresh ? 1 : own0;
     const bool h1 = pt_flags(p1) & HAS_PIXEL, h2 = pt_flags(p2) & HAS_PIXEL;
     const int W = R.width, H = R.height;
-    uint32_t c1, c2;
-    if (h1 && !h2) {
-        c1 = key_px(R.keys_dev, F.fk, W, H, pt_x(p1), pt_y(p1));
-        c2 = nB > 0 ? (c1 & 0x00FFFFFFu) : c1;
-    } else if (h2 && !h1) {
-        c2 = key_px(R.keys_dev, F.fnext, W, H, pt_x(p2), pt_y(p2));
-        c1 = nA > 0 ? (c2 & 0x00FFFFFFu) : c2;
-    } else {
-        c1 = key_px(R.keys_dev, F.fk, W, H, pt_x(p1), pt_y(p1));
-        c2 = key_px(R.keys_dev, F.fnext, W, H, pt_x(p2), pt_y(p2));
-    }
-    const uint32_t col = atom_colour(R, F, p1, c1, c2, lagp, slopep);
+    // both key pixels are read whatever the case (selects, not branches)
+    const uint32_t k1 = key_px(R.keys_dev, F.fk, W, H, pt_x(p1), pt_y(p1));
+    const uint32_t k2 = key_px(R.keys_dev, F.fnext, W, H, pt_x(p2), pt_y(p2));
+    uint32_t c1 = k1, c2 = k2;
+    if (h1 && !h2) c2 = nB > 0 ? (k1 & 0x00FFFFFFu) : k1;
+    else if (h2 && !h1) c1 = nA > 0 ? (k2 & 0x00FFFFFFu) : k2;
     uint32_t qx, qy, qxf, qyf;
-    traj_point(R, F, a, p1, p2, qx, qy, qxf, qyf);
+    traj_point(R, F, a, p1, p2, qx, qy, qxf, qyf, tk);
+    const uint32_t col = atom_colour(R, F, p1, c1, c2, lagp, slopep);
     // point2xy works in float (atomorph.h:316-319)
     const float ptx = float(qx) + float(qxf) / float(256), pty = float(qy) + float(qyf) / float(256);
     const double gxv = fmin(fmax((double(ptx) * sp) + 10.0, 1.0), double(W * sp + 10.0));
     const double gyv = fmin(fmax((double(pty) * sp) + 10.0, 1.0), double(H * sp + 10.0));
-    P.gx[a] = gxv;
-    P.gy[a] = gyv;
     double Rr = ch(col, 0) / 255.0, Gg = ch(col, 1) / 255.0, Bb = ch(col, 2) / 255.0, Aa = ch(col, 3) / 255.0;
     if (R.show_blobs == 2 && bcol) {  // DISTINCT
         const double *q = bcol + (int64_t(F.fk) * R.nchains + c) * 4;
@@ -1482,12 +1493,20 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
         const double *q = bcol + (int64_t(F.fk) * R.nchains + c) * 4;
         Rr = q[0]; Gg = q[1]; Bb = q[2]; Aa = q[3];
     }
-    P.R[a] = Rr; P.G[a] = Gg; P.B[a] = Bb; P.A[a] = Aa;
     const double t = F.tl;
-    double r = t * Rr + (1.0 - t) * P.r[a], g = t * Gg + (1.0 - t) * P.g[a];
-    double b = t * Bb + (1.0 - t) * P.b[a], al = t * Aa + (1.0 - t) * P.a[a];
-    if (!P.mature[a]) {
-        if (P.owner[a]) {
+    double r = t * Rr + (1.0 - t) * r0, g = t * Gg + (1.0 - t) * g0;
+    double b = t * Bb + (1.0 - t) * b0, al = t * Aa + (1.0 - t) * a0;
+    if (fresh) {
+        P.u[a] = 0.0; P.v[a] = 0.0;
+        P.mature[a] = 0;
+        P.owner[a] = 1;
+        P.key[a] = F.fk;
+    }
+    P.gx[a] = gxv;
+    P.gy[a] = gyv;
+    P.R[a] = Rr; P.G[a] = Gg; P.B[a] = Bb; P.A[a] = Aa;
+    if (!mature) {  // (a fresh particle's strength 1.0 is replaced here)
+        if (owner) {
             P.x[a] = gxv;
             P.y[a] = gyv;
             r = Rr; g = Gg; b = Bb; al = Aa;
@@ -1511,17 +1530,20 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
 __device__ __forceinline__ uint32_t splat_one(Particles P, int i, double t, int W, int H, double sp, int bx1, int by1,
                                               int bx2, int by2, uint32_t *__restrict__ xy,
                                               double *__restrict__ taps, uint32_t *__restrict__ cols) {
-    if (!P.active[i]) {
+    // every load before the first store (see k_fluid_update)
+    const int act = P.active[i];
+    const double s = P.strength[i], x = P.x[i], y = P.y[i];
+    const double R0 = P.R[i], G0 = P.G[i], B0 = P.B[i], A0 = P.A[i];
+    const double r0 = P.r[i], g0 = P.g[i], b0 = P.b[i], a0 = P.a[i];
+    if (!act) {
         xy[i] = kNoTap;
         return kNoTap;
     }
-    const double s = P.strength[i];
     const double w = (1.0 / (100.0 * pow((t - (1.0 * (1.0 - s))), 2.0) + 1.0)) * pow(s, 2.0);
-    const double r = (w * P.R[i]) + (1.0 - w) * P.r[i], g = (w * P.G[i]) + (1.0 - w) * P.g[i];
-    const double b = (w * P.B[i]) + (1.0 - w) * P.b[i], al = (w * P.A[i]) + (1.0 - w) * P.a[i];
+    const double r = (w * R0) + (1.0 - w) * r0, g = (w * G0) + (1.0 - w) * g0;
+    const double b = (w * B0) + (1.0 - w) * b0, al = (w * A0) + (1.0 - w) * a0;
     P.r[i] = r; P.g[i] = g; P.b[i] = b; P.a[i] = al;
     double px, py;
-    const double x = P.x[i], y = P.y[i];
     if (x <= 10.0) px = 0.0;
     else if (x >= W * sp + 10.0) px = (W - 1);
     else px = (x - 10.0) / sp;

@@ -87,9 +87,11 @@ __device__ uint32_t fade_colour(uint32_t c1, uint32_t c2, double px, double py, 
 }
 
 __device__ __forceinline__ uint32_t key_px(const uint32_t *keys, int f, int W, int H, uint32_t x, uint32_t y) {
-    if (x >= uint32_t(W) || y >= uint32_t(H)) return 0u;
-    const uint32_t c = keys[int64_t(f) * W * H + int64_t(y) * W + x];
-    return (c >> 24) ? c : 0u;
+    // branch-free: a pixel outside the frame reads the frame's first pixel
+    // and returns 0, so the load is never waited for inside a branch
+    const bool in = x < uint32_t(W) && y < uint32_t(H);
+    const uint32_t c = keys[int64_t(f) * W * H + (in ? int64_t(y) * W + x : 0)];
+    return (in && (c >> 24)) ? c : 0u;
 }
 
 
@@ -111,8 +113,21 @@ __device__ uint32_t background(const amh_render &R, const Frame &F, int x, int y
 // point coordinates (truncating), SPLINE evaluates the closed Catmull-Rom
 // through the atom's key positions (spline.cpp:29-57) and rounds the
 // fraction to /255.
+// The key points a SPLINE trajectory reads (the four around the frame's
+// segment); other motions read none.  Loaded apart from traj_point so that a
+// caller can issue them with its other loads.
+__device__ __forceinline__ void traj_keys(const amh_render &R, const Frame &F, int64_t a, uint64_t q[4]) {
+    if (R.motion != kSpline) return;
+    const int h = R.nkeys;
+    const double dt = 1.0 / h;
+    const int pk = (int)(F.tn / dt);
+    auto wrap = [h](int i) { return i < 0 ? h - 1 : (i >= h ? i - h : i); };
+    for (int k = 0; k < 4; ++k) q[k] = R.pts_dev[int64_t(wrap(pk - 1 + k)) * R.wtot + a];
+}
+
 __device__ inline void traj_point(const amh_render &R, const Frame &F, int64_t a, uint64_t p1, uint64_t p2,
-                                  uint32_t &qx, uint32_t &qy, uint32_t &qxf, uint32_t &qyf) {
+                                  uint32_t &qx, uint32_t &qy, uint32_t &qxf, uint32_t &qyf,
+                                  const uint64_t *keys = nullptr) {
     const int h = R.nkeys;
     qx = pt_x(p1); qy = pt_y(p1); qxf = pt_xf(p1); qyf = pt_yf(p1);
     if (R.motion == kLinear) {
@@ -127,16 +142,19 @@ __device__ inline void traj_point(const amh_render &R, const Frame &F, int64_t a
     } else if (R.motion == kSpline) {
         const double dt = 1.0 / h;
         const int pk = (int)(F.tn / dt);
-        auto wrap = [h](int i) { return i < 0 ? h - 1 : (i >= h ? i - h : i); };
-        const int i0 = wrap(pk - 1), i1 = wrap(pk), i2 = wrap(pk + 1), i3 = wrap(pk + 2);
         const double lt = (F.tn - dt * (double)pk) / dt;
         const double t2 = lt * lt, t3 = t2 * lt;
         const double b1 = 0.5 * (-t3 + 2.0 * t2 - lt);
         const double b2 = 0.5 * (3.0 * t3 - 5.0 * t2 + 2.0);
         const double b3 = 0.5 * (-3.0 * t3 + 4.0 * t2 + lt);
         const double b4 = 0.5 * (t3 - t2);
-        const uint64_t q0 = R.pts_dev[int64_t(i0) * R.wtot + a], q1 = R.pts_dev[int64_t(i1) * R.wtot + a];
-        const uint64_t q2 = R.pts_dev[int64_t(i2) * R.wtot + a], q3 = R.pts_dev[int64_t(i3) * R.wtot + a];
+        uint64_t q[4];
+        if (keys) {
+            for (int k = 0; k < 4; ++k) q[k] = keys[k];
+        } else {
+            traj_keys(R, F, a, q);
+        }
+        const uint64_t q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
         auto cx = [](uint64_t q) { return pt_x(q) + pt_xf(q) / double(256); };
         auto cy = [](uint64_t q) { return pt_y(q) + pt_yf(q) / double(256); };
         const double sx = cx(q0) * b1 + cx(q1) * b2 + cx(q2) * b3 + cx(q3) * b4;
@@ -173,17 +191,26 @@ __device__ __forceinline__ int find_chain(const int64_t *off, int nchains, int64
     return lo;
 }
 
-// Chain of atom a for a workgroup of consecutive atoms starting at a0 =
-// blockIdx.x * blockDim.x: one binary search per workgroup (for a0), then
-// each thread walks forward over the few chains that start inside the
-// workgroup -- instead of a dependent chain of ~log2(chains) global loads
-// per thread.  Every thread of the workgroup calls it.
+// Chain of atom a (the last c with off[c] <= a; off ascending from 0) for a
+// workgroup of consecutive atoms starting at a0 = blockIdx.x * blockDim.x.
+// The chain of a0 is found by a blockDim-way search: each round every
+// thread probes one offset and __syncthreads_count of the probes at or
+// below a0 narrows the range blockDim-fold -- one round, one load per thread
+// for up to blockDim chains, instead of a chain of log2(chains) dependent
+// loads by one thread.  Then each thread walks forward over the few chains
+// that start inside the workgroup.  Every thread of the workgroup calls it.
 __device__ __forceinline__ int block_chain(const int64_t *off, int nchains, int64_t a) {
-    __shared__ int c0;
-    __syncthreads();  // c0 may still be read from a previous call
-    if (threadIdx.x == 0) c0 = find_chain(off, nchains, int64_t(blockIdx.x) * blockDim.x);
-    __syncthreads();
-    int c = c0;
+    const int64_t a0 = int64_t(blockIdx.x) * blockDim.x;
+    const int nt = int(blockDim.x), t = int(threadIdx.x);
+    int lo = 0, n = nchains;  // the chain of a0 is in [lo, lo + n); off[lo] <= a0
+    while (n > 1) {
+        const int stride = (n + nt - 1) / nt;
+        const bool le = t * stride < n && off[lo + t * stride] <= a0;
+        const int below = __syncthreads_count(le);  // probes at or below a0: a prefix, >= 1
+        lo += (below - 1) * stride;
+        n = min(stride, n - (below - 1) * stride);
+    }
+    int c = lo;
     while (c + 1 < nchains && off[c + 1] <= a) ++c;
     return c;
 }
