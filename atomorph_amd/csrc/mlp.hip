@@ -39,16 +39,29 @@ __host__ __device__ constexpr size_t w_off(int l) {
 __host__ __device__ constexpr size_t b_off(int l) { return w_off(l) + size_t(kDims[l]) * kDims[l + 1]; }
 static_assert(b_off(4) + kOut == kParams, "parameter layout");
 
-// FP32 -> bfloat16 bits, round to nearest even (NaN/Inf kept)
-__device__ __forceinline__ uint16_t to_bf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) != 0x7F800000u) u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
+// FP32 -> bfloat16 bits, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions (1964),
+// formula 7.1.26: for x >= 0, erf x = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 +
+// a5 t^5) exp(-x^2) with t = 1 / (1 + p x), |error| <= 1.5e-7; odd in x.
+// About a third of the instructions of erff, and the GELUs (4 x 128 per
+// row) are what bounds the fused forward; the activations are rounded to
+// bfloat16 (2^-8 relative) right after.
+__device__ __forceinline__ float erf_as(float x) {
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.0f));
+    float poly = fmaf(t, 1.061405429f, -1.453152027f);
+    poly = fmaf(t, poly, 1.421413741f);
+    poly = fmaf(t, poly, -0.284496736f);
+    poly = fmaf(t, poly, 0.254829592f);
+    poly *= t;
+    const float y = fmaf(-poly, __expf(-ax * ax), 1.0f);
+    return copysignf(y, x);
 }
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_d(float z) {
-    const float cdf = 0.5f * (1.0f + erff(z * 0.70710678118654752f));
-    const float pdf = expf(-0.5f * z * z) * 0.39894228040143268f;
+    const float cdf = 0.5f * (1.0f + erf_as(z * 0.70710678118654752f));
+    const float pdf = __expf(-0.5f * z * z) * 0.39894228040143268f;
     return cdf + z * pdf;
 }
 
@@ -260,7 +273,7 @@ __global__ __launch_bounds__(256) void k_fused_stage(const uint16_t *__restrict_
     if (t < kOut) S->b5[t] = p[b_off(4) + t];
 }
 
-constexpr int kFusedThreads = 512;  // 8 waves (2 per SIMD, 256 registers each) share one copy of the weights
+constexpr int kFusedThreads = 1024;  // 16 waves (4 per SIMD, <= 128 registers each) share one copy of the weights
 
 // Bias + GELU of one 32-feature output block (accumulator layout: feature
 // 8*(r/4) + 4*h + r%4 of block mb) -> the two B operands it feeds.

@@ -285,6 +285,31 @@ def cpu_baseline(frames, gpu_attempts, match_steps, fluid_steps):
                                  "rendering are single-threaded in the reference"}}
 
 
+def counter_calibration():
+    """Kernels of known HBM traffic for calibrating FETCH_SIZE / WRITE_SIZE
+    and the read-request-size counters: a read-only reduction of 1 GiB,
+    reads of one 4-byte word per 128 B and per 256 B line (each line read
+    once), and a 256 MiB device copy.  Under a counter profiler each shows as
+    its own dispatch."""
+    import torch
+    torch.cuda.set_device(0)
+    gib = 1 << 30
+    x = torch.ones(gib // 4, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    for what, fn in (("read-only sum of 1 GiB", lambda: x.sum()),
+                     ("one word per 128 B line, 1 GiB span", lambda: x.view(-1, 32)[:, 0].sum()),
+                     ("one word per 256 B line, 1 GiB span", lambda: x.view(-1, 64)[:, 0].sum())):
+        log(what)
+        fn()
+        torch.cuda.synchronize()
+    a = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    log("device copy of 256 MiB")
+    b.copy_(a)
+    torch.cuda.synchronize()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,11 +324,16 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-time-to-cost", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="job without the learned warm start")
+    ap.add_argument("--counter-calibration", action="store_true",
+                    help="only run reads and a copy of known byte counts (for calibrating HBM counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); this build runs the morph engine on a "
               f"single MI355X. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.counter_calibration:
+        counter_calibration()
+        return
 
     import numpy as np
     import torch
