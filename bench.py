@@ -294,7 +294,7 @@ def counter_calibration():
     import torch
     torch.cuda.set_device(0)
     gib = 1 << 30
-    x = torch.ones(gib // 4, dtype=torch.int32, device="cuda")
+    x = torch.ones(gib // 4, dtype=torch.float32, device="cuda")  # float: the sums read it as it is
     torch.cuda.synchronize()
     for what, fn in (("read-only sum of 1 GiB", lambda: x.sum()),
                      ("one word per 128 B line, 1 GiB span", lambda: x.view(-1, 32)[:, 0].sum()),
