@@ -104,6 +104,9 @@ PROTOTYPES = {
     "amh_render_frame": (C.c_int, [C.POINTER(RenderArgs), f64, vp, sz, vp, vp]),
     "amh_render_noalloc_max_bytes": (sz, []),
     "amh_render_frame_noalloc": (C.c_int, [C.POINTER(RenderArgs), f64, vp, vp]),
+    "amh_render_graph_create": (C.c_int, [C.POINTER(vp), C.POINTER(RenderArgs), vp, sz, vp]),
+    "amh_render_graph_frame": (C.c_int, [vp, f64, vp, vp]),
+    "amh_render_graph_destroy": (C.c_int, [vp]),
     "amh_fluid_create": (C.c_int, [C.POINTER(vp), i32, i32, i32, i32, i32, i32, vp]),
     "amh_fluid_destroy": (C.c_int, [vp]),
     "amh_fluid_reset": (C.c_int, [vp, f64, vp]),

@@ -168,11 +168,11 @@ void prof_end(const char *kernel, hipStream_t s);
 struct ProfScope {
     const char *k;
     hipStream_t s;
-    ProfScope(const char *kernel, hipStream_t st) : k(kernel), s(st) {
-        if (profiling()) prof_begin(k, s);
+    ProfScope(const char *kernel, hipStream_t st) : k(kernel), s(st) {  // kernel NULL: not timed
+        if (k && profiling()) prof_begin(k, s);
     }
     ~ProfScope() {
-        if (profiling()) prof_end(k, s);
+        if (k && profiling()) prof_end(k, s);
     }
 };
 // An event pair for one timed launch of `kernel` (false: none available),

@@ -39,7 +39,19 @@ struct Rec {       // one atom's splat, 16 bytes
     int32_t blob;  // composite index
 };
 
-__global__ __launch_bounds__(256) void k_prepare(amh_render R, Frame F, Rec *__restrict__ recs, int *__restrict__ bmin) {
+// What changes from frame to frame when frames replay from a captured
+// hipGraph: the kernels read it from device memory instead of their
+// arguments.
+struct RenderParams {
+    Frame F;
+    uint32_t *out;
+};
+
+__global__ void k_set_render_params(RenderParams *dst, RenderParams v) { *dst = v; }
+
+__global__ __launch_bounds__(256) void k_prepare(amh_render R, Frame F, Rec *__restrict__ recs, int *__restrict__ bmin,
+                                                 const RenderParams *__restrict__ fp) {
+    if (fp) F = fp->F;
     __shared__ int lagp[512], slopep[512];
     if (R.fading == kPerlin) {
         for (int i = threadIdx.x; i < 512; i += 256) {
@@ -262,7 +274,12 @@ struct Fold {
 
 __global__ __launch_bounds__(256) void k_composite(amh_render R, Frame F, int *__restrict__ bmin, Acc acc,
                                                    const int *__restrict__ ovstart, const Ov *__restrict__ ov,
-                                                   uint32_t *__restrict__ out, int clear_bmin) {
+                                                   uint32_t *__restrict__ out, int clear_bmin,
+                                                   const RenderParams *__restrict__ fp) {
+    if (fp) {
+        F = fp->F;
+        out = fp->out;
+    }
     __shared__ int lagp[512], slopep[512];
     if (R.keep_background && R.fading == kPerlin) {
         for (int i = threadIdx.x; i < 512; i += 256) {
@@ -418,7 +435,10 @@ bool render_args_ok(const amh_render *R) {
 
 // The frame itself, on a workspace laid out by `L` whose accumulators,
 // overflow counts and smallest-blob map are clean; leaves them clean.
-int render_frame_on(const amh_render *R, double t, char *base, const Layout &L, uint32_t *out, hipStream_t s);
+// fp (device, may be NULL): the frame's time and output are read from it
+// (graph capture; then nothing is timed, since events would be captured).
+int render_frame_on(const amh_render *R, double t, char *base, const Layout &L, uint32_t *out, hipStream_t s,
+                    const RenderParams *fp = nullptr);
 
 }  // namespace
 
@@ -512,8 +532,9 @@ extern "C" int amh_render_frame_noalloc(const amh_render *R, double t, uint32_t 
 }
 
 namespace {
-int render_frame_on(const amh_render *R, double t, char *base, const Layout &L, uint32_t *out, hipStream_t s) {
-    amh::ProfScope whole_("render_frame", s);
+int render_frame_on(const amh_render *R, double t, char *base, const Layout &L, uint32_t *out, hipStream_t s,
+                    const RenderParams *fp) {
+    amh::ProfScope whole_(fp ? nullptr : "render_frame", s);
     int *bmin = reinterpret_cast<int *>(base + L.bmin);
     int *ovcount = reinterpret_cast<int *>(base + L.ovcount);
     int *ovstart = reinterpret_cast<int *>(base + L.ovstart);
@@ -541,20 +562,108 @@ int render_frame_on(const amh_render *R, double t, char *base, const Layout &L, 
     const uint32_t gpx = uint32_t((L.npx + 255) / 256);
     const uint32_t gat = uint32_t((Rc.wtot + 255) / 256);
     if (Rc.wtot > 0) {
-        amh::launch_timed("k_prepare", k_prepare, dim3(gat), dim3(256), 0, s, Rc, F, recs, bmin);
-        amh::launch_timed("k_splat", k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc,
-                          ovcount);
+        if (fp) {
+            hipLaunchKernelGGL(k_prepare, dim3(gat), dim3(256), 0, s, Rc, F, recs, bmin, fp);
+            hipLaunchKernelGGL(k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc, ovcount);
+        } else {
+            amh::launch_timed("k_prepare", k_prepare, dim3(gat), dim3(256), 0, s, Rc, F, recs, bmin, fp);
+            amh::launch_timed("k_splat", k_splat, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, acc,
+                              ovcount);
+        }
     }
     scan::exclusive(ovcount, L.npx, sums, offs, ovstart, s);
     if (Rc.wtot > 0)
         hipLaunchKernelGGL(k_overflow, dim3(gat), dim3(256), 0, s, recs, Rc.wtot, Rc.width, bmin, ovcount, ovstart,
                            ov);
     const int separate_clear = Rc.feather > 0 ? 1 : 0;
-    amh::launch_timed("k_composite", k_composite, dim3(gpx), dim3(256), 0, s, Rc, F, bmin, acc, ovstart, ov, out,
-                      separate_clear ? 0 : 1);
+    if (fp)
+        hipLaunchKernelGGL(k_composite, dim3(gpx), dim3(256), 0, s, Rc, F, bmin, acc, ovstart, ov, out,
+                           separate_clear ? 0 : 1, fp);
+    else
+        amh::launch_timed("k_composite", k_composite, dim3(gpx), dim3(256), 0, s, Rc, F, bmin, acc, ovstart, ov,
+                          out, separate_clear ? 0 : 1, fp);
     if (separate_clear)
         hipLaunchKernelGGL(k_fill_i32, dim3(gpx), dim3(256), 0, s, bmin, L.npx, 0x7FFFFFFF);
     AMH_LAUNCH_CHECK("amh_render_frame");
     return AMH_OK;
 }
 }  // namespace
+
+// --------------------------------------------------- frames from a graph
+// One frame's launches captured once and replayed for every later frame of
+// the same chains and options; only the frame's time and output, in a small
+// device block, change between replays.
+struct amh_render_graph_s {
+    amh_render R;
+    char *ws = nullptr;
+    Layout L;
+    RenderParams *fp = nullptr;  // device
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int amh_render_graph_create(void **graph, const amh_render *R, void *ws, size_t ws_bytes, void *stream) {
+    if (!graph || !render_args_ok(R) || !ws || !stream) {
+        set_error("amh_render_graph_create: invalid argument (a created stream is required)");
+        return AMH_E_ARG;
+    }
+    *graph = nullptr;
+    const Layout L = layout(R->width, R->height, R->wtot);
+    if (ws_bytes < L.total) {
+        set_error("amh_render_graph_create: workspace %zu < %zu bytes", ws_bytes, L.total);
+        return AMH_E_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    amh_render_graph_s *g = new amh_render_graph_s;
+    g->R = *R;
+    g->ws = static_cast<char *>(ws);
+    g->L = L;
+    if (int r = check_hip(hipMalloc(&g->fp, sizeof(RenderParams)), "amh_render_graph_create")) {
+        delete g;
+        return r;
+    }
+    int r = check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (!r) {
+        // time 0 only fixes which launches the graph holds (a frame with
+        // atoms); every replay reads its own frame from g->fp
+        const int rr = render_frame_on(&g->R, 0.0, g->ws, g->L, nullptr, s, g->fp);
+        hipGraph_t gr = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &gr);
+        r = rr ? rr : check_hip(e, "hipStreamEndCapture");
+        if (!r) r = check_hip(hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0), "hipGraphInstantiate");
+        if (gr) (void)hipGraphDestroy(gr);
+    }
+    if (r) {
+        (void)hipFree(g->fp);
+        delete g;
+        return r;
+    }
+    *graph = g;
+    return AMH_OK;
+}
+
+extern "C" int amh_render_graph_frame(void *graph, double t, uint32_t *out, void *stream) {
+    amh_render_graph_s *g = static_cast<amh_render_graph_s *>(graph);
+    if (!g || !out) {
+        set_error("amh_render_graph_frame: invalid argument");
+        return AMH_E_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    RenderParams v;
+    v.F = frame_of(t, g->R.nkeys);
+    v.out = out;
+    // a frame with nothing to draw (no atoms, or a time past the keys)
+    // launches fewer kernels than the graph holds: it runs eagerly
+    if (g->R.wtot == 0 || v.F.fk >= g->R.nkeys) return render_frame_on(&g->R, t, g->ws, g->L, out, s);
+    hipLaunchKernelGGL(k_set_render_params, dim3(1), dim3(1), 0, s, g->fp, v);
+    if (int r = check_hip(hipGetLastError(), "amh_render_graph_frame")) return r;
+    return check_hip(hipGraphLaunch(g->exec, s), "amh_render_graph_frame");
+}
+
+extern "C" int amh_render_graph_destroy(void *graph) {
+    amh_render_graph_s *g = static_cast<amh_render_graph_s *>(graph);
+    if (!g) return AMH_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    (void)hipFree(g->fp);
+    delete g;
+    return AMH_OK;
+}

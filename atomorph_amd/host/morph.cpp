@@ -218,6 +218,10 @@ struct morph::engine {
     int32_t nitems = 0, items_tile = -1;
     uint64_t visits = 0, passes = 0;
     void *graph = nullptr;
+    // atom-path frames replayed from a captured graph, and what it was
+    // captured for (the render arguments and the workspace)
+    void *rgraph = nullptr;
+    std::vector<unsigned char> rgraph_key;
     int graph_tile = -1;
     unsigned graph_np = 0, graph_rounds = 0;
     // autotuned batched schedule for the current chains (-1: not yet)
@@ -306,6 +310,7 @@ morph::engine::~engine() {
     cv.notify_all();
     if (th.joinable()) th.join();
     if (graph) amh_graph_destroy(graph);
+    if (rgraph) amh_render_graph_destroy(rgraph);
     if (fluid) amh_fluid_destroy(fluid);
     if (mlp) amh_mlp_destroy(mlp);
     for (auto &v : out_blobs)
@@ -349,6 +354,11 @@ void morph::engine::reset_worker() {
     if (graph) {
         amh_graph_destroy(graph);
         graph = nullptr;
+    }
+    if (rgraph) {
+        amh_render_graph_destroy(rgraph);
+        rgraph = nullptr;
+        rgraph_key.clear();
     }
     have_snap = false;
     fluid_inputs = false;
@@ -1449,13 +1459,33 @@ bool morph::render_device(const double *times, size_t n, void *out_dev, float *g
     e.fill_render(r);
     const bool fluid = e.opt.fluidsteps > 0;
     if (fluid && !e.prepare_fluid()) return false;
+    if (!fluid && n > 1) {
+        // several atom-path frames: one captured frame replayed for each
+        // (kept while the render arguments and the workspace stay the same)
+        std::vector<unsigned char> key(sizeof(r) + sizeof(void *) + sizeof(size_t));
+        std::memcpy(key.data(), &r, sizeof(r));  // fill_render zeroes the padding
+        std::memcpy(key.data() + sizeof(r), &e.d_rws.p, sizeof(void *));
+        std::memcpy(key.data() + sizeof(r) + sizeof(void *), &e.d_rws.n, sizeof(size_t));
+        if (e.rgraph && key != e.rgraph_key) {
+            amh_render_graph_destroy(e.rgraph);
+            e.rgraph = nullptr;
+        }
+        if (!e.rgraph) {
+            if (amh_render_graph_create(&e.rgraph, &r, e.d_rws.p, e.d_rws.n, e.rstream) == AMH_OK)
+                e.rgraph_key = key;
+            else
+                e.rgraph = nullptr;  // frames run eagerly
+        }
+    }
     void *timer = nullptr;
     if (amh_timer_begin(&timer, e.rstream)) return e.fail("timer");
     const size_t npx = size_t(e.CW) * e.CH;
     for (size_t i = 0; i < n; ++i) {
         uint32_t *dst = static_cast<uint32_t *>(out_dev) + i * npx;
-        const bool ok = fluid ? e.render_fluid(times[i], dst)
-                              : amh_render_frame(&r, times[i], e.d_rws.p, e.d_rws.n, dst, e.rstream) == AMH_OK;
+        bool ok;
+        if (fluid) ok = e.render_fluid(times[i], dst);
+        else if (n > 1 && e.rgraph) ok = amh_render_graph_frame(e.rgraph, times[i], dst, e.rstream) == AMH_OK;
+        else ok = amh_render_frame(&r, times[i], e.d_rws.p, e.d_rws.n, dst, e.rstream) == AMH_OK;
         if (!ok) {
             amh_timer_free(timer);
             return fluid ? false : e.fail("render");

@@ -293,6 +293,18 @@ int amh_render_frame(const amh_render *r, double t, void *ws_dev, size_t ws_byte
  * streams, two threads) race. */
 size_t amh_render_noalloc_max_bytes(void);
 int amh_render_frame_noalloc(const amh_render *r, double t, uint32_t *out_dev, void *stream);
+/* amh_render_frame replayed from a hipGraph (the reference renders frame by
+ * frame, morph.cpp:1302-1429; the graph is this engine's own).  Create
+ * captures one frame's launches for r and the workspace (prepared by
+ * amh_render_workspace_init) on a stream made by amh_stream_create; each
+ * amh_render_graph_frame then writes the frame's time and out_dev into a
+ * small device block and replays the graph, with the pixels of
+ * amh_render_frame(r, t, ...).  r's sizes, options and device arrays (not
+ * their contents) and the workspace must stay as at creation; frames with
+ * nothing to draw run eagerly. */
+int amh_render_graph_create(void **graph, const amh_render *r, void *ws_dev, size_t ws_bytes, void *stream);
+int amh_render_graph_frame(void *graph, double t, uint32_t *out_dev, void *stream);
+int amh_render_graph_destroy(void *graph);
 
 /* --------------------------------------------------------- fluid path -- */
 /* The fluid renderer (fluidsteps > 0): morph::step_fluid, FluidModel::step

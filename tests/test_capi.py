@@ -55,6 +55,10 @@ def test_error_path_without_device_work():
     assert rc == -1 and b"invalid" in lib.amh_last_error()
     rc = lib.amh_refine_chain(None, 2, 10, 0, 5, 0, 1, 0, 0, None, None)
     assert rc == -1
+    g = C.c_void_p()
+    assert lib.amh_render_graph_create(C.byref(g), None, None, 0, None) == -1 and not g.value
+    assert lib.amh_render_graph_frame(None, 0.5, None, None) == -1
+    assert lib.amh_render_graph_destroy(None) == 0
 
 
 @pytest.mark.parametrize("gsx,gsy,w,h", [(65535, 64, 64, 64), (64, 70000, 64, 64), (64, 64, 65536, 8),

@@ -119,3 +119,34 @@ def test_render_at_density_within_one_lsb_of_reference(golden_density, density, 
     if cfg.show_blobs == 0:  # AVERAGE / DISTINCT draw the blob's own alpha (morph.cpp:1321-1334)
         assert halved > 0  # pixels with fewer taps than the density exist
     assert worst <= 1
+
+
+@pytest.mark.parametrize("variant", ["default", "spline_perlin_bg_feather2"])
+def test_render_graph_replays_equal_eager_frames(golden_cfg1, variant):
+    """amh_render_graph_frame (one captured frame replayed with the time and
+    output in a device block) gives the eager frames bit for bit, replay
+    after replay, with eager frames rendered between the replays."""
+    rend, _ = _renderer_for(golden_cfg1, variant)
+    lib, T = G.lib(), G.torch()
+    ts = [float(t) for t in golden_cfg1[f"render_{variant}_t"]][:6] + [-0.75, 2.3]
+    eager = [rend.frame(t) for t in ts]
+    outs = [G.zeros(rend.W * rend.H, T.int32) for _ in ts]
+    G.sync()
+    stream, graph = C.c_void_p(), C.c_void_p()
+    G.check(lib.amh_stream_create(C.byref(stream)), "stream")  # capture needs a created stream
+    try:
+        G.check(lib.amh_render_graph_create(C.byref(graph), C.byref(rend.args), G.dptr(rend.ws), rend.wsb, stream),
+                "amh_render_graph_create")
+        for rep in range(3):
+            G.log(f"render graph {variant}: replay round {rep}")
+            for t, o in zip(ts, outs):
+                G.check(lib.amh_render_graph_frame(graph, t, G.dptr(o), stream), "amh_render_graph_frame")
+            G.check(lib.amh_stream_sync(stream), "sync")
+            for t, o, want in zip(ts, outs, eager):
+                got = G.to_host(o, np.uint32).view(np.uint8).reshape(rend.H, rend.W, 4)
+                assert (got == want).all(), (rep, t)
+            # eager work between replays on the same workspace
+            assert (rend.frame(ts[rep]) == eager[rep]).all()
+    finally:
+        lib.amh_render_graph_destroy(graph)
+        lib.amh_stream_destroy(stream)
