@@ -497,10 +497,10 @@ def main():
     achieved = dom_bytes / dom_s / 1e9
     roof_all = {k: {"bytes_per_launch": int(b), "launch_us": round(t * 1e6, 3), "GBs": round(b / t / 1e9, 1),
                     "total_ms": round(total_ms(k), 3)} for k, (b, t, _) in rooflines.items()}
-    # measured HBM bytes per launch of the dominant kernel: two rocprofv3
-    # counter passes (FETCH_SIZE, WRITE_SIZE) on this workload, each scaled
-    # by what the counter reads for a plain copy of known size; kept as data
-    # beside the profile summaries (null when absent for this workload)
+    # measured HBM bytes per launch of the dominant kernel: rocprofv3
+    # counter passes on this workload (reads from the L2's read requests by
+    # size, writes from WRITE_SIZE), kept as data beside the profile
+    # summaries (null when absent for this workload)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"{args.workload}_hbm_traffic.json")
     if os.path.exists(tpath):
