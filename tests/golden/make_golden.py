@@ -546,10 +546,65 @@ def fluid_draw_fixture():
     return d
 
 
+# The command-line driver's run of the cfg1 key frames with greedy blobs
+# (-t 40 -m 6) at the driver's defaults (demo/options.h:27-62: density 2,
+# degeneration 10000, cycle length 100000, 100 dust samples, seed -s 1):
+# 500 matching steps, chain build, one refinement visit per chain, then 8
+# frames (SPLINE, PERLIN, textured blobs).
+CLI_MATCH_STEPS = 500
+CLI_FRAMES = 8
+
+
+def cli_fixture():
+    """The reference's refined chains and its 8 frames for that run, so that
+    the driver, resumed from those chains, can be checked frame for frame."""
+    r = op.ref()
+    frames = synth.workload("ellipses64")
+    m = r.ref_morph_new()
+    for k, v in (("seed", 1), ("threads", 0), ("cycle_length", 100000), ("density", 2), ("degeneration", 10000),
+                 ("blob_box_samples", 100), ("blob_box_grip", 65535), ("blob_threshold", 40 / 255.0),
+                 ("blob_min_size", 6), ("blob_number", 1), ("motion", 4), ("fading", 6), ("show_blobs", 0),
+                 ("feather", 0), ("keep_background", 0), ("finite", 0)):
+        r.ref_morph_set_option(m, k.encode(), float(v))
+    for f, img in enumerate(frames):
+        a = np.ascontiguousarray(img)
+        r.ref_morph_add_rgba(m, f, op.ptr(a), 64, 64)
+    r.ref_morph_set_resolution(m, 64, 64)
+    state = r.ref_morph_run_until(m, 2, 1 << 40)
+    assert state == 2, STATE.get(state)
+    # the driver's counted loop: match_steps + 1 worker steps in matching
+    state = r.ref_morph_run(m, CLI_MATCH_STEPS + 1)
+    if state == 2:
+        r.ref_morph_next_state(m)
+    state = r.ref_morph_run(m, 1)
+    assert state == 3, STATE.get(state)
+    nch = r.ref_morph_chain_count(m)
+    state = r.ref_morph_run(m, nch)  # one visit per chain
+    widths, pts = [], []
+    for key in range(nch):
+        hh, ms = C.c_size_t(), C.c_size_t()
+        w = r.ref_morph_chain(m, key, C.byref(hh), C.byref(ms), None)
+        q = np.zeros(w * hh.value, np.uint64)
+        r.ref_morph_chain(m, key, C.byref(hh), C.byref(ms), op.ptr(q))
+        widths.append(w)
+        pts.append(q)
+    r.ref_morph_next_state(m)
+    state = r.ref_morph_run(m, 1)
+    assert state == 4, STATE.get(state)
+    imgs = np.zeros((CLI_FRAMES, 64, 64, 4), np.uint8)
+    ts = np.zeros(CLI_FRAMES)
+    for i in range(CLI_FRAMES):
+        ts[i] = r.ref_morph_get_time(m, i, CLI_FRAMES)
+        r.ref_morph_render(m, ts[i], op.ptr(imgs[i]))
+    r.ref_morph_free(m)
+    return {"frames": np.stack(frames), "match_steps": np.int64(CLI_MATCH_STEPS), "chain_w": np.array(widths, np.int64),
+            "chain_pts": np.concatenate(pts), "render": imgs, "render_t": ts}
+
+
 def main():
     if not op.have_ref():
         sys.exit("oracle/_ref/libamref.so is missing: run `make -C oracle ref` where the reference exists")
-    which = set(sys.argv[1:]) or {"functions", "cfg1", "blobify", "match", "density", "fluid"}
+    which = set(sys.argv[1:]) or {"functions", "cfg1", "blobify", "match", "density", "fluid", "cli"}
     if "functions" in which:
         f = functions_fixture()
         np.savez_compressed(os.path.join(OUT, "functions.npz"), **f)
@@ -582,6 +637,11 @@ def main():
         mt = match_fixture()
         np.savez_compressed(os.path.join(OUT, "match.npz"), **mt)
         print("match: blobs per case", [mt[f"m{c}_nblobs0"].tolist() for c in range(len(MATCH_CASES))])
+    if "cli" in which:
+        cl = cli_fixture()
+        np.savez_compressed(os.path.join(OUT, "cli.npz"), **cl)
+        print("cli: chain widths", cl["chain_w"].tolist(), "drawn pixels per frame",
+              [int((f[..., 3] > 0).sum()) for f in cl["render"]])
     for name in sorted(which):
         print(name + ".npz", os.path.getsize(os.path.join(OUT, name + ".npz")), "bytes")
 

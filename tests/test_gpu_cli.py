@@ -78,3 +78,52 @@ def test_cli_frames_equal_library_frames(tmp_path, variant):
         assert got.shape == (64, 64, 4)
         assert np.array_equal(got, want[i]), f"frame {i + 1}"
     assert (want[0][..., 3] > 0).sum() > 100
+
+
+def test_cli_resumed_from_reference_chains_equals_reference_frames(tmp_path):
+    """The driver at its defaults with greedy blobs (-t 40 -m 6): its blob
+    lists and its matching are the reference's (tests/test_gpu_match.py), so
+    resumed from the chains the compiled reference refined in the same run
+    (--load-chains) with no refinement of its own, its 8 PNG frames are
+    within 1 LSB of the frames the reference rendered from those chains
+    (golden/cli.npz), at the driver's default density 2."""
+    if not os.path.exists(CLI):
+        pytest.fail("atomorph_cli not built: python -m atomorph_amd.build")
+    fx = dict(np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cli.npz")))
+    frames = list(fx["frames"])
+    h, steps = len(frames), int(fx["match_steps"])
+    # a checkpoint of the reference's chains in this engine's format: the
+    # library reaches the same chain shapes, takes the points and saves them
+    m = M.Morph()
+    m.set(seed=1, cycle_length=100000, degeneration=10000, density=2, threads=8, blob_box_samples=100,
+          blob_threshold=40 / 255.0, blob_min_size=6)
+    for i, f in enumerate(frames):
+        m.add_frame_rgba(i, f)
+    m.set_resolution(64, 64)
+    assert m.run(1) == M.STATE_BLOB_UNIFICATION
+    assert m.run(1) == M.STATE_BLOB_MATCHING
+    if m.run(steps + 1) == M.STATE_BLOB_MATCHING:
+        m.next_state()
+    assert m.run(1) == M.STATE_ATOM_MORPHING, m.error()
+    _, widths = m.chains()
+    assert np.array_equal(widths, fx["chain_w"])
+    m.put_chains(np.ascontiguousarray(fx["chain_pts"].reshape(-1, h).T))
+    ckpt = tmp_path / "reference.chains"
+    m.save_chains(ckpt)
+    names = []
+    for i, f in enumerate(frames):
+        names.append(f"key_{i}.png")
+        P.write_rgba8(tmp_path / names[-1], f, filters=(0, 1, 2, 3, 4))
+    out = tmp_path / "out"
+    out.mkdir()
+    cmd = [CLI, "-i", str(tmp_path), "-o", str(out), "-f", "t", "-s", "1", "-F", str(len(fx["render"])),
+           "-t", "40", "-m", "6", "--match-steps", str(steps), "--morph-steps", "0", "--load-chains", str(ckpt)]
+    G.log("cli resumed from the reference's chains: " + " ".join(cmd[1:]))
+    r = subprocess.run(cmd + names, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for i, want in enumerate(fx["render"]):
+        got = P.read_rgba8(out / f"t_{i + 1:04d}.png")
+        d = np.abs(got.astype(np.int32) - want.astype(np.int32))
+        G.log(f"  frame {i + 1}: worst channel difference {int(d.max())}, differing channels {int((d > 0).sum())}")
+        assert d.max() <= 1, f"frame {i + 1}"
+        assert (want[..., 3] > 0).sum() > 1000
