@@ -13,7 +13,9 @@ Workloads:
                    random colour per blob -- the 512^2 / 1024^2 workloads;
   powerlaw_blob_frame  N separate blobs (rotated filled ellipses) whose areas
                    follow a power law p(s) ~ s^-alpha on [smin, smax] -- the
-                   2048^2 workload.
+                   2048^2 workloads: on [8, 60000] (about 0.2 M blob pixels
+                   per frame), and on [200, 60000] (about 2 M per frame, 4 M
+                   over the pair).
 """
 import numpy as np
 
@@ -215,6 +217,8 @@ def workload(name):
         return noise_blob_frames(1024, 4, 150, 0.18, seed=3)
     if name == "powerlaw2048":    # 2 x 2048^2, 1200 blobs, areas ~ s^-1.8 on [8, 60000]
         return [powerlaw_blob_frame(2048, 1200, 1.8, 8, 60000, seed=5 * 1000003 + f)[0] for f in range(2)]
+    if name == "powerlaw2048_4m":  # the same law on [200, 60000]: about 2 M blob pixels per frame
+        return [powerlaw_blob_frame(2048, 1200, 1.8, 200, 60000, seed=7 * 1000003 + f)[0] for f in range(2)]
     raise KeyError(name)
 
 

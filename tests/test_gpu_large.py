@@ -1,11 +1,14 @@
-"""The 2048^2 power-law workload end to end on the GPU: two key frames of
-1200 separate blobs each, areas drawn from p(s) ~ s^-1.8 on [8, 60000] px.
+"""The 2048^2 power-law workloads end to end on the GPU: two key frames of
+1200 separate blobs each, areas drawn from p(s) ~ s^-1.8 on [8, 60000] px
+(about 0.2 M blob pixels per frame) and on [200, 60000] px (about 2 M per
+frame, 4.1 M over the pair: the scale of a ~4 M-atom pair).
 
 Blob detection equals the oracle's 4-connected labelling; the learned warm
 start keeps every chain row a permutation and lowers the starting cost; the
-autotuned batched descent (hipGraph replays) never raises the exact cost and
-keeps the rows permutations; 120 frames render into HBM in one call and
-sampled frames equal the oracle's rendering of the same chains within 1 LSB."""
+autotuned batched descent (hipGraph replays) never raises the exact cost,
+keeps the rows permutations and ends on the cost the oracle recomputes from
+the chains; 120 frames render into HBM in one call and sampled frames equal
+the oracle's rendering of the same chains within 1 LSB."""
 import ctypes as C
 
 import numpy as np
@@ -33,8 +36,9 @@ def _same_rows(a, b, widths):
     return True
 
 
-def test_powerlaw2048_learned_autotuned_pipeline():
-    frames = synth.workload("powerlaw2048")
+@pytest.mark.parametrize("name,sampled", [("powerlaw2048", (0, 37, 61, 119)), ("powerlaw2048_4m", (0, 61))])
+def test_powerlaw2048_learned_autotuned_pipeline(name, sampled):
+    frames = synth.workload(name)
     H, W = frames[0].shape[:2]
     params = L.load_default()
     if params is None:
@@ -44,7 +48,7 @@ def test_powerlaw2048_learned_autotuned_pipeline():
     for i, f in enumerate(frames):
         m.add_frame_rgba(i, f)
     m.set_resolution(W, H)
-    G.log("powerlaw2048: detection")
+    G.log(f"{name}: detection")
     assert m.run(1) == M.STATE_BLOB_UNIFICATION
     for f in range(2):
         lab, n = op.segment(frames[f])
@@ -55,14 +59,14 @@ def test_powerlaw2048_learned_autotuned_pipeline():
         bounds = np.flatnonzero(np.diff(lab[order])) + 1
         want = sorted(tuple(np.sort(g).tolist()) for g in np.split(order, bounds) if lab[g[0]] >= 0)
         assert got == want, f"frame {f}"
-    G.log("powerlaw2048: matching + chain build")
+    G.log(f"{name}: matching + chain build")
     assert m.run(2001) == M.STATE_BLOB_MATCHING
     m.next_state()
     assert m.run(1) == M.STATE_ATOM_MORPHING
     pts0, widths = m.chains()
     c_plain = m.cost()
 
-    G.log(f"powerlaw2048: learned warm start ({int(widths.sum())} atoms, {len(widths)} chains)")
+    G.log(f"{name}: learned warm start ({int(widths.sum())} atoms, {len(widths)} chains)")
     m.set_learned(params)
     m.warm_start()
     m.set_learned(None)
@@ -71,7 +75,7 @@ def test_powerlaw2048_learned_autotuned_pipeline():
     assert c_warm < c_plain
     assert _same_rows(pts1, pts0, widths)
 
-    G.log("powerlaw2048: autotuned batched descent in hipGraph replays")
+    G.log(f"{name}: autotuned batched descent in hipGraph replays")
     m.set_refine(M.REFINE_BATCHED, rounds=0, tile=-1, graph_passes=16)
     costs = [c_warm]
     for _ in range(4):
@@ -80,17 +84,19 @@ def test_powerlaw2048_learned_autotuned_pipeline():
     assert all(b <= a for a, b in zip(costs, costs[1:])) and costs[-1] < costs[0], costs
     pts2, _ = m.chains()
     assert _same_rows(pts2, pts0, widths)
+    h = pts2.shape[0]
+    assert op.chain_cost(G.atom_major(pts2.ravel(), h), h) == costs[-1]
 
-    G.log(f"powerlaw2048: render {FRAMES_OUT} frames into HBM")
+    G.log(f"{name}: render {FRAMES_OUT} frames into HBM")
     T = G.torch()
     out = T.empty(FRAMES_OUT * H * W * 4, dtype=T.uint8, device="cuda")
     times = [m.get_time(i, FRAMES_OUT) for i in range(FRAMES_OUT)]
     ms = m.render_device(times, out.data_ptr())
     assert ms > 0
+    G.log(f"{name}: {FRAMES_OUT} frames in {ms:.1f} ms")
     frames_dev = out.view(FRAMES_OUT, H, W, 4)
 
     # the oracle's rendering of the same chains and blob order
-    h = pts2.shape[0]
     nb = m.blob_count(0)
     order = np.zeros((h, nb), np.int32)
     for f in range(h):
@@ -105,8 +111,8 @@ def test_powerlaw2048_learned_autotuned_pipeline():
     am = G.atom_major(pts2.ravel(), h)
     w64 = widths.astype(np.int64)
     want = np.zeros((H, W, 4), np.uint8)
-    for i in (0, 37, 61, 119):
-        G.log(f"powerlaw2048: oracle frame {i}")
+    for i in sampled:
+        G.log(f"{name}: oracle frame {i}")
         op.oracle().or_render_atoms(C.byref(cfg), op.ptr(keys), op.ptr(am), op.ptr(off), op.ptr(w64), len(widths),
                                     op.ptr(order), nb, None, float(times[i]), op.ptr(want))
         got = frames_dev[i].cpu().numpy()
