@@ -1277,6 +1277,7 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
 extern "C" int amh_fluid_destroy(void *fluid) {
     amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);
     if (!f) return AMH_OK;
+    (void)hipDeviceSynchronize();  // frames may still be queued on the caller's stream
     for (auto &g : f->graphs)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (f->buf) (void)hipFree(f->buf);

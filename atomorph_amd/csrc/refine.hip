@@ -734,6 +734,7 @@ extern "C" int amh_graph_launch(void *graph, void *stream) {
 extern "C" int amh_graph_destroy(void *graph) {
     amh_graph_s *g = static_cast<amh_graph_s *>(graph);
     if (!g) return AMH_OK;
+    (void)hipDeviceSynchronize();  // replays may still be queued on the caller's stream
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;

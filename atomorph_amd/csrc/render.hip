@@ -662,6 +662,7 @@ extern "C" int amh_render_graph_frame(void *graph, double t, uint32_t *out, void
 extern "C" int amh_render_graph_destroy(void *graph) {
     amh_render_graph_s *g = static_cast<amh_render_graph_s *>(graph);
     if (!g) return AMH_OK;
+    (void)hipDeviceSynchronize();  // replays may still be queued on the caller's stream
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     (void)hipFree(g->fp);
     delete g;

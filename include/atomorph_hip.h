@@ -203,6 +203,8 @@ int amh_refine_graph_create(void **graph, uint64_t *pts_dev, int32_t nkeys, int6
                             int32_t nitems, int32_t tile, int32_t npasses, int32_t rounds, uint64_t seed,
                             uint32_t *queue_dev, amh_refine_stats *stats_dev, void *stream);
 int amh_graph_launch(void *graph, void *stream);
+/* Waits for the device's pending work (replays may still be queued), then
+ * frees the graph. */
 int amh_graph_destroy(void *graph);
 
 /* Autotuner of the batched descent (no counterpart in the reference: its
@@ -304,6 +306,8 @@ int amh_render_frame_noalloc(const amh_render *r, double t, uint32_t *out_dev, v
  * nothing to draw run eagerly. */
 int amh_render_graph_create(void **graph, const amh_render *r, void *ws_dev, size_t ws_bytes, void *stream);
 int amh_render_graph_frame(void *graph, double t, uint32_t *out_dev, void *stream);
+/* Waits for the device's pending work (replays may still be queued), then
+ * frees the graph. */
 int amh_render_graph_destroy(void *graph);
 
 /* --------------------------------------------------------- fluid path -- */
@@ -317,6 +321,8 @@ int amh_render_graph_destroy(void *graph);
  * 8 <= gsx, gsy < 65535 and 1 <= width, height <= 65535 (AMH_E_ARG beyond). */
 int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t count, int32_t width, int32_t height,
                      int32_t nkeys, void *stream);
+/* Waits for the device's pending work (frames may still be queued), then
+ * frees the model and its graphs. */
 int amh_fluid_destroy(void *fluid);
 /* Particles back to the state morph::synchronize gives a new model
  * (morph.cpp:243-262): centred, inactive, green. */

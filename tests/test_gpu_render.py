@@ -147,6 +147,18 @@ def test_render_graph_replays_equal_eager_frames(golden_cfg1, variant):
                 assert (got == want).all(), (rep, t)
             # eager work between replays on the same workspace
             assert (rend.frame(ts[rep]) == eager[rep]).all()
+        # destroyed with replays still queued: it waits for them
+        for o in outs:
+            o.zero_()
+        G.sync()
+        for t, o in zip(ts, outs):
+            G.check(lib.amh_render_graph_frame(graph, t, G.dptr(o), stream), "amh_render_graph_frame")
+        G.check(lib.amh_render_graph_destroy(graph), "amh_render_graph_destroy")
+        graph = C.c_void_p()
+        G.check(lib.amh_stream_sync(stream), "sync")
+        for t, o, want in zip(ts, outs, eager):
+            got = G.to_host(o, np.uint32).view(np.uint8).reshape(rend.H, rend.W, 4)
+            assert (got == want).all(), ("after destroy", t)
     finally:
         lib.amh_render_graph_destroy(graph)
         lib.amh_stream_destroy(stream)
