@@ -647,9 +647,38 @@ struct P2gItem {
 
 // Mass (= density), density gradient and strength-weighted colour of the
 // tile's nodes from the moments of the cells that reach them.
+//
+// Chunked: the block first counting-sorts its chunk by cell (the kReach x
+// kReach cells in reach, then the entries out of reach) through LDS, sums
+// the moments in that order, and leaves the chunk of the bin list in that
+// order for k_forces and k_g2p.  Particles move about two cells per step,
+// so the frame's cell order is gone after one step; without the sort, runs
+// of lanes on one cell (summed with DPP before one LDS atomic per run)
+// shrink to about one lane and every lane pays its own atomics.
+//
+// Seven waves per SIMD: the LDS (18 KB) allows eight blocks per CU, and
+// without the bound the node assembly's scheduling takes 90 VGPRs (five).
 template <bool kChunked>
-__global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_p2g(Particles P, Grid Gd, int gsx,
+                                                                                   int gsy, Bins B) {
     __shared__ float cs[kCellSums][kReach * kReach];
+    constexpr int kCells = kReach * kReach;  // local cells; bucket kCells: out of reach
+    constexpr int kS = kChunked ? kT : 1;
+    __shared__ int cellpos[kChunked ? kCells + 2 : 1];  // per bucket: count, then first place
+    __shared__ int2 mine[kS];  // per thread: its list entry, (bucket << 8) | rank (-1: no entry)
+    __shared__ int perm[kS];   // sorted place -> thread
+    // The chunk's items (position and colour terms), one array per field so
+    // that consecutive lanes write consecutive words; dead once the moments
+    // are summed, so they share their space with the node assembly's partials.
+    union Stage {
+        struct {
+            double x[kS], y[kS];
+            float t[5][kS];
+        } item;
+        float part[3][8][kTile * kTile];
+    };
+    __shared__ Stage stage;
+    static_assert(kCells + 2 <= 2 * 64, "one wave scans the buckets, two per lane");
     const int nchunks = kChunked ? *B.nchunks : 0;
     for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
         int tile, s0, s1;
@@ -666,25 +695,35 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
         };
         const auto clear = [&] {
             for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
+            if (kChunked) {
+                if (threadIdx.x < kCells + 2) cellpos[threadIdx.x] = 0;
+                // an empty chunk (an empty tile has one) never runs the body
+                mine[threadIdx.x % kS] = make_int2(0, -1);
+            }
             __syncthreads();
         };
-        for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool, const P2gItem &it) {
-            // branch-free: selects instead of conditional blocks, so no load
-            // is deferred into a branch
-            const int cx = static_cast<int>(it.x - 0.5), cy = static_cast<int>(it.y - 0.5);
-            const bool valid = slot >= 0 && in_reach(cx, cy, x0, y0);
+        // A particle's colour terms: strength-weighted running mean ==
+        // sum(s c) / sum(s), over mature particles of positive strength.
+        const auto tints = [](const P2gItem &it, float (&t)[5]) {
+            const bool tint = it.mature && it.s > 0.0;
+            t[0] = tint ? float(it.s * it.r) : 0.0f;
+            t[1] = tint ? float(it.s * it.g) : 0.0f;
+            t[2] = tint ? float(it.s * it.b) : 0.0f;
+            t[3] = tint ? float(it.s * it.a) : 0.0f;
+            t[4] = tint ? float(it.s) : 0.0f;
+        };
+        // The moments of one particle at (x, y) with colour terms t (valid:
+        // in reach of the tile) added to its cell's LDS sums; every lane of
+        // the block calls it.  Branch-free: selects instead of conditional
+        // blocks, so no load is deferred into a branch.
+        const auto moments = [&](bool valid, double x, double y, const float (&t)[5]) {
+            const int cx = static_cast<int>(x - 0.5), cy = static_cast<int>(y - 0.5);
             float c[kCellSums];
-            const double e = it.x - cx - 1.0, h = it.y - cy - 1.0;
+            const double e = x - cx - 1.0, h = y - cy - 1.0;
             const double ep[3] = {1.0, e, e * e}, hp[3] = {1.0, h, h * h};
             for (int p = 0; p < 3; ++p)
                 for (int q = 0; q < 3; ++q) c[p * 3 + q] = valid ? float(ep[p] * hp[q]) : 0.0f;
-            // strength-weighted running mean == sum(s c) / sum(s)
-            const bool tint = valid && it.mature && it.s > 0.0;
-            c[9] = tint ? float(it.s * it.r) : 0.0f;
-            c[10] = tint ? float(it.s * it.g) : 0.0f;
-            c[11] = tint ? float(it.s * it.b) : 0.0f;
-            c[12] = tint ? float(it.s * it.a) : 0.0f;
-            c[13] = tint ? float(it.s) : 0.0f;
+            for (int k = 0; k < 5; ++k) c[9 + k] = valid ? t[k] : 0.0f;
             const Seg S = segments(valid, cx, cy);
             for (int k = 0; k < kCellSums; ++k) c[k] = seg_sum(c[k], S);
             if (valid && S.tail) {
@@ -692,11 +731,66 @@ __global__ __launch_bounds__(kT) void k_p2g(Particles P, Grid Gd, int gsx, int g
                 for (int k = 0; k < kCellSums; ++k)
                     if (c[k] != 0.0f) atomicAdd(&cs[k][idx], c[k]);
             }
-        }, clear);
+        };
+        if constexpr (!kChunked) {
+            for_bin<true>(B, s0, s1, fetch, [&](int slot, bool, const P2gItem &it) {
+                const int cx = static_cast<int>(it.x - 0.5), cy = static_cast<int>(it.y - 0.5);
+                float t[5];
+                tints(it, t);
+                moments(slot >= 0 && in_reach(cx, cy, x0, y0), it.x, it.y, t);
+            }, clear);
+        } else {
+            // at most kT entries: one pass, ranks fit in 8 bits
+            for_bin<false>(B, s0, s1, fetch, [&](int slot, bool home, const P2gItem &it) {
+                const int cx = static_cast<int>(it.x - 0.5), cy = static_cast<int>(it.y - 0.5);
+                const int bucket = in_reach(cx, cy, x0, y0) ? (cx - x0 + 2) * kReach + (cy - y0 + 2) : kCells;
+                if (slot >= 0) {
+                    const int l = threadIdx.x;
+                    mine[l] = make_int2(slot | (home ? kHome : 0), (bucket << 8) | atomicAdd(&cellpos[bucket], 1));
+                    float t[5];
+                    tints(it, t);
+                    stage.item.x[l] = it.x;
+                    stage.item.y[l] = it.y;
+                    for (int k = 0; k < 5; ++k) stage.item.t[k][l] = t[k];
+                }
+            }, clear);
+            __syncthreads();
+            if (threadIdx.x < 64) {  // exclusive scan of the bucket counts into first places
+                const int lane = int(threadIdx.x);
+                const int v0 = 2 * lane < kCells + 1 ? cellpos[2 * lane] : 0;
+                const int v1 = 2 * lane + 1 < kCells + 1 ? cellpos[2 * lane + 1] : 0;
+                int incl = v0 + v1;
+                for (int off = 1; off < 64; off <<= 1) {
+                    const int v = __shfl_up(incl, off, 64);
+                    if (lane >= off) incl += v;
+                }
+                const int excl = incl - v0 - v1;
+                if (2 * lane < kCells + 1) cellpos[2 * lane] = excl;
+                if (2 * lane + 1 < kCells + 1) cellpos[2 * lane + 1] = excl + v0;
+            }
+            __syncthreads();
+            {
+                const int2 m = mine[threadIdx.x];
+                if (m.y >= 0) {
+                    const int dest = cellpos[m.y >> 8] + (m.y & 255);
+                    perm[dest] = int(threadIdx.x);
+                    B.list[s0 + dest] = m.x;
+                }
+            }
+            __syncthreads();
+            // sorted place threadIdx.x; the entries in reach come first
+            const bool in = int(threadIdx.x) < cellpos[kCells];
+            const int src = in ? perm[threadIdx.x] : 0;
+            // place 0's item may be stale or unset: keep the position finite
+            const double x = in ? stage.item.x[src] : 0.0, y = in ? stage.item.y[src] : 0.0;
+            float t[5];
+            for (int k = 0; k < 5; ++k) t[k] = stage.item.t[k][src];
+            moments(in, x, y, t);
+        }
         __syncthreads();
         // Nodes from the cells: thread (a, node) sums the three cells
         // (node - (a, b)), then one thread per node adds the three partials.
-        __shared__ float part[3][8][kTile * kTile];
+        auto &part = stage.part;
         static_assert(3 * kTile * kTile <= kT, "one thread per (a, node)");
         {
             const int t = threadIdx.x, node = t % (kTile * kTile), a = t / (kTile * kTile);
