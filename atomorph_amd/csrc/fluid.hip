@@ -299,15 +299,19 @@ struct Bins {
     int32_t *start;     // [tiles + 1]
     int32_t *rank;      // [4][n]: place of particle i's j-th entry in its bin
     int32_t *list;      // [<= 4n] slot | kHome, grouped by bin
-    // Chunks of kT entries, each one workgroup (NULL: one workgroup per
-    // tile): a dense bin is split over several workgroups, which add their
-    // node sums with atomics to nodes cleared when the bins were placed;
-    // a tile with one chunk (every tile has at least one) stores its nodes.
-    // chunk[c] = (tile, first entry, end entry, only chunk of its tile),
-    // c < *nchunks.
+    // Work items of kT entries at most, each one workgroup (chunk NULL:
+    // one workgroup per tile).  A dense bin is split over several items,
+    // which add their node sums with atomics to nodes cleared when the bins
+    // were placed; a tile with one item stores its nodes.  Only tiles with
+    // entries get items, plus one (empty) item for each tile that had
+    // entries in the step before and has none now, which stores zeros: a
+    // tile empty in both steps keeps the zeros stored then.
+    // chunk[c] = (tile, first entry, end entry, only item of its tile),
+    // c < *nchunks, in no particular order (appended with atomics).
     int4 *chunk;
-    int32_t *nchunks;      // [1] chunks written by the last placing
-    int32_t max_chunks;    // capacity of chunk (an upper bound of *nchunks)
+    int32_t *nchunks;       // this step's item count (appended by the placing)
+    int32_t *nchunks_next;  // the next step's, cleared by the placing
+    int32_t max_chunks;     // capacity of chunk (an upper bound of *nchunks)
 };
 constexpr int kLdsTiles = 4096;  // up to this many tiles the placing blocks scan the counts in LDS
 
@@ -425,6 +429,19 @@ __global__ __launch_bounds__(kT) void k_bin_count(Tiles tiles_of, int n, Bins B,
     count_entries(act, tx0, tx1, ty0, ty1, i, n, B, par);
 }
 
+// Work items of a tile with c entries that had `before` entries in the step
+// before: ceil(c / kT) chunks, or one empty item (which zeroes the tile's
+// nodes) when the tile has just emptied, or none.
+__device__ __forceinline__ int work_items(int c, int before) {
+    return c > 0 ? (c + kT - 1) / kT : (before > 0 ? 1 : 0);
+}
+
+// The nc work items of tile t (entries [b0, b1)) at B.chunk[c0 ...].
+__device__ __forceinline__ void put_work_items(const Bins &B, int c0, int nc, int t, int b0, int b1) {
+    for (int j = 0; j < nc; ++j)
+        B.chunk[c0 + j] = make_int4(t, b0 + j * kT, nc == 1 ? b1 : min(b1, b0 + (j + 1) * kT), nc == 1);
+}
+
 // Particles into their bins (the tiles recomputed from the unchanged
 // positions) and the other parity's counts cleared for the next step.  With
 // kLds every block scans the counts into LDS (block 0 also stores them in
@@ -445,12 +462,16 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
     for (int j = 0; j < 4; ++j) rank[j] = B.rank[int64_t(j) * n + ic];
     if (kLds) {
         const int per = (tiles + kT - 1) / kT;
+        static_assert(kLdsTiles / kT <= 32, "a thread's tiles fit a 32-bit mask");
         const int lo = min(int(threadIdx.x) * per, tiles), hi = min(lo + per, tiles);
         int sum = 0;
+        uint32_t before = 0;  // block 0: bit k - lo set when tile k had entries in the step before
+        const bool items = B.chunk && blockIdx.x == 0;
         for (int k = lo; k < hi; ++k) {
-            const int c = B.count[par][k];
+            const int c = B.count[par][k], b = items ? B.count[par ^ 1][k] : 0;
             sstart[k] = sum;
             sum += c;
+            before |= (b > 0 ? 1u : 0u) << (k - lo);
         }
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         int incl = sum;
@@ -468,10 +489,13 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
         if (blockIdx.x == 0)
             for (int k = threadIdx.x; k <= tiles; k += kT) B.start[k] = sstart[k];
         start = sstart;
-        if (B.chunk && blockIdx.x == 0) {
-            // chunks per tile max(1, ceil(count / kT)), scanned like the counts
+        if (blockIdx.x == 0) {
+            // block 0, thread by thread over its tiles lo..hi: the work items
+            // in tile order (their counts scanned like the entry counts),
+            // then the other parity's counts cleared for the next step
             int csum = 0;
-            for (int k = lo; k < hi; ++k) csum += max(1, (sstart[k + 1] - sstart[k] + kT - 1) / kT);
+            if (B.chunk)
+                for (int k = lo; k < hi; ++k) csum += work_items(sstart[k + 1] - sstart[k], (before >> (k - lo)) & 1);
             int cincl = csum;
             for (int o = 1; o < 64; o <<= 1) {
                 const int v = __shfl_up(cincl, o, 64);
@@ -483,20 +507,33 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
             int c0 = cincl - csum;
             for (int w = 0; w < wave; ++w) c0 += wsum[w];
             for (int k = lo; k < hi; ++k) {
-                const int b0 = sstart[k], b1 = sstart[k + 1];
-                const int nc = max(1, (b1 - b0 + kT - 1) / kT);
-                for (int j = 0; j < nc; ++j)
-                    B.chunk[c0 + j] = make_int4(k, b0 + j * kT, nc == 1 ? b1 : min(b1, b0 + (j + 1) * kT), nc == 1);
-                c0 += nc;
+                if (B.chunk) {
+                    const int b0 = sstart[k], b1 = sstart[k + 1];
+                    const int nc = work_items(b1 - b0, (before >> (k - lo)) & 1);
+                    put_work_items(B, c0, nc, k, b0, b1);
+                    c0 += nc;
+                }
+                B.count[par ^ 1][k] = 0;
             }
-            if (threadIdx.x == kT - 1) *B.nchunks = c0;
+            if (B.chunk && threadIdx.x == kT - 1) *B.nchunks = c0;
         }
-        if (B.chunk) {  // nodes of tiles split over chunks start from zero
-            for (int t = blockIdx.x; t < tiles; t += gridDim.x)
-                if (sstart[t + 1] - sstart[t] > kT) tiles_of.clear_tile(t, B.nty);
+    } else {
+        // one thread per tile: its work items appended (in no order), then
+        // the other parity's count cleared for the next step
+        for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) {
+            if (B.chunk) {
+                const int b0 = start[k], b1 = start[k + 1];
+                const int nc = work_items(b1 - b0, B.count[par ^ 1][k]);
+                if (nc > 0) put_work_items(B, atomicAdd(B.nchunks, nc), nc, k, b0, b1);
+            }
+            B.count[par ^ 1][k] = 0;
         }
+        if (B.chunk && blockIdx.x == 0 && threadIdx.x == 0) *B.nchunks_next = 0;
     }
-    for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) B.count[par ^ 1][k] = 0;
+    if (B.chunk) {  // nodes of tiles split over several items start from zero
+        for (int t = blockIdx.x; t < tiles; t += gridDim.x)
+            if (start[t + 1] - start[t] > kT) tiles_of.clear_tile(t, B.nty);
+    }
     if (!binned) return;
     for (int j = 0; j < 4; ++j) {
         const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
@@ -567,22 +604,13 @@ __device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fet
     }
 }
 
-// Work item b of a tile kernel: the entries [s0, s1) of tile t -- chunk b
-// when B is chunked, else the whole bin of tile b -- and whether it is the
-// tile's only work item (then it stores its nodes; otherwise it adds them
-// atomically).  false: past the last item.  Workgroups take items b =
-// blockIdx.x, blockIdx.x + gridDim.x, ...; nchunks = *B.nchunks when B is
-// chunked.
+// Work item b of a tile kernel: the entries [s0, s1) of tile t (at most
+// kT) and whether it is the tile's only work item (then it stores its
+// nodes; otherwise it adds them atomically).  false: past the last item.
+// Workgroups take items b = blockIdx.x, blockIdx.x + gridDim.x, ...;
+// nchunks = *B.nchunks.
 __device__ __forceinline__ bool block_work(const Bins &B, int nchunks, int b, int &t, int &s0, int &s1,
                                            bool &single) {
-    if (!B.chunk) {
-        if (b >= B.ntx * B.nty) return false;
-        t = b;
-        s0 = B.start[t];
-        s1 = B.start[t + 1];
-        single = true;
-        return true;
-    }
     // loaded before the count is compared, so the two loads overlap (the
     // grid never exceeds max_chunks; later items clamp their index)
     const int4 c = B.chunk[min(b, B.max_chunks - 1)];
@@ -648,7 +676,7 @@ struct P2gItem {
 // Mass (= density), density gradient and strength-weighted colour of the
 // tile's nodes from the moments of the cells that reach them.
 //
-// Chunked: the block first counting-sorts its chunk by cell (the kReach x
+// The block first counting-sorts its chunk by cell (the kReach x
 // kReach cells in reach, then the entries out of reach) through LDS, sums
 // the moments in that order, and leaves the chunk of the bin list in that
 // order for k_forces and k_g2p.  Particles move about two cells per step,
@@ -658,15 +686,14 @@ struct P2gItem {
 //
 // Seven waves per SIMD: the LDS (18 KB) allows eight blocks per CU, and
 // without the bound the node assembly's scheduling takes 90 VGPRs (five).
-template <bool kChunked>
 __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_p2g(Particles P, Grid Gd, int gsx,
                                                                                    int gsy, Bins B) {
     __shared__ float cs[kCellSums][kReach * kReach];
     constexpr int kCells = kReach * kReach;  // local cells; bucket kCells: out of reach
-    constexpr int kS = kChunked ? kT : 1;
-    __shared__ int cellpos[kChunked ? kCells + 2 : 1];  // per bucket: count, then first place
-    __shared__ int2 mine[kS];  // per thread: its list entry, (bucket << 8) | rank (-1: no entry)
-    __shared__ int perm[kS];   // sorted place -> thread
+    constexpr int kS = kT;
+    __shared__ int cellpos[kCells + 2];  // per bucket: count, then first place
+    __shared__ int2 mine[kS];            // per thread: its list entry, (bucket << 8) | rank (-1: no entry)
+    __shared__ int perm[kS];             // sorted place -> thread
     // The chunk's items (position and colour terms), one array per field so
     // that consecutive lanes write consecutive words; dead once the moments
     // are summed, so they share their space with the node assembly's partials.
@@ -679,7 +706,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_
     };
     __shared__ Stage stage;
     static_assert(kCells + 2 <= 2 * 64, "one wave scans the buckets, two per lane");
-    const int nchunks = kChunked ? *B.nchunks : 0;
+    const int nchunks = *B.nchunks;
     for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
         int tile, s0, s1;
         bool single;
@@ -695,11 +722,9 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_
         };
         const auto clear = [&] {
             for (int k = threadIdx.x; k < kCellSums * kReach * kReach; k += kT) (&cs[0][0])[k] = 0.0f;
-            if (kChunked) {
-                if (threadIdx.x < kCells + 2) cellpos[threadIdx.x] = 0;
-                // an empty chunk (an empty tile has one) never runs the body
-                mine[threadIdx.x % kS] = make_int2(0, -1);
-            }
+            if (threadIdx.x < kCells + 2) cellpos[threadIdx.x] = 0;
+            // an empty item (a tile that has just emptied) never runs the body
+            mine[threadIdx.x] = make_int2(0, -1);
             __syncthreads();
         };
         // A particle's colour terms: strength-weighted running mean ==
@@ -732,14 +757,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                     if (c[k] != 0.0f) atomicAdd(&cs[k][idx], c[k]);
             }
         };
-        if constexpr (!kChunked) {
-            for_bin<true>(B, s0, s1, fetch, [&](int slot, bool, const P2gItem &it) {
-                const int cx = static_cast<int>(it.x - 0.5), cy = static_cast<int>(it.y - 0.5);
-                float t[5];
-                tints(it, t);
-                moments(slot >= 0 && in_reach(cx, cy, x0, y0), it.x, it.y, t);
-            }, clear);
-        } else {
+        {
             // at most kT entries: one pass, ranks fit in 8 bits
             for_bin<false>(B, s0, s1, fetch, [&](int slot, bool home, const P2gItem &it) {
                 const int cx = static_cast<int>(it.x - 0.5), cy = static_cast<int>(it.y - 0.5);
@@ -854,11 +872,10 @@ __device__ __forceinline__ void store_tile2(const float (&acc)[2][kTile * kTile]
 }
 
 // Node accelerations: pressure and wall forces (fluidmodel.cpp:251-357).
-template <bool kChunked>
 __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float acc[2][kTile * kTile];
     __shared__ float nodes[3][kHalo * kHalo];  // m, gx, gy
-    const int nchunks = kChunked ? *B.nchunks : 0;
+    const int nchunks = *B.nchunks;
     for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
         int tile, s0, s1;
         bool single;
@@ -880,7 +897,7 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
             const int q = slot >= 0 ? slot : 0;
             return double2{P.x[q], P.y[q]};
         };
-        for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool, const double2 &it) {
+        for_bin<false>(B, s0, s1, fetch, [&](int slot, bool, const double2 &it) {
             int cx, cy;
             double px[3], py[3], gx[3], gy[3];
             double pressure = 0.0, fx = 0.0, fy = 0.0;
@@ -911,11 +928,10 @@ struct G2pItem {
 // Particle velocity from node accelerations plus the attractor (written by
 // the particle's home tile), then the momentum of the tile's nodes
 // (fluidmodel.cpp:359-437).
-template <bool kChunked>
 __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int gsy, Bins B) {
     __shared__ float acc[2][kTile * kTile];
     __shared__ float accel[2][kHalo * kHalo];  // node accelerations per mass
-    const int nchunks = kChunked ? *B.nchunks : 0;
+    const int nchunks = *B.nchunks;
     for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
         int tile, s0, s1;
         bool single;
@@ -935,7 +951,7 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
             const int q = slot >= 0 ? slot : 0;
             return G2pItem{P.x[q], P.y[q], P.gx[q], P.gy[q], P.u[q], P.v[q], P.mature[q]};
         };
-        for_bin<!kChunked>(B, s0, s1, fetch, [&](int slot, bool home, const G2pItem &it) {
+        for_bin<false>(B, s0, s1, fetch, [&](int slot, bool home, const G2pItem &it) {
             int cx, cy;
             double px[3], py[3], gx[3], gy[3];
             const double x = it.x, y = it.y;
@@ -1258,7 +1274,7 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     (void)nkeys;
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
            (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) +
-           max_chunks(tiles, n) * 16 + 4 + 3 * 256 +
+           max_chunks(tiles, n) * 16 + 8 + 3 * 256 +
            nn * 4 * kGridFields + 6 * 256 + npx * 4 * 2 + 64 * 256 + 256;
 }
 
@@ -1314,7 +1330,7 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->bins.list = reinterpret_cast<int32_t *>(take(n * 4 * 4));
     f->bins.max_chunks = int32_t(max_chunks(tiles, n));
     f->bins.chunk = reinterpret_cast<int4 *>(take(size_t(f->bins.max_chunks) * 16));
-    f->bins.nchunks = reinterpret_cast<int32_t *>(take(4));
+    f->bins.nchunks = reinterpret_cast<int32_t *>(take(8));  // one count per step parity
     f->pbins.ntx = (width + kPxTile - 1) / kPxTile;
     f->pbins.nty = (height + kPxTile - 1) / kPxTile;
     const size_t ptiles = size_t(f->pbins.ntx) * size_t(f->pbins.nty);
@@ -1325,6 +1341,7 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->pbins.list = f->bins.list;
     f->pbins.chunk = nullptr;  // one workgroup per pixel tile
     f->pbins.nchunks = nullptr;
+    f->pbins.nchunks_next = nullptr;
     f->pbins.max_chunks = 0;
     f->sxy = reinterpret_cast<uint32_t *>(take(n * 4));
     f->staps = reinterpret_cast<double *>(take(n * 32));
@@ -1433,16 +1450,22 @@ namespace {
 // Counting sort of the particles' entries into the bins of B (tiles bins,
 // counts of parity par; the other parity is cleared for the next use).
 // counted: the counts and ranks are in place already.
+// Beyond kLdsTiles tiles the counts are scanned first: by the three-launch
+// device scan when workspace (sums[scan::blocks(tiles)], offs[+1]) is
+// given, else by one workgroup.
 template <class Tiles>
 void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s,
-                   bool counted = false) {
+                   bool counted = false, int *sums = nullptr, int *offs = nullptr) {
     const uint32_t gp = uint32_t((n + kT - 1) / kT);
     if (!counted) hipLaunchKernelGGL(k_bin_count<Tiles>, dim3(gp), dim3(kT), 0, s, tiles_of, n, B, par);
     const uint32_t gb = uint32_t((std::max(n, tiles) + kT - 1) / kT);
     if (tiles <= kLdsTiles) {
         hipLaunchKernelGGL((k_bin_place<true, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
     } else {
-        hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count[par], tiles, B.start);
+        if (sums && offs)
+            scan::exclusive(B.count[par], tiles, sums, offs, B.start, s);
+        else
+            hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count[par], tiles, B.start);
         hipLaunchKernelGGL((k_bin_place<false, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
     }
 }
@@ -1457,26 +1480,24 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     amh::ProfScope step_("fluid_step", s);
     if (f->count == 0) return AMH_OK;
     const int tiles = f->bins.ntx * f->bins.nty;
-    // dense bins are split over workgroups when the placing blocks can scan
-    // the counts in LDS (where the chunks are laid out)
+    // the work items of this step's parity (the counts alternate the same way)
     Bins B = f->bins;
-    const bool chunked = tiles <= kLdsTiles;
-    if (!chunked) B.chunk = nullptr;
-    // about one chunk per workgroup (a bin averages under one chunk); a
-    // workgroup takes further chunks when there are more
-    const uint32_t grid = uint32_t(chunked ? std::min(B.max_chunks, 2 * tiles) : tiles);
+    const int par = f->bin_par;
+    f->bin_par ^= 1;
+    B.nchunks = f->bins.nchunks + par;
+    B.nchunks_next = f->bins.nchunks + (par ^ 1);
+    // about one work item per workgroup (a bin averages under one chunk of
+    // kT entries, and empty tiles get none); a workgroup takes further items
+    // when there are more
+    const uint32_t grid = uint32_t(std::min(B.max_chunks, 2 * std::min(tiles, kLdsTiles)));
     {
         amh::ProfScope ps_("fluid_bins", s);
-        const int par = f->bin_par;
-        f->bin_par ^= 1;
-        bin_particles(StencilTiles{Q, f->gsx, f->gsy, f->G}, f->count, B, tiles, par, s, counted);
+        bin_particles(StencilTiles{Q, f->gsx, f->gsy, f->G}, f->count, B, tiles, par, s, counted, f->scan_sums,
+                      f->scan_offs);
     }
-    amh::launch_timed("k_p2g", chunked ? k_p2g<true> : k_p2g<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx,
-                      f->gsy, B);
-    amh::launch_timed("k_forces", chunked ? k_forces<true> : k_forces<false>, dim3(grid), dim3(kT), 0, s, Q, f->G,
-                      f->gsx, f->gsy, B);
-    amh::launch_timed("k_g2p", chunked ? k_g2p<true> : k_g2p<false>, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx,
-                      f->gsy, B);
+    amh::launch_timed("k_p2g", k_p2g, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+    amh::launch_timed("k_forces", k_forces, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
+    amh::launch_timed("k_g2p", k_g2p, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     amh::launch_timed("k_advect", k_advect, dim3(uint32_t((f->count + kT - 1) / kT)), dim3(kT), 0, s, Q, f->count,
                       f->G, f->gsx, f->gsy, int(steps_left), freedom_radius, seed, atom, B,
                       count_next ? f->bin_par : -1, fp, step_k);

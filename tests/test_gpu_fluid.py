@@ -96,6 +96,60 @@ def test_fluid_step_matches_oracle_many_particles(layout):
         assert (outs[k][:, 16] == Q[:, 16]).all()
 
 
+def _clustered(rng, n, lo, hi, shift=0.0):
+    """Particles in runs of 256 around random centres in [lo, hi)^2 (moved by
+    shift cells), with velocities, attractors and colours."""
+    P = np.zeros((n, 20))
+    centres = rng.uniform(lo, hi, (n // 256, 2)) + shift
+    P[:, 0] = np.clip(np.repeat(centres[:, 0], 256) + rng.normal(0, 3, n), 8, hi + 20)
+    P[:, 1] = np.clip(np.repeat(centres[:, 1], 256) + rng.normal(0, 3, n), 8, hi + 20)
+    P[:, 2:4] = rng.normal(0, 0.2, (n, 2))
+    P[:, 4] = P[:, 0] + rng.normal(0, 3, n)
+    P[:, 5] = P[:, 1] + rng.normal(0, 3, n)
+    P[:, 6] = 1.0
+    P[:, 7:15] = rng.uniform(0, 1, (n, 8))
+    P[:, 15] = rng.choice([0.1, 1.0], n)
+    P[:, 16] = 1.0
+    P[:, 17] = rng.integers(0, 2, n)
+    return P
+
+
+@pytest.mark.parametrize("grid", [276, 600])
+def test_fluid_steps_after_new_particles_match_oracle(grid):
+    """Work items exist only for tiles with entries, plus one zeroing item
+    for each tile that had entries the step before: after a step of one
+    particle set, a second set a few cells away (sharing the first one's
+    node halos) steps as the oracle steps it from an empty grid.  276^2 has
+    1,225 tiles (counts scanned in LDS), 600^2 has 5,625 (scanned on the
+    device first)."""
+    rng = np.random.default_rng(11)
+    n = 20480
+    A = _clustered(rng, n, 30, grid - 60)
+    B = _clustered(rng, n, 30, grid - 60, shift=4.0)
+    lib = G.lib()
+    f = _model(n, grid, grid)
+    try:
+        G.log(f"fluid grid {grid}: one step of the first set, then the second")
+        QA = np.ascontiguousarray(A.copy())
+        G.check(lib.amh_fluid_set_particles(f, op.ptr(QA), G.stream()), "set A")
+        G.check(lib.amh_fluid_step(f, 0, 20.0, 0, G.stream()), "step A")
+        Q = np.ascontiguousarray(B.copy())
+        G.check(lib.amh_fluid_set_particles(f, op.ptr(Q), G.stream()), "set B")
+        outs = []
+        for k in range(3):
+            G.check(lib.amh_fluid_step(f, 2 - k, 20.0, 0, G.stream()), "step B")
+            G.check(lib.amh_fluid_get_particles(f, op.ptr(Q), G.stream()), "get")
+            outs.append(Q.copy())
+    finally:
+        lib.amh_fluid_destroy(f)
+    R = B.copy()
+    for k in range(3):
+        op.oracle().or_fluid_step(op.ptr(R), n, grid, grid, 2 - k, 20.0, 0)
+        for col in (0, 1, 2, 3, 11, 12, 13, 14):
+            d = np.abs(outs[k][:, col] - R[:, col]).max()
+            assert d <= 2e-3, (k, col, d)
+
+
 def _fluid_frame(g, variant, t, fluidsteps, sp=1.0, reset=True, fluid=None, rend=None):
     """Render one fluid frame of the cfg1 chains through the C ABI."""
     lib = G.lib()
