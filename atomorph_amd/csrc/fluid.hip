@@ -690,17 +690,16 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_
                                                                                    int gsy, Bins B) {
     __shared__ float cs[kCellSums][kReach * kReach];
     constexpr int kCells = kReach * kReach;  // local cells; bucket kCells: out of reach
-    constexpr int kS = kT;
     __shared__ int cellpos[kCells + 2];  // per bucket: count, then first place
-    __shared__ int2 mine[kS];            // per thread: its list entry, (bucket << 8) | rank (-1: no entry)
-    __shared__ int perm[kS];             // sorted place -> thread
+    __shared__ int2 mine[kT];            // per thread: its list entry, (bucket << 8) | rank (-1: no entry)
+    __shared__ int perm[kT];             // sorted place -> thread
     // The chunk's items (position and colour terms), one array per field so
     // that consecutive lanes write consecutive words; dead once the moments
     // are summed, so they share their space with the node assembly's partials.
     union Stage {
         struct {
-            double x[kS], y[kS];
-            float t[5][kS];
+            double x[kT], y[kT];
+            float t[5][kT];
         } item;
         float part[3][8][kTile * kTile];
     };
