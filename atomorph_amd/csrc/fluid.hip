@@ -300,18 +300,20 @@ struct Bins {
     int32_t *rank;      // [4][n]: place of particle i's j-th entry in its bin
     int32_t *list;      // [<= 4n] slot | kHome, grouped by bin
     // Work items of kT entries at most, each one workgroup (chunk NULL:
-    // one workgroup per tile).  A dense bin is split over several items,
-    // which add their node sums with atomics to nodes cleared when the bins
-    // were placed; a tile with one item stores its nodes.  Only tiles with
-    // entries get items, plus one (empty) item for each tile that had
-    // entries in the step before and has none now, which stores zeros: a
-    // tile empty in both steps keeps the zeros stored then.
-    // chunk[c] = (tile, first entry, end entry, only item of its tile),
-    // c < *nchunks, in no particular order (appended with atomics).
+    // whole bins, one workgroup per tile, from start[]).  A dense bin is
+    // split over several items, which add their node sums with atomics to
+    // nodes cleared when the bins were placed; a tile with one item stores
+    // its nodes.  Only tiles with entries get items, plus one (empty) item
+    // for each tile that had entries in the step before and has none now,
+    // which stores zeros: a tile empty in both steps keeps the zeros stored
+    // then.  chunk[b] = (tile, entries, only item of its tile, 0), b <
+    // *nchunks, in tile order; item b's entries are list[kT * b ...], so a
+    // tile kernel loads its entry together with the descriptor.
     int4 *chunk;
-    int32_t *nchunks;       // this step's item count (appended by the placing)
-    int32_t *nchunks_next;  // the next step's, cleared by the placing
-    int32_t max_chunks;     // capacity of chunk (an upper bound of *nchunks)
+    int32_t *nchunks;     // this step's item count
+    int32_t max_chunks;   // capacity of chunk (an upper bound of *nchunks)
+    int32_t *items;       // [tiles] items per tile (scratch of grids beyond kLdsTiles)
+    int32_t par;          // this step's count parity (the other is cleared by k_p2g)
 };
 constexpr int kLdsTiles = 4096;  // up to this many tiles the placing blocks scan the counts in LDS
 
@@ -429,6 +431,34 @@ __global__ __launch_bounds__(kT) void k_bin_count(Tiles tiles_of, int n, Bins B,
     count_entries(act, tx0, tx1, ty0, ty1, i, n, B, par);
 }
 
+// Exclusive scan of value(k) over tiles k < tiles into out[0..tiles] (out[tiles]
+// = total) by the whole block in LDS: thread t takes a run of tiles.  value
+// is called once per tile, by the thread of its run, in tile order.
+template <class Value>
+__device__ __forceinline__ void block_scan_tiles(int tiles, const Value &value, int *out, int *wsum) {
+    const int per = (tiles + kT - 1) / kT;
+    const int lo = min(int(threadIdx.x) * per, tiles), hi = min(lo + per, tiles);
+    int sum = 0;
+    for (int k = lo; k < hi; ++k) {
+        const int v = value(k);
+        out[k] = sum;
+        sum += v;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int off = incl - sum;
+    for (int w = 0; w < wave; ++w) off += wsum[w];
+    for (int k = lo; k < hi; ++k) out[k] += off;
+    if (threadIdx.x == kT - 1) out[tiles] = off + sum;
+    __syncthreads();
+}
+
 // Work items of a tile with c entries that had `before` entries in the step
 // before: ceil(c / kT) chunks, or one empty item (which zeroes the tile's
 // nodes) when the tile has just emptied, or none.
@@ -436,17 +466,16 @@ __device__ __forceinline__ int work_items(int c, int before) {
     return c > 0 ? (c + kT - 1) / kT : (before > 0 ? 1 : 0);
 }
 
-// The nc work items of tile t (entries [b0, b1)) at B.chunk[c0 ...].
-__device__ __forceinline__ void put_work_items(const Bins &B, int c0, int nc, int t, int b0, int b1) {
-    for (int j = 0; j < nc; ++j)
-        B.chunk[c0 + j] = make_int4(t, b0 + j * kT, nc == 1 ? b1 : min(b1, b0 + (j + 1) * kT), nc == 1);
+// The nc work items of tile t (c entries) at B.chunk[c0 ...].
+__device__ __forceinline__ void put_items(const Bins &B, int c0, int nc, int t, int c) {
+    for (int j = 0; j < nc; ++j) B.chunk[c0 + j] = make_int4(t, max(0, min(kT, c - j * kT)), nc == 1, 0);
 }
 
-// Particles into their bins (the tiles recomputed from the unchanged
-// positions) and the other parity's counts cleared for the next step.  With
-// kLds every block scans the counts into LDS (block 0 also stores them in
-// B.start); otherwise B.start holds the scan already.  The grid covers
-// max(n, tiles) threads.
+// Particles into whole bins (the splat's pixel tiles; the tiles recomputed
+// from the unchanged positions) and the other parity's counts cleared for
+// the next use.  With kLds every block scans the counts into LDS (block 0
+// also stores them in B.start); otherwise B.start holds the scan already.
+// The grid covers max(n, tiles) threads.
 template <bool kLds, class Tiles>
 __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B, int tiles, int par) {
     __shared__ int sstart[kLds ? kLdsTiles + 1 : 1];
@@ -461,79 +490,12 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
     int rank[4];
     for (int j = 0; j < 4; ++j) rank[j] = B.rank[int64_t(j) * n + ic];
     if (kLds) {
-        const int per = (tiles + kT - 1) / kT;
-        static_assert(kLdsTiles / kT <= 32, "a thread's tiles fit a 32-bit mask");
-        const int lo = min(int(threadIdx.x) * per, tiles), hi = min(lo + per, tiles);
-        int sum = 0;
-        uint32_t before = 0;  // block 0: bit k - lo set when tile k had entries in the step before
-        const bool items = B.chunk && blockIdx.x == 0;
-        for (int k = lo; k < hi; ++k) {
-            const int c = B.count[par][k], b = items ? B.count[par ^ 1][k] : 0;
-            sstart[k] = sum;
-            sum += c;
-            before |= (b > 0 ? 1u : 0u) << (k - lo);
-        }
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        int incl = sum;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int v = __shfl_up(incl, off, 64);
-            if (lane >= off) incl += v;
-        }
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
-        int off = incl - sum;
-        for (int w = 0; w < wave; ++w) off += wsum[w];
-        for (int k = lo; k < hi; ++k) sstart[k] += off;
-        if (threadIdx.x == kT - 1) sstart[tiles] = off + sum;
-        __syncthreads();
+        block_scan_tiles(tiles, [&](int k) { return B.count[par][k]; }, sstart, wsum);
         if (blockIdx.x == 0)
             for (int k = threadIdx.x; k <= tiles; k += kT) B.start[k] = sstart[k];
         start = sstart;
-        if (blockIdx.x == 0) {
-            // block 0, thread by thread over its tiles lo..hi: the work items
-            // in tile order (their counts scanned like the entry counts),
-            // then the other parity's counts cleared for the next step
-            int csum = 0;
-            if (B.chunk)
-                for (int k = lo; k < hi; ++k) csum += work_items(sstart[k + 1] - sstart[k], (before >> (k - lo)) & 1);
-            int cincl = csum;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int v = __shfl_up(cincl, o, 64);
-                if (lane >= o) cincl += v;
-            }
-            __syncthreads();
-            if (lane == 63) wsum[wave] = cincl;
-            __syncthreads();
-            int c0 = cincl - csum;
-            for (int w = 0; w < wave; ++w) c0 += wsum[w];
-            for (int k = lo; k < hi; ++k) {
-                if (B.chunk) {
-                    const int b0 = sstart[k], b1 = sstart[k + 1];
-                    const int nc = work_items(b1 - b0, (before >> (k - lo)) & 1);
-                    put_work_items(B, c0, nc, k, b0, b1);
-                    c0 += nc;
-                }
-                B.count[par ^ 1][k] = 0;
-            }
-            if (B.chunk && threadIdx.x == kT - 1) *B.nchunks = c0;
-        }
-    } else {
-        // one thread per tile: its work items appended (in no order), then
-        // the other parity's count cleared for the next step
-        for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) {
-            if (B.chunk) {
-                const int b0 = start[k], b1 = start[k + 1];
-                const int nc = work_items(b1 - b0, B.count[par ^ 1][k]);
-                if (nc > 0) put_work_items(B, atomicAdd(B.nchunks, nc), nc, k, b0, b1);
-            }
-            B.count[par ^ 1][k] = 0;
-        }
-        if (B.chunk && blockIdx.x == 0 && threadIdx.x == 0) *B.nchunks_next = 0;
     }
-    if (B.chunk) {  // nodes of tiles split over several items start from zero
-        for (int t = blockIdx.x; t < tiles; t += gridDim.x)
-            if (start[t + 1] - start[t] > kT) tiles_of.clear_tile(t, B.nty);
-    }
+    for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) B.count[par ^ 1][k] = 0;
     if (!binned) return;
     for (int j = 0; j < 4; ++j) {
         const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
@@ -541,23 +503,70 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
     }
 }
 
-// Calls body(slot, home, item) for the entries [s0, s1), kT at a time (slot
-// -1 past the end), item = fetch(slot) holding the particle fields the body
-// reads.  kPrefetch: the next kT entries and their fields are loaded before
-// the current ones are processed, so a whole dense bin overlaps its memory
-// latency; without it (work items of at most kT entries, where the loads of
-// the next item belong to another workgroup) the registers of the second
-// item are saved.  Trip counts are uniform over the block.
-struct NoPrologue {
-    __device__ void operator()() const {}
-};
+// Work items per tile from its count and the step before's (grids beyond
+// kLdsTiles tiles, before the device scan of B.items into B.start).
+__global__ __launch_bounds__(kT) void k_tile_items(Bins B, int tiles, int par) {
+    const int k = blockIdx.x * kT + threadIdx.x;
+    if (k < tiles) B.items[k] = work_items(B.count[par][k], B.count[par ^ 1][k]);
+}
 
-// prologue() runs once the first chunk's loads are issued and before it is
-// processed, so its own memory traffic (a node halo staged in LDS, with the
-// barrier that publishes it) overlaps them.
-template <bool kPrefetch = true, class Fetch, class Body, class Prologue = NoPrologue>
-__device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fetch &fetch, const Body &body,
-                                        const Prologue &prologue = Prologue()) {
+// The fluid steps' bins: tile t's entries are laid out from list[kT *
+// first[t]], kT per work item (the last item of a tile partly filled), so
+// that item b's entries sit at list[kT * b ...].  first[] is the scan of
+// the tiles' item counts: with kLds every block scans them in LDS (block 0
+// also writes the item descriptors and their number); otherwise B.start
+// holds it already (k_tile_items and the device scan).  The step before's
+// counts, which every placing block reads here, are cleared by k_p2g.  The
+// grid covers max(n, tiles) threads.
+template <bool kLds>
+__global__ __launch_bounds__(kT) void k_item_place(StencilTiles tiles_of, int n, Bins B, int tiles, int par) {
+    __shared__ int sfirst[kLds ? kLdsTiles + 1 : 1];
+    __shared__ int scount[kLds ? kLdsTiles : 1];
+    __shared__ int wsum[kT / 64];
+    const int32_t *first = B.start;
+    // the particle and its entries' ranks first, as in k_bin_place
+    const int i = blockIdx.x * kT + threadIdx.x;
+    const int ic = i < n ? i : 0;
+    int tx0, tx1, ty0, ty1;
+    const bool binned = tiles_of(ic, tx0, tx1, ty0, ty1) && i < n;
+    int rank[4];
+    for (int j = 0; j < 4; ++j) rank[j] = B.rank[int64_t(j) * n + ic];
+    if (kLds) {
+        block_scan_tiles(tiles, [&](int k) {
+            const int c = B.count[par][k], before = B.count[par ^ 1][k];
+            scount[k] = c;
+            return work_items(c, before);
+        }, sfirst, wsum);
+        first = sfirst;
+        if (blockIdx.x == 0) {
+            for (int k = threadIdx.x; k < tiles; k += kT) put_items(B, sfirst[k], sfirst[k + 1] - sfirst[k], k, scount[k]);
+            if (threadIdx.x == 0) *B.nchunks = sfirst[tiles];
+        }
+        // nodes of tiles split over several items start from zero
+        for (int t = blockIdx.x; t < tiles; t += gridDim.x)
+            if (scount[t] > kT) tiles_of.clear_tile(t, B.nty);
+    } else {
+        for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT)
+            put_items(B, first[k], first[k + 1] - first[k], k, B.count[par][k]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *B.nchunks = first[tiles];
+        for (int t = blockIdx.x; t < tiles; t += gridDim.x)
+            if (B.count[par][t] > kT) tiles_of.clear_tile(t, B.nty);
+    }
+    if (!binned) return;
+    for (int j = 0; j < 4; ++j) {
+        const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
+        if (t >= 0) B.list[int64_t(first[t]) * kT + rank[j]] = i | (j == 0 ? kHome : 0);
+    }
+}
+
+// Calls body(slot, home, item) for the entries [s0, s1) of a whole bin (the
+// splat's pixel tiles), kT at a time (slot -1 past the end), item =
+// fetch(slot) holding the particle fields the body reads.  The next kT
+// entries and their fields are loaded before the current ones are
+// processed, so a dense bin overlaps its memory latency.  Trip counts are
+// uniform over the block.
+template <class Fetch, class Body>
+__device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fetch &fetch, const Body &body) {
     auto entry = [&](int base, int &slot, bool &home) {
         const int k = base + int(threadIdx.x);
         slot = -1;
@@ -568,29 +577,10 @@ __device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fet
             home = e < 0;
         }
     };
-    if constexpr (!kPrefetch) {
-        // one call site of the prologue, after the first entries' loads are
-        // issued (a prologue on two paths has its common part hoisted above
-        // the bin list's load)
-        int slot;
-        bool home;
-        entry(s0, slot, home);
-        auto item = fetch(slot);
-        prologue();
-        for (int base = s0; base < s1; base += kT) {
-            if (base > s0) {
-                entry(base, slot, home);
-                item = fetch(slot);
-            }
-            body(slot, home, item);
-        }
-        return;
-    }
     int slot, nslot = -1;
     bool home, nhome = false;
     entry(s0, slot, home);
     auto item = fetch(slot);
-    prologue();
     for (int base = s0; base < s1; base += kT) {
         auto next = item;
         if (base + kT < s1) {
@@ -604,22 +594,39 @@ __device__ __forceinline__ void for_bin(const Bins &B, int s0, int s1, const Fet
     }
 }
 
-// Work item b of a tile kernel: the entries [s0, s1) of tile t (at most
-// kT) and whether it is the tile's only work item (then it stores its
-// nodes; otherwise it adds them atomically).  false: past the last item.
+// Work item b of a tile kernel: tile t, whether it is the tile's only item
+// (then it stores its nodes; otherwise it adds them atomically), and this
+// thread's entry of it (slot -1 past the item's entries, home: the
+// particle's home tile).  The item's entries are list[kT * b ...], so the
+// entry is loaded together with the descriptor, not after it (both reads
+// are in bounds for every b: b is clamped to the descriptors' capacity and
+// the list holds kT entries per descriptor).  false: past the last item.
 // Workgroups take items b = blockIdx.x, blockIdx.x + gridDim.x, ...;
 // nchunks = *B.nchunks.
-__device__ __forceinline__ bool block_work(const Bins &B, int nchunks, int b, int &t, int &s0, int &s1,
-                                           bool &single) {
-    // loaded before the count is compared, so the two loads overlap (the
-    // grid never exceeds max_chunks; later items clamp their index)
-    const int4 c = B.chunk[min(b, B.max_chunks - 1)];
+__device__ __forceinline__ bool tile_item(const Bins &B, int nchunks, int b, int &t, bool &single, int &slot,
+                                          bool &home) {
+    const int bc = min(b, B.max_chunks - 1);
+    const int4 c = B.chunk[bc];
+    const int e = B.list[int64_t(bc) * kT + threadIdx.x];
     if (b >= nchunks) return false;
     t = c.x;
-    s0 = c.y;
-    s1 = c.z;
-    single = c.w != 0;
+    single = c.z != 0;
+    const bool has = int(threadIdx.x) < c.y;
+    slot = has ? (e & 0x7FFFFFFF) : -1;
+    home = has && e < 0;
     return true;
+}
+
+// body(slot, home, item) on this thread's entry, item = fetch(slot) holding
+// the particle fields the body reads; prologue() runs once those loads are
+// issued and before they are used, so its own memory traffic (a node halo
+// staged in LDS, with the barrier that publishes it) overlaps them.
+template <class Fetch, class Body, class Prologue>
+__device__ __forceinline__ void run_item(int slot, bool home, const Fetch &fetch, const Body &body,
+                                         const Prologue &prologue) {
+    const auto item = fetch(slot);
+    prologue();
+    body(slot, home, item);
 }
 
 __device__ __forceinline__ void put_node(float *f, int64_t k, float v, bool single) {
@@ -684,9 +691,10 @@ struct P2gItem {
 // of lanes on one cell (summed with DPP before one LDS atomic per run)
 // shrink to about one lane and every lane pays its own atomics.
 //
-// Seven waves per SIMD: the LDS (18 KB) allows eight blocks per CU, and
-// without the bound the node assembly's scheduling takes 90 VGPRs (five).
-__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_p2g(Particles P, Grid Gd, int gsx,
+// Six waves per SIMD (79 VGPRs): the LDS (18 KB) allows eight blocks per
+// CU; seven waves spill 12 bytes per lane, which measured slower, and
+// without a bound the node assembly's scheduling takes 95 VGPRs (five).
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(6))) void k_p2g(Particles P, Grid Gd, int gsx,
                                                                                    int gsy, Bins B) {
     __shared__ float cs[kCellSums][kReach * kReach];
     constexpr int kCells = kReach * kReach;  // local cells; bucket kCells: out of reach
@@ -706,14 +714,18 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_
     __shared__ Stage stage;
     static_assert(kCells + 2 <= 2 * 64, "one wave scans the buckets, two per lane");
     const int nchunks = *B.nchunks;
+    // the step before's counts, read by every placing block, are cleared
+    // here for the next step's counting
+    for (int k = blockIdx.x * kT + threadIdx.x; k < B.ntx * B.nty; k += gridDim.x * kT) B.count[B.par ^ 1][k] = 0;
     for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
-        int tile, s0, s1;
-        bool single;
-        if (!block_work(B, nchunks, item, tile, s0, s1, single)) break;
+        int tile, slot0;
+        bool single, home0;
+        if (!tile_item(B, nchunks, item, tile, single, slot0, home0)) break;
+        const int64_t s0 = int64_t(item) * kT;  // the item's entries in the list
         const int tx = tile / B.nty, ty = tile % B.nty;
         const int x0 = tx * kTile, y0 = ty * kTile;
         // every field load is unconditional (slot -1, past the end of the
-        // bin, reads particle 0 and is masked), so the eight loads are issued
+        // item, reads particle 0 and is masked), so the eight loads are issued
         // together, and the sums are cleared while they are in flight
         auto fetch = [&](int slot) {
             const int q = slot >= 0 ? slot : 0;
@@ -758,7 +770,7 @@ __global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(7))) void k_
         };
         {
             // at most kT entries: one pass, ranks fit in 8 bits
-            for_bin<false>(B, s0, s1, fetch, [&](int slot, bool home, const P2gItem &it) {
+            run_item(slot0, home0, fetch, [&](int slot, bool home, const P2gItem &it) {
                 const int cx = static_cast<int>(it.x - 0.5), cy = static_cast<int>(it.y - 0.5);
                 const int bucket = in_reach(cx, cy, x0, y0) ? (cx - x0 + 2) * kReach + (cy - y0 + 2) : kCells;
                 if (slot >= 0) {
@@ -876,14 +888,14 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
     __shared__ float nodes[3][kHalo * kHalo];  // m, gx, gy
     const int nchunks = *B.nchunks;
     for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
-        int tile, s0, s1;
-        bool single;
-        if (!block_work(B, nchunks, item, tile, s0, s1, single)) break;
+        int tile, slot0;
+        bool single, home0;
+        if (!tile_item(B, nchunks, item, tile, single, slot0, home0)) break;
         const int tx = tile / B.nty, ty = tile % B.nty;
         const int x0 = tx * kTile, y0 = ty * kTile;
         for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
-        // the halo's loads are issued before the bin list's and the
-        // particles', which are unconditional (slot -1 reads particle 0)
+        // the halo's loads are issued beside the particles', which are
+        // unconditional (slot -1 reads particle 0)
         Halo<3> halo;
         halo.load(x0, y0, gsx, gsy,
                   [&](int f, int64_t k) { return f == 0 ? Gd.muvw[4 * k] : f == 1 ? Gd.gx[k] : Gd.gy[k]; });
@@ -896,7 +908,7 @@ __global__ __launch_bounds__(kT) void k_forces(Particles P, Grid Gd, int gsx, in
             const int q = slot >= 0 ? slot : 0;
             return double2{P.x[q], P.y[q]};
         };
-        for_bin<false>(B, s0, s1, fetch, [&](int slot, bool, const double2 &it) {
+        run_item(slot0, home0, fetch, [&](int slot, bool, const double2 &it) {
             int cx, cy;
             double px[3], py[3], gx[3], gy[3];
             double pressure = 0.0, fx = 0.0, fy = 0.0;
@@ -932,13 +944,13 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
     __shared__ float accel[2][kHalo * kHalo];  // node accelerations per mass
     const int nchunks = *B.nchunks;
     for (int item = int(blockIdx.x);; item += int(gridDim.x)) {
-        int tile, s0, s1;
-        bool single;
-        if (!block_work(B, nchunks, item, tile, s0, s1, single)) break;
+        int tile, slot0;
+        bool single, home0;
+        if (!tile_item(B, nchunks, item, tile, single, slot0, home0)) break;
         const int tx = tile / B.nty, ty = tile % B.nty;
         const int x0 = tx * kTile, y0 = ty * kTile;
         for (int k = threadIdx.x; k < 2 * kTile * kTile; k += kT) (&acc[0][0])[k] = 0.0f;
-        // as in k_forces: halo loads first, unconditional particle loads
+        // as in k_forces: halo loads beside the unconditional particle loads
         Halo<3> halo;
         halo.load(x0, y0, gsx, gsy,
                   [&](int f, int64_t k) { return f == 0 ? Gd.ax[k] : f == 1 ? Gd.ay[k] : Gd.muvw[4 * k]; });
@@ -950,7 +962,7 @@ __global__ __launch_bounds__(kT) void k_g2p(Particles P, Grid Gd, int gsx, int g
             const int q = slot >= 0 ? slot : 0;
             return G2pItem{P.x[q], P.y[q], P.gx[q], P.gy[q], P.u[q], P.v[q], P.mature[q]};
         };
-        for_bin<false>(B, s0, s1, fetch, [&](int slot, bool home, const G2pItem &it) {
+        run_item(slot0, home0, fetch, [&](int slot, bool home, const G2pItem &it) {
             int cx, cy;
             double px[3], py[3], gx[3], gy[3];
             const double x = it.x, y = it.y;
@@ -1264,6 +1276,9 @@ namespace {
 // Upper bound of the chunks: every tile has at least one, and a particle
 // has at most 4 entries.
 size_t max_chunks(size_t tiles, size_t n) { return tiles + (4 * n + kT - 1) / kT + 1; }
+// Entries of the list: kT per work item (the steps' items start on kT
+// boundaries), at least the 4n of whole bins (the splat shares the list).
+size_t list_entries(size_t tiles, size_t n) { return std::max(4 * n, size_t(kT) * max_chunks(tiles, n)); }
 
 size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     const size_t n = size_t(count), nn = size_t(gsx) * gsy, npx = size_t(W) * H;
@@ -1271,10 +1286,11 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     const size_t tiles = size_t((gsx + kTile - 1) / kTile) * size_t((gsy + kTile - 1) / kTile);
     const size_t ptiles = size_t((W + kPxTile - 1) / kPxTile) * size_t((H + kPxTile - 1) / kPxTile);
     (void)nkeys;
+    // every carved array is rounded up to 256 bytes: the last term covers that
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
-           (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 8 + (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) +
-           max_chunks(tiles, n) * 16 + 8 + 3 * 256 +
-           nn * 4 * kGridFields + 6 * 256 + npx * 4 * 2 + 64 * 256 + 256;
+           (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 4 + list_entries(tiles, n) * 4 + tiles * 4 +
+           (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) + max_chunks(tiles, n) * 16 + 4 + nn * 4 * kGridFields +
+           npx * 4 * 2 + 256 + 128 * 256;
 }
 
 void carve_particles(Particles &Q, size_t n, const std::function<char *(size_t)> &take) {
@@ -1326,10 +1342,11 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->bins.count[1] = reinterpret_cast<int32_t *>(take(tiles * 4));
     f->bins.start = reinterpret_cast<int32_t *>(take((tiles + 1) * 4));
     f->bins.rank = reinterpret_cast<int32_t *>(take(n * 4 * 4));
-    f->bins.list = reinterpret_cast<int32_t *>(take(n * 4 * 4));
     f->bins.max_chunks = int32_t(max_chunks(tiles, n));
+    f->bins.list = reinterpret_cast<int32_t *>(take(list_entries(tiles, n) * 4));
     f->bins.chunk = reinterpret_cast<int4 *>(take(size_t(f->bins.max_chunks) * 16));
-    f->bins.nchunks = reinterpret_cast<int32_t *>(take(8));  // one count per step parity
+    f->bins.nchunks = reinterpret_cast<int32_t *>(take(4));
+    f->bins.items = reinterpret_cast<int32_t *>(take(tiles * 4));
     f->pbins.ntx = (width + kPxTile - 1) / kPxTile;
     f->pbins.nty = (height + kPxTile - 1) / kPxTile;
     const size_t ptiles = size_t(f->pbins.ntx) * size_t(f->pbins.nty);
@@ -1340,7 +1357,7 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->pbins.list = f->bins.list;
     f->pbins.chunk = nullptr;  // one workgroup per pixel tile
     f->pbins.nchunks = nullptr;
-    f->pbins.nchunks_next = nullptr;
+    f->pbins.items = nullptr;
     f->pbins.max_chunks = 0;
     f->sxy = reinterpret_cast<uint32_t *>(take(n * 4));
     f->staps = reinterpret_cast<double *>(take(n * 32));
@@ -1352,6 +1369,13 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->mask = reinterpret_cast<int32_t *>(take(npx * 4));
     f->img = reinterpret_cast<uint32_t *>(take(npx * 4));
     f->fp = reinterpret_cast<FrameParams *>(take(sizeof(FrameParams)));
+    if (size_t(p - static_cast<char *>(f->buf)) > bytes) {
+        set_error("amh_fluid_create: layout of %zu bytes exceeds its allocation of %zu",
+                  size_t(p - static_cast<char *>(f->buf)), bytes);
+        (void)hipFree(f->buf);
+        delete f;
+        return AMH_E_ARG;
+    }
     // cells ranked in Morton (Z) order of (x, y): a run of consecutive ranks
     // is a compact patch of the grid
     std::vector<int32_t> by_code(nn), rank(nn);
@@ -1449,23 +1473,37 @@ namespace {
 // Counting sort of the particles' entries into the bins of B (tiles bins,
 // counts of parity par; the other parity is cleared for the next use).
 // counted: the counts and ranks are in place already.
-// Beyond kLdsTiles tiles the counts are scanned first: by the three-launch
-// device scan when workspace (sums[scan::blocks(tiles)], offs[+1]) is
-// given, else by one workgroup.
+// Whole bins (the splat's pixel tiles).  Beyond kLdsTiles tiles the counts
+// are scanned first by one workgroup.
 template <class Tiles>
 void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s,
-                   bool counted = false, int *sums = nullptr, int *offs = nullptr) {
+                   bool counted = false) {
     const uint32_t gp = uint32_t((n + kT - 1) / kT);
     if (!counted) hipLaunchKernelGGL(k_bin_count<Tiles>, dim3(gp), dim3(kT), 0, s, tiles_of, n, B, par);
     const uint32_t gb = uint32_t((std::max(n, tiles) + kT - 1) / kT);
     if (tiles <= kLdsTiles) {
         hipLaunchKernelGGL((k_bin_place<true, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
     } else {
-        if (sums && offs)
-            scan::exclusive(B.count[par], tiles, sums, offs, B.start, s);
-        else
-            hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count[par], tiles, B.start);
+        hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count[par], tiles, B.start);
         hipLaunchKernelGGL((k_bin_place<false, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
+    }
+}
+
+// The fluid steps' bins and work items (counted: the counts and ranks are in
+// place already).  Beyond kLdsTiles tiles the tiles' item counts are
+// scanned first, by the three-launch device scan on the workspace sums,
+// offs (scan::blocks(tiles) and one more).
+void place_items(const StencilTiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s,
+                 bool counted, int *sums, int *offs) {
+    const uint32_t gp = uint32_t((n + kT - 1) / kT);
+    if (!counted) hipLaunchKernelGGL(k_bin_count<StencilTiles>, dim3(gp), dim3(kT), 0, s, tiles_of, n, B, par);
+    const uint32_t gb = uint32_t((std::max(n, tiles) + kT - 1) / kT);
+    if (tiles <= kLdsTiles) {
+        hipLaunchKernelGGL(k_item_place<true>, dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
+    } else {
+        hipLaunchKernelGGL(k_tile_items, dim3(uint32_t((tiles + kT - 1) / kT)), dim3(kT), 0, s, B, tiles, par);
+        scan::exclusive(B.items, tiles, sums, offs, B.start, s);
+        hipLaunchKernelGGL(k_item_place<false>, dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
     }
 }
 
@@ -1479,20 +1517,19 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     amh::ProfScope step_("fluid_step", s);
     if (f->count == 0) return AMH_OK;
     const int tiles = f->bins.ntx * f->bins.nty;
-    // the work items of this step's parity (the counts alternate the same way)
+    // this step's count parity: the counts alternate from step to step
     Bins B = f->bins;
     const int par = f->bin_par;
     f->bin_par ^= 1;
-    B.nchunks = f->bins.nchunks + par;
-    B.nchunks_next = f->bins.nchunks + (par ^ 1);
+    B.par = par;
     // about one work item per workgroup (a bin averages under one chunk of
     // kT entries, and empty tiles get none); a workgroup takes further items
     // when there are more
     const uint32_t grid = uint32_t(std::min(B.max_chunks, 2 * std::min(tiles, kLdsTiles)));
     {
         amh::ProfScope ps_("fluid_bins", s);
-        bin_particles(StencilTiles{Q, f->gsx, f->gsy, f->G}, f->count, B, tiles, par, s, counted, f->scan_sums,
-                      f->scan_offs);
+        place_items(StencilTiles{Q, f->gsx, f->gsy, f->G}, f->count, B, tiles, par, s, counted, f->scan_sums,
+                    f->scan_offs);
     }
     amh::launch_timed("k_p2g", k_p2g, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     amh::launch_timed("k_forces", k_forces, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
