@@ -1192,10 +1192,13 @@ __global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, 
 }
 
 // Slot s of particle i in the cell order: S.atom[s] = i, and slot[i] = s
-// (slot may alias pos: each thread reads its own entry first).
+// (slot may alias pos: each thread reads its own entry first).  The
+// buckets' counts, scanned already, are cleared for the next frame's sort.
 __global__ __launch_bounds__(kT) void k_sort_slots(int n, const int32_t *__restrict__ cell, const int32_t *pos,
-                                                  const int32_t *__restrict__ start, int32_t *slot, Particles S) {
+                                                  const int32_t *__restrict__ start, int32_t *slot, Particles S,
+                                                  int32_t *__restrict__ cnt, int64_t nk) {
     const int i = blockIdx.x * kT + threadIdx.x;
+    for (int64_t k = i; k < nk; k += int64_t(gridDim.x) * kT) cnt[k] = 0;
     if (i >= n) return;
     const int s = start[cell[i]] + pos[i];
     S.atom[s] = i;
@@ -1204,20 +1207,34 @@ __global__ __launch_bounds__(kT) void k_sort_slots(int n, const int32_t *__restr
 
 // S = P in cell order, one thread per slot: the 18 fields are written
 // coalesced and read through the slot's particle (a scattered store per
-// field moved whole cache lines for 8 bytes).
-__global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, Particles S) {
+// field moved whole cache lines for 8 bytes).  par >= 0: the frame's first
+// step's bin entries are counted too (as k_bin_count does, into B.count[par]
+// and B.rank), from the position just gathered.
+__global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, Particles S, int gsx, int gsy, Bins B,
+                                                   int par) {
     const int s = blockIdx.x * kT + threadIdx.x;
-    if (s >= n) return;
-    const int i = S.atom[s];
-    // one field at a time: with all 18 random gathers of a wave in flight at
-    // once the L2 held fewer of their lines (36 against 30 us per frame)
-    S.x[s] = P.x[i]; S.y[s] = P.y[i]; S.u[s] = P.u[i]; S.v[s] = P.v[i];
-    S.gx[s] = P.gx[i]; S.gy[s] = P.gy[i]; S.fr[s] = P.fr[i];
-    S.R[s] = P.R[i]; S.G[s] = P.G[i]; S.B[s] = P.B[i]; S.A[s] = P.A[i];
-    S.r[s] = P.r[i]; S.g[s] = P.g[i]; S.b[s] = P.b[i]; S.a[s] = P.a[i];
-    S.strength[s] = P.strength[i];
-    S.active[s] = P.active[i];
-    S.mature[s] = P.mature[i];
+    const bool in = s < n;
+    int act = 0;
+    double x = 0.0, y = 0.0;
+    if (in) {
+        const int i = S.atom[s];
+        // one field at a time: with all 18 random gathers of a wave in flight at
+        // once the L2 held fewer of their lines (36 against 30 us per frame)
+        x = P.x[i];
+        y = P.y[i];
+        S.x[s] = x; S.y[s] = y; S.u[s] = P.u[i]; S.v[s] = P.v[i];
+        S.gx[s] = P.gx[i]; S.gy[s] = P.gy[i]; S.fr[s] = P.fr[i];
+        S.R[s] = P.R[i]; S.G[s] = P.G[i]; S.B[s] = P.B[i]; S.A[s] = P.A[i];
+        S.r[s] = P.r[i]; S.g[s] = P.g[i]; S.b[s] = P.b[i]; S.a[s] = P.a[i];
+        S.strength[s] = P.strength[i];
+        act = P.active[i];
+        S.active[s] = act;
+        S.mature[s] = P.mature[i];
+    }
+    if (par < 0) return;  // uniform: no step follows
+    int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    if (in) stencil_tiles(x, y, gsx, gsy, tx0, tx1, ty0, ty1);
+    count_entries(in && act != 0, tx0, tx1, ty0, ty1, s, n, B, par);  // every lane of the wave
 }
 
 // What the steps and the splat change goes back to particle order, one
@@ -1541,19 +1558,23 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     return AMH_OK;
 }
 
-// f->S = f->P in cell order (f->S.atom maps slots back to particles).
-int sort_particles(amh_fluid_s *f, hipStream_t s) {
+// f->S = f->P in cell order (f->S.atom maps slots back to particles).  The
+// bucket counts f->cnt are zero on entry (from the handle's creation, then
+// from the last sort's k_sort_slots).  count_first: the first step's bins
+// are counted on the way (its parity f->bin_par).
+int sort_particles(amh_fluid_s *f, hipStream_t s, bool count_first) {
     amh::ProfScope ps_("fluid_sort", s);
     const int64_t nn = int64_t(f->gsx) * f->gsy;
     const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
     const int64_t nk = nn + kIdleBuckets;
-    hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((nk + kT - 1) / kT)), dim3(kT), 0, s, f->cnt, nk);
     hipLaunchKernelGGL(k_sort_count, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->gsx, f->gsy, f->cell_rank,
                        f->cell, f->pos, f->cnt);
     scan::exclusive(f->cnt, nk, f->scan_sums, f->scan_offs, f->start, s);
     // f->pos becomes each particle's slot (k_sort_scatter reads it after the steps)
-    hipLaunchKernelGGL(k_sort_slots, dim3(gp), dim3(kT), 0, s, f->count, f->cell, f->pos, f->start, f->pos, f->S);
-    hipLaunchKernelGGL(k_sort_gather, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->S);
+    hipLaunchKernelGGL(k_sort_slots, dim3(gp), dim3(kT), 0, s, f->count, f->cell, f->pos, f->start, f->pos, f->S,
+                       f->cnt, nk);
+    hipLaunchKernelGGL(k_sort_gather, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->S, f->gsx, f->gsy, f->bins,
+                       count_first ? f->bin_par : -1);
     AMH_LAUNCH_CHECK("fluid sort");
     return AMH_OK;
 }
@@ -1996,9 +2017,10 @@ int enqueue_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
         amh::launch_timed("k_fluid_update", k_fluid_update, dim3(gp), dim3(kT), 0, s, R, f->fp, f->P, fa->rank_dev,
                           fa->surface_dev, fa->blob_col_dev, fa->sparseness);
         // the steps and the splat run on the cell-ordered copy
-        if (int r = sort_particles(f, s)) return r;
+        // the sort counts the first step's bins; each step's advection the next's
+        if (int r = sort_particles(f, s, fa->fluidsteps > 0)) return r;
         for (int k = 0; k < fa->fluidsteps; ++k) {
-            const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), 0.0, 0, s, k > 0,
+            const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), 0.0, 0, s, true,
                                         k + 1 < fa->fluidsteps, f->fp, k);
             if (r) return r;
         }
@@ -2147,7 +2169,7 @@ extern "C" int amh_fluid_draw(void *fluid, const amh_fluid_frame *fa, double t, 
     v.out = out;
     hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, s, f->fp, v);
     if (v.F.fk < R.nkeys && f->count > 0) {
-        if (int r = sort_particles(f, s)) return r;
+        if (int r = sort_particles(f, s, false)) return r;  // no step follows: nothing to count
         draw_splat(f, R, fa->sparseness, s);
     } else {
         const int64_t npx = int64_t(f->W) * f->H;
