@@ -356,12 +356,15 @@ def test_fluid_draw_within_one_lsb_of_reference():
         lib.amh_fluid_destroy(fluid)
 
 
-def test_fluid_frames_replayed_from_a_graph_equal_eager_frames(golden_cfg1):
+@pytest.mark.parametrize("steps", [4, 3])
+def test_fluid_frames_replayed_from_a_graph_equal_eager_frames(golden_cfg1, steps):
     """amh_fluid_render on a stream of its own captures a frame's launches
     into a hipGraph once and replays it, the per-frame values coming from
     the parameter block: 12 frames (with other GPU work and host syncs
     between the replays) equal the same frames launched one by one within
-    1 LSB, and the particles agree within 1e-4."""
+    1 LSB, and the particles agree within 1e-4.  With an odd number of
+    steps per frame the bins' count parity alternates from frame to frame,
+    so the frames replay two graphs in turn."""
     g = golden_cfg1
     lib = G.lib()
     T = G.torch()
@@ -382,7 +385,7 @@ def test_fluid_frames_replayed_from_a_graph_equal_eager_frames(golden_cfg1):
             rank[y, o_ + np.lexsort((np.arange(w), cls))] = np.arange(w)
             surf[y, c] = int((f1 > 0).sum())
     keep = [G.to_dev(rank.ravel()), G.to_dev(surf.ravel())]
-    fa = A.FluidFrame(C.pointer(rend.args), keep[0].data_ptr(), keep[1].data_ptr(), None, 1.0, 4, 0)
+    fa = A.FluidFrame(C.pointer(rend.args), keep[0].data_ptr(), keep[1].data_ptr(), None, 1.0, steps, 0)
     stream = C.c_void_p()
     G.check(lib.amh_stream_create(C.byref(stream)), "stream")
     models = []
@@ -397,7 +400,7 @@ def test_fluid_frames_replayed_from_a_graph_equal_eager_frames(golden_cfg1):
         G.sync()
         for i in range(12):
             t = i / 12
-            G.log(f"fluid graph replay frame {i}")
+            G.log(f"fluid graph replay frame {i} ({steps} steps)")
             for f, o in zip(models, outs):
                 G.check(lib.amh_fluid_render(f, C.byref(fa), t, C.c_void_p(o.data_ptr() + 4 * 64 * 64 * i), stream),
                         "render")
