@@ -471,13 +471,19 @@ __device__ __forceinline__ void put_items(const Bins &B, int c0, int nc, int t, 
     for (int j = 0; j < nc; ++j) B.chunk[c0 + j] = make_int4(t, max(0, min(kT, c - j * kT)), nc == 1, 0);
 }
 
+// Nothing more for particle i (k_bin_place's default tail).
+struct NoTail {
+    __device__ void operator()(int) const {}
+};
+
 // Particles into whole bins (the splat's pixel tiles; the tiles recomputed
 // from the unchanged positions) and the other parity's counts cleared for
 // the next use.  With kLds every block scans the counts into LDS (block 0
 // also stores them in B.start); otherwise B.start holds the scan already.
-// The grid covers max(n, tiles) threads.
-template <bool kLds, class Tiles>
-__global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B, int tiles, int par) {
+// Then tail(i) for every particle i < n.  The grid covers max(n, tiles)
+// threads.
+template <bool kLds, class Tiles, class Tail>
+__global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B, int tiles, int par, Tail tail) {
     __shared__ int sstart[kLds ? kLdsTiles + 1 : 1];
     __shared__ int wsum[kT / 64];
     const int32_t *start = B.start;
@@ -496,11 +502,12 @@ __global__ __launch_bounds__(kT) void k_bin_place(Tiles tiles_of, int n, Bins B,
         start = sstart;
     }
     for (int k = blockIdx.x * kT + threadIdx.x; k < tiles; k += gridDim.x * kT) B.count[par ^ 1][k] = 0;
-    if (!binned) return;
-    for (int j = 0; j < 4; ++j) {
-        const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
-        if (t >= 0) B.list[start[t] + rank[j]] = i | (j == 0 ? kHome : 0);
-    }
+    if (binned)
+        for (int j = 0; j < 4; ++j) {
+            const int t = entry_tile(j, tx0, tx1, ty0, ty1, B.nty);
+            if (t >= 0) B.list[start[t] + rank[j]] = i | (j == 0 ? kHome : 0);
+        }
+    if (i < n) tail(i);
 }
 
 // Work items per tile from its count and the step before's (grids beyond
@@ -1161,14 +1168,15 @@ __device__ __forceinline__ int cell_of(double x, double y, int gsx, int gsy) {
 // counter, and those serialise.
 constexpr int kIdleBuckets = 256;
 
-__global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, int gsy,
-                                                  const int32_t *__restrict__ cell_rank, int32_t *__restrict__ cell,
-                                                  int32_t *__restrict__ pos, int32_t *__restrict__ cnt) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    const bool in = i < n;
-    const int nn = gsx * gsy;
-    const int idle = nn + int((uint32_t(i) >> 6) & uint32_t(kIdleBuckets - 1));
-    const int key = !in ? -1 : P.active[i] ? cell_rank[cell_of(P.x[i], P.y[i], gsx, gsy)] : idle;
+__device__ __forceinline__ int idle_bucket(int i, int nn) {
+    return nn + int((uint32_t(i) >> 6) & uint32_t(kIdleBuckets - 1));
+}
+
+// Particle i's sort bucket `key` (in: i < n) counted into cnt[key], its rank
+// among the bucket's particles into pos[i] and the key into cell[i].  Every
+// lane of the wave calls it.
+__device__ __forceinline__ void sort_rank(int key, bool in, int i, int32_t *__restrict__ cell,
+                                          int32_t *__restrict__ pos, int32_t *__restrict__ cnt) {
     // group the wave's lanes by key: one atomic per distinct key, all in flight together
     const int lane = threadIdx.x & 63;
     int lead = lane, pre = 0, total = 0;
@@ -1189,6 +1197,23 @@ __global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, 
     if (!in) return;
     cell[i] = key;
     pos[i] = b + pre;
+}
+
+// The cell sort's counting pass done by another kernel (k_sort_count's work): the
+// buckets, ranks and keys it writes.  cnt == nullptr: not counted.
+struct SortCount {
+    const int32_t *cell_rank;
+    int32_t *cell, *pos, *cnt;
+    int gsx, gsy;
+};
+
+__global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, int gsy,
+                                                  const int32_t *__restrict__ cell_rank, int32_t *__restrict__ cell,
+                                                  int32_t *__restrict__ pos, int32_t *__restrict__ cnt) {
+    const int i = blockIdx.x * kT + threadIdx.x;
+    const bool in = i < n;
+    const int key = !in ? -1 : P.active[i] ? cell_rank[cell_of(P.x[i], P.y[i], gsx, gsy)] : idle_bucket(i, gsx * gsy);
+    sort_rank(key, in, i, cell, pos, cnt);
 }
 
 // Slot s of particle i in the cell order: S.atom[s] = i, and slot[i] = s
@@ -1237,18 +1262,21 @@ __global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, Particle
     count_entries(in && act != 0, tx0, tx1, ty0, ty1, s, n, B, par);  // every lane of the wave
 }
 
-// What the steps and the splat change goes back to particle order, one
-// thread per particle (coalesced stores).
-__global__ __launch_bounds__(kT) void k_sort_scatter(Particles S, int n, const int32_t *__restrict__ slot,
-                                                    Particles P) {
-    const int i = blockIdx.x * kT + threadIdx.x;
-    if (i >= n) return;
-    const int s = slot[i];
-    if (!S.active[s]) return;
-    P.x[i] = S.x[s]; P.y[i] = S.y[s]; P.u[i] = S.u[s]; P.v[i] = S.v[s];
-    P.r[i] = S.r[s]; P.g[i] = S.g[s]; P.b[i] = S.b[s]; P.a[i] = S.a[s];
-    P.mature[i] = S.mature[s];
-}
+// What the steps and the splat change goes back to particle order, for
+// particle i from its slot (coalesced stores over consecutive i).
+struct ScatterBack {
+    Particles S;
+    const int32_t *slot;
+    Particles P;
+    __device__ void operator()(int i) const {
+        const int s = slot[i];
+        if (!S.active[s]) return;
+        P.x[i] = S.x[s]; P.y[i] = S.y[s]; P.u[i] = S.u[s]; P.v[i] = S.v[s];
+        P.r[i] = S.r[s]; P.g[i] = S.g[s]; P.b[i] = S.b[s]; P.a[i] = S.a[s];
+        P.mature[i] = S.mature[s];
+    }
+};
+
 
 }  // namespace
 
@@ -1492,17 +1520,19 @@ namespace {
 // counted: the counts and ranks are in place already.
 // Whole bins (the splat's pixel tiles).  Beyond kLdsTiles tiles the counts
 // are scanned first by one workgroup.
-template <class Tiles>
+template <class Tiles, class Tail = NoTail>
 void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s,
-                   bool counted = false) {
+                   bool counted = false, const Tail &tail = Tail()) {
     const uint32_t gp = uint32_t((n + kT - 1) / kT);
     if (!counted) hipLaunchKernelGGL(k_bin_count<Tiles>, dim3(gp), dim3(kT), 0, s, tiles_of, n, B, par);
     const uint32_t gb = uint32_t((std::max(n, tiles) + kT - 1) / kT);
     if (tiles <= kLdsTiles) {
-        hipLaunchKernelGGL((k_bin_place<true, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
+        hipLaunchKernelGGL((k_bin_place<true, Tiles, Tail>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par,
+                           tail);
     } else {
         hipLaunchKernelGGL(scan::k_scan_sums, dim3(1), dim3(1024), 0, s, B.count[par], tiles, B.start);
-        hipLaunchKernelGGL((k_bin_place<false, Tiles>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par);
+        hipLaunchKernelGGL((k_bin_place<false, Tiles, Tail>), dim3(gb), dim3(kT), 0, s, tiles_of, n, B, tiles, par,
+                           tail);
     }
 }
 
@@ -1558,19 +1588,26 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     return AMH_OK;
 }
 
+// The counting pass of sort_particles done by another kernel on the way.
+SortCount sort_count_of(amh_fluid_s *f) {
+    return SortCount{f->cell_rank, f->cell, f->pos, f->cnt, f->gsx, f->gsy};
+}
+
 // f->S = f->P in cell order (f->S.atom maps slots back to particles).  The
 // bucket counts f->cnt are zero on entry (from the handle's creation, then
 // from the last sort's k_sort_slots).  count_first: the first step's bins
-// are counted on the way (its parity f->bin_par).
-int sort_particles(amh_fluid_s *f, hipStream_t s, bool count_first) {
+// are counted on the way (its parity f->bin_par).  counted: the buckets are
+// counted already (k_fluid_update with sort_count_of(f)).
+int sort_particles(amh_fluid_s *f, hipStream_t s, bool count_first, bool counted) {
     amh::ProfScope ps_("fluid_sort", s);
     const int64_t nn = int64_t(f->gsx) * f->gsy;
     const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
     const int64_t nk = nn + kIdleBuckets;
-    hipLaunchKernelGGL(k_sort_count, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->gsx, f->gsy, f->cell_rank,
-                       f->cell, f->pos, f->cnt);
+    if (!counted)
+        hipLaunchKernelGGL(k_sort_count, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->gsx, f->gsy, f->cell_rank,
+                           f->cell, f->pos, f->cnt);
     scan::exclusive(f->cnt, nk, f->scan_sums, f->scan_offs, f->start, s);
-    // f->pos becomes each particle's slot (k_sort_scatter reads it after the steps)
+    // f->pos becomes each particle's slot (ScatterBack reads it after the steps)
     hipLaunchKernelGGL(k_sort_slots, dim3(gp), dim3(kT), 0, s, f->count, f->cell, f->pos, f->start, f->pos, f->S,
                        f->cnt, nk);
     hipLaunchKernelGGL(k_sort_gather, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->S, f->gsx, f->gsy, f->bins,
@@ -1603,23 +1640,13 @@ __global__ void k_fluid_reset(Particles P, int n, double x0, double y0) {
     P.active[i] = 0; P.mature[i] = 0; P.owner[i] = 0; P.key[i] = -1;
 }
 
-// step_fluid's bookkeeping and update_particle (morph.cpp:680-1050).
-__global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FrameParams *__restrict__ fp, Particles P,
-                                                    const int32_t *__restrict__ rank,
-                                                    const int32_t *__restrict__ surf,
-                                                    const double *__restrict__ bcol, double sp) {
-    const Frame F = fp->F;
-    __shared__ int lagp[512], slopep[512];
-    if (R.fading == kPerlin) {
-        for (int i = threadIdx.x; i < 512; i += kT) {
-            lagp[i] = R.lag_perm_dev[i];
-            slopep[i] = R.slope_perm_dev[i];
-        }
-    }
-    __syncthreads();
-    const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
-    const int c = block_chain(R.chain_off_dev, R.nchains, a);
-    if (a >= R.wtot) return;
+// update_particle for atom a of chain c (morph.cpp:680-1050); returns the
+// particle's sort bucket when sc.cnt is set (its cell's rank, or an idle
+// bucket when it is not active).
+__device__ __forceinline__ int update_one(const amh_render &R, const Frame &F, Particles P,
+                                          const int32_t *__restrict__ rank, const int32_t *__restrict__ surf,
+                                          const double *__restrict__ bcol, double sp, int64_t a, int c,
+                                          const int *lagp, const int *slopep, const SortCount &sc) {
     // every load first: the compiler must assume the stores to P below may
     // alias the other arrays, and would wait for each load in turn
     const int nA = surf[int64_t(F.fk) * R.nchains + c], nB = surf[int64_t(F.fnext) * R.nchains + c];
@@ -1627,12 +1654,13 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
     const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a], p2 = R.pts_dev[int64_t(F.fnext) * R.wtot + a];
     const int act0 = P.active[a], key0 = P.key[a], mat0 = P.mature[a], own0 = P.owner[a];
     const double r0 = P.r[a], g0 = P.g[a], b0 = P.b[a], a0 = P.a[a];
+    const double x0 = sc.cnt ? P.x[a] : 0.0, y0 = sc.cnt ? P.y[a] : 0.0;
     uint64_t tk[4] = {p1, p1, p2, p2};
     traj_keys(R, F, a, tk);
     const int limit = int(round((1.0 - F.tl) * double(nA) + F.tl * double(nB)));
     if (rk >= limit) {
         P.active[a] = 0;
-        return;
+        return idle_bucket(int(a), sc.gsx * sc.gsy);
     }
     // a new particle (Particle::clear, fluidmodel.cpp:80-95)
     const bool fresh = !(act0 && key0 == F.fk);
@@ -1672,6 +1700,7 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
     P.gx[a] = gxv;
     P.gy[a] = gyv;
     P.R[a] = Rr; P.G[a] = Gg; P.B[a] = Bb; P.A[a] = Aa;
+    const bool placed = !mature && owner;  // the particle moves to its target
     if (!mature) {  // (a fresh particle's strength 1.0 is replaced here)
         if (owner) {
             P.x[a] = gxv;
@@ -1683,6 +1712,30 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
     }
     P.r[a] = r; P.g[a] = g; P.b[a] = b; P.a[a] = al;
     P.active[a] = 1;
+    return sc.cnt ? sc.cell_rank[cell_of(placed ? gxv : x0, placed ? gyv : y0, sc.gsx, sc.gsy)] : 0;
+}
+
+// step_fluid's bookkeeping and update_particle (morph.cpp:680-1050), one
+// thread per atom; with sc.cnt set, also the counting pass of the cell sort
+// that follows (as k_sort_count).
+__global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FrameParams *__restrict__ fp, Particles P,
+                                                    const int32_t *__restrict__ rank,
+                                                    const int32_t *__restrict__ surf,
+                                                    const double *__restrict__ bcol, double sp, SortCount sc) {
+    const Frame F = fp->F;
+    __shared__ int lagp[512], slopep[512];
+    if (R.fading == kPerlin) {
+        for (int i = threadIdx.x; i < 512; i += kT) {
+            lagp[i] = R.lag_perm_dev[i];
+            slopep[i] = R.slope_perm_dev[i];
+        }
+    }
+    __syncthreads();
+    const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
+    const int c = block_chain(R.chain_off_dev, R.nchains, a);
+    const bool in = a < R.wtot;
+    const int key = in ? update_one(R, F, P, rank, surf, bcol, sp, a, c, lagp, slopep, sc) : -1;
+    if (sc.cnt) sort_rank(key, in, int(a), sc.cell, sc.pos, sc.cnt);  // uniform; every lane of the wave
 }
 
 // draw_fluid, splat (morph.cpp:1080-1167), in two passes without atomics
@@ -1988,10 +2041,11 @@ void draw_splat(amh_fluid_s *f, const amh_render &R, double sparseness, hipStrea
     f->pbin_par ^= 1;
     hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->fp, f->W, f->H, sparseness,
                        R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps, f->scol, f->pbins, par);
-    bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true);
+    // the placing also scatters the particles back (f->pos: each one's slot)
+    bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true,
+                  ScatterBack{f->S, f->pos, f->P});
     hipLaunchKernelGGL(k_splat_tile, dim3(uint32_t(ptiles)), dim3(kT), 0, s, f->pbins, f->sxy, f->staps, f->scol,
                        f->W, f->H, f->fp, R.keep_background, f->img, f->mask);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->pos, f->P);
 }
 
 // The outward feather layers and the background blend (morph.cpp:1211-1298)
@@ -2014,11 +2068,12 @@ int enqueue_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
     const amh_render &R = *fa->r;
     if (drawn) {
         const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
+        // the update counts the cell sort's buckets on the way
         amh::launch_timed("k_fluid_update", k_fluid_update, dim3(gp), dim3(kT), 0, s, R, f->fp, f->P, fa->rank_dev,
-                          fa->surface_dev, fa->blob_col_dev, fa->sparseness);
+                          fa->surface_dev, fa->blob_col_dev, fa->sparseness, sort_count_of(f));
         // the steps and the splat run on the cell-ordered copy
         // the sort counts the first step's bins; each step's advection the next's
-        if (int r = sort_particles(f, s, fa->fluidsteps > 0)) return r;
+        if (int r = sort_particles(f, s, fa->fluidsteps > 0, true)) return r;
         for (int k = 0; k < fa->fluidsteps; ++k) {
             const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), 0.0, 0, s, true,
                                         k + 1 < fa->fluidsteps, f->fp, k);
@@ -2169,7 +2224,7 @@ extern "C" int amh_fluid_draw(void *fluid, const amh_fluid_frame *fa, double t, 
     v.out = out;
     hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, s, f->fp, v);
     if (v.F.fk < R.nkeys && f->count > 0) {
-        if (int r = sort_particles(f, s, false)) return r;  // no step follows: nothing to count
+        if (int r = sort_particles(f, s, false, false)) return r;  // no step follows: nothing to count
         draw_splat(f, R, fa->sparseness, s);
     } else {
         const int64_t npx = int64_t(f->W) * f->H;
