@@ -370,6 +370,15 @@ constexpr int kPxTile = 32;  // pixels per side of a splat tile
 
 constexpr uint32_t kNoTap = 0xFFFFFFFFu;  // splat: particle draws nothing (X, Y <= 65534)
 
+// The pixel tiles of a splat's taps at v = X | Y << 16 (v != kNoTap).
+__device__ __forceinline__ void splat_tiles(uint32_t v, int W, int H, int &tx0, int &tx1, int &ty0, int &ty1) {
+    const int X = int(v & 0xFFFFu), Y = int(v >> 16);
+    tx0 = X / kPxTile;
+    tx1 = min(X + 1, W - 1) / kPxTile;
+    ty0 = Y / kPxTile;
+    ty1 = min(Y + 1, H - 1) / kPxTile;
+}
+
 struct SplatTiles {  // the splat: pixel tiles holding the 2x2 taps at (X, Y)
     const uint32_t *xy;  // X | Y << 16, kNoTap = nothing to draw
     int W, H;
@@ -377,11 +386,7 @@ struct SplatTiles {  // the splat: pixel tiles holding the 2x2 taps at (X, Y)
     __device__ __forceinline__ bool operator()(int i, int &tx0, int &tx1, int &ty0, int &ty1) const {
         const uint32_t v = xy[i];
         if (v == kNoTap) return false;
-        const int X = int(v & 0xFFFFu), Y = int(v >> 16);
-        tx0 = X / kPxTile;
-        tx1 = min(X + 1, W - 1) / kPxTile;
-        ty0 = Y / kPxTile;
-        ty1 = min(Y + 1, H - 1) / kPxTile;
+        splat_tiles(v, W, H, tx0, tx1, ty0, ty1);
         return true;
     }
 };
@@ -1042,20 +1047,106 @@ __device__ __forceinline__ float4 keep_if(bool ok, float4 v) {
     return float4{ok ? v.x : 0.0f, ok ? v.y : 0.0f, ok ? v.z : 0.0f, ok ? v.w : 0.0f};
 }
 
+// The frame's splat arguments.
+struct SplatArgs {
+    int W, H;
+    double sp;
+    int bx1, by1, bx2, by2;  // the blob bounding box of the frame
+    uint32_t *xy;            // [n] X | Y << 16 of the top-left tap, kNoTap = not drawn
+    double *taps;            // [n][4] tap weights
+    uint32_t *cols;          // [n] packed colour
+};
+
+// An active particle's splat from its values (strength s, position, colour
+// targets R0..A0 and colours r0..a0): the colour update, top-left tap pixel
+// (xy[i], kNoTap when nothing is drawn), four tap weights and packed
+// colour.  Returns xy[i].
+__device__ __forceinline__ uint32_t splat_core(Particles P, int i, double t, const SplatArgs &a, double s, double x,
+                                               double y, double R0, double G0, double B0, double A0, double r0,
+                                               double g0, double b0, double a0) {
+    const int W = a.W, H = a.H, bx1 = a.bx1, by1 = a.by1, bx2 = a.bx2, by2 = a.by2;
+    const double sp = a.sp;
+    uint32_t *__restrict__ xy = a.xy;
+    double *__restrict__ taps = a.taps;
+    uint32_t *__restrict__ cols = a.cols;
+    const double w = (1.0 / (100.0 * pow((t - (1.0 * (1.0 - s))), 2.0) + 1.0)) * pow(s, 2.0);
+    const double r = (w * R0) + (1.0 - w) * r0, g = (w * G0) + (1.0 - w) * g0;
+    const double b = (w * B0) + (1.0 - w) * b0, al = (w * A0) + (1.0 - w) * a0;
+    P.r[i] = r; P.g[i] = g; P.b[i] = b; P.a[i] = al;
+    double px, py;
+    if (x <= 10.0) px = 0.0;
+    else if (x >= W * sp + 10.0) px = (W - 1);
+    else px = (x - 10.0) / sp;
+    if (y <= 10.0) py = 0.0;
+    else if (y >= H * sp + 10.0) py = (H - 1);
+    else py = (y - 10.0) / sp;
+    const int X = min(int(px), W - 1), Y = min(int(py), H - 1);
+    P.mature[i] = 1;
+    double ix, iy;
+    const double xf = modf(px, &ix), yf = modf(py, &iy);
+    if ((ix >= W || iy >= H) && (ix > bx2 || ix < bx1 || iy > by2 || iy < by1)) {
+        xy[i] = kNoTap;
+        return kNoTap;
+    }
+    const double w11 = ((1.0 - xf) * (1.0 - yf)), w21 = (xf * (1.0 - yf));
+    const double w12 = ((1.0 - xf) * yf), w22 = (xf * yf);
+    // a tap is drawn when its weight is positive, its guard holds and it lies
+    // inside the image; taps (X,Y) (X+1,Y) (X,Y+1) (X+1,Y+1)
+    double wt[4] = {0.0, 0.0, 0.0, 0.0};
+    if (w11 > 0.0) wt[0] = w11;
+    if ((X < bx2 || X + 1 < W) && w21 > 0.0 && X + 1 < W) wt[1] = w21;
+    if ((Y < by2 || Y + 1 < H) && w12 > 0.0 && Y + 1 < H) wt[2] = w12;
+    if (w22 > 0.0 && ((Y < by2 && X < bx2) || (Y + 1 < H && X + 1 < W)) && X + 1 < W && Y + 1 < H) wt[3] = w22;
+    const uint32_t v = uint32_t(X) | (uint32_t(Y) << 16);
+    xy[i] = v;
+    for (int q = 0; q < 4; ++q) taps[int64_t(i) * 4 + q] = wt[q];
+    cols[i] = rgba_pack(narrow8(round(r * 255.0)), narrow8(round(g * 255.0)), narrow8(round(b * 255.0)),
+                        narrow8(round(al * 255.0)));
+    return v;
+}
+
+// Particle i's splat (splat_core from its stored values; xy[i] = kNoTap
+// when it is not active).  Returns xy[i].
+__device__ __forceinline__ uint32_t splat_one(Particles P, int i, double t, const SplatArgs &a) {
+    // every load before the first store (see k_fluid_update)
+    const int act = P.active[i];
+    const double s = P.strength[i], x = P.x[i], y = P.y[i];
+    const double R0 = P.R[i], G0 = P.G[i], B0 = P.B[i], A0 = P.A[i];
+    const double r0 = P.r[i], g0 = P.g[i], b0 = P.b[i], a0 = P.a[i];
+    if (!act) {
+        a.xy[i] = kNoTap;
+        return kNoTap;
+    }
+    return splat_core(P, i, t, a, s, x, y, R0, G0, B0, A0, r0, g0, b0, a0);
+}
+
 // One particle's advection (fluidmodel.cpp:443-564); returns its new
 // position.  atom: the particle held in each slot when P is the cell-ordered
 // copy (NULL: slot i holds particle i); the wall jitter is keyed by particle.
 // Every load is unconditional and issued up front (nodes outside the grid
 // read node 0 and count as zero), so the particle costs two memory round
 // trips -- its fields, then its nine nodes -- instead of one per node.
+// The values of an advected particle that its splat takes (the frame's last
+// step splats each particle right after moving it).
+struct Advected {
+    double s;               // strength
+    double R, G, B, A;      // colour targets
+    double r, g, b, a;      // colours after the diffusion
+};
+
 __device__ __forceinline__ double2 advect_one(Particles P, int i, const Grid &Gd, int gsx, int gsy, int steps_left,
-                                              double freedom_radius, uint64_t seed, const int32_t *__restrict__ atom) {
+                                              double freedom_radius, uint64_t seed, const int32_t *__restrict__ atom,
+                                              Advected *out = nullptr) {
     const int id = atom ? atom[i] : i;
     double x = P.x[i], y = P.y[i];
     const int mature = P.mature[i];
     const double R0 = P.R[i], G0 = P.G[i], B0 = P.B[i], A0 = P.A[i];
     const double r0 = P.r[i], g0 = P.g[i], b0 = P.b[i], a0 = P.a[i];
     const double tgx = P.gx[i], tgy = P.gy[i], fr = P.fr[i], u2 = P.u2[i], v2 = P.v2[i];
+    if (out) {
+        out->s = P.strength[i];
+        out->R = R0; out->G = G0; out->B = B0; out->A = A0;
+    }
     int cx, cy;
     double px[3], py[3], gx[3], gy[3];
     weights(x, y, cx, cy, px, py, gx, gy);
@@ -1093,10 +1184,12 @@ __device__ __forceinline__ double2 advect_one(Particles P, int i, const Grid &Gd
         const double wr = fabs(nR - R0), wg = fabs(nG - G0), wb = fabs(nB - B0), wa = fabs(nA - A0);
         const double r1 = mature ? (1.0 - wr) * r0 + wr * nR : nR, g1 = mature ? (1.0 - wg) * g0 + wg * nG : nG;
         const double b1 = mature ? (1.0 - wb) * b0 + wb * nB : nB, a1 = mature ? (1.0 - wa) * a0 + wa * nA : nA;
-        P.r[i] = diffuse ? r1 : r0;
-        P.g[i] = diffuse ? g1 : g0;
-        P.b[i] = diffuse ? b1 : b0;
-        P.a[i] = diffuse ? a1 : a0;
+        const double r = diffuse ? r1 : r0, g = diffuse ? g1 : g0, b = diffuse ? b1 : b0, a = diffuse ? a1 : a0;
+        if (out) {  // the splat stores the colours it relaxes from these
+            out->r = r; out->g = g; out->b = b; out->a = a;
+        } else {
+            P.r[i] = r; P.g[i] = g; P.b[i] = b; P.a[i] = a;
+        }
     }
     x += gu;
     y += gv;
@@ -1127,10 +1220,14 @@ __device__ __forceinline__ double2 advect_one(Particles P, int i, const Grid &Gd
 // gathers through the bin list cost more than the node reads it saves.)
 // With count_par >= 0 the moved particles are also counted into the next
 // step's bins (B.count[count_par]), which saves that step's counting pass.
+// kSplat (a frame's last step, fp set): each particle is splatted once
+// moved (k_splat_prep's work, counting its taps into the pixel-tile bins
+// B.count[count_par]).
+template <bool kSplat>
 __global__ __launch_bounds__(kT) void k_advect(Particles P, int n, Grid Gd, int gsx, int gsy, int steps_left,
                                                double freedom_radius, uint64_t seed,
                                                const int32_t *__restrict__ atom, Bins B, int count_par,
-                                               const FrameParams *__restrict__ fp, int step_k) {
+                                               const FrameParams *__restrict__ fp, int step_k, SplatArgs sa) {
     if (fp) {  // a frame's step: radius and jitter seed of the frame
         freedom_radius = fp->radius;
         seed = step_seed(fp->frame, step_k);
@@ -1138,11 +1235,22 @@ __global__ __launch_bounds__(kT) void k_advect(Particles P, int n, Grid Gd, int 
     const int i = blockIdx.x * kT + threadIdx.x;
     const bool act = i < n && P.active[i];
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+    bool counted = act;
     if (act) {
-        const double2 q = advect_one(P, i, Gd, gsx, gsy, steps_left, freedom_radius, seed, atom);
-        stencil_tiles(q.x, q.y, gsx, gsy, tx0, tx1, ty0, ty1);
+        if (kSplat) {
+            Advected o;
+            const double2 q = advect_one(P, i, Gd, gsx, gsy, steps_left, freedom_radius, seed, atom, &o);
+            const uint32_t v = splat_core(P, i, fp->F.tl, sa, o.s, q.x, q.y, o.R, o.G, o.B, o.A, o.r, o.g, o.b, o.a);
+            counted = v != kNoTap;
+            if (counted) splat_tiles(v, sa.W, sa.H, tx0, tx1, ty0, ty1);
+        } else {
+            const double2 q = advect_one(P, i, Gd, gsx, gsy, steps_left, freedom_radius, seed, atom);
+            stencil_tiles(q.x, q.y, gsx, gsy, tx0, tx1, ty0, ty1);
+        }
+    } else if (kSplat && i < n) {
+        sa.xy[i] = kNoTap;
     }
-    if (count_par >= 0) count_entries(act, tx0, tx1, ty0, ty1, i, n, B, count_par);
+    if (count_par >= 0) count_entries(counted, tx0, tx1, ty0, ty1, i, n, B, count_par);
 }
 
 
@@ -1243,6 +1351,14 @@ __global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, Particle
     double x = 0.0, y = 0.0;
     if (in) {
         const int i = S.atom[s];
+        act = P.active[i];
+        S.active[s] = act;
+    }
+    // an inactive particle's slot keeps only its flag: every reader of the
+    // other fields (steps, splat, scatter back) skips inactive slots, and the
+    // inactive particles fill the last slots, so whole waves skip the gather
+    if (act) {
+        const int i = S.atom[s];
         // one field at a time: with all 18 random gathers of a wave in flight at
         // once the L2 held fewer of their lines (36 against 30 us per frame)
         x = P.x[i];
@@ -1252,8 +1368,6 @@ __global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, Particle
         S.R[s] = P.R[i]; S.G[s] = P.G[i]; S.B[s] = P.B[i]; S.A[s] = P.A[i];
         S.r[s] = P.r[i]; S.g[s] = P.g[i]; S.b[s] = P.b[i]; S.a[s] = P.a[i];
         S.strength[s] = P.strength[i];
-        act = P.active[i];
-        S.active[s] = act;
         S.mature[s] = P.mature[i];
     }
     if (par < 0) return;  // uniform: no step follows
@@ -1560,7 +1674,8 @@ void place_items(const StencilTiles &tiles_of, int n, const Bins &B, int tiles, 
 // particles).
 int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32_t steps_left,
                   double freedom_radius, uint64_t seed, hipStream_t s, bool counted = false,
-                  bool count_next = false, const FrameParams *fp = nullptr, int step_k = 0) {
+                  bool count_next = false, const FrameParams *fp = nullptr, int step_k = 0,
+                  const SplatArgs *splat = nullptr, int splat_par = 0) {
     amh::ProfScope step_("fluid_step", s);
     if (f->count == 0) return AMH_OK;
     const int tiles = f->bins.ntx * f->bins.nty;
@@ -1581,9 +1696,14 @@ int fluid_step_on(amh_fluid_s *f, const Particles &Q, const int32_t *atom, int32
     amh::launch_timed("k_p2g", k_p2g, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     amh::launch_timed("k_forces", k_forces, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
     amh::launch_timed("k_g2p", k_g2p, dim3(grid), dim3(kT), 0, s, Q, f->G, f->gsx, f->gsy, B);
-    amh::launch_timed("k_advect", k_advect, dim3(uint32_t((f->count + kT - 1) / kT)), dim3(kT), 0, s, Q, f->count,
-                      f->G, f->gsx, f->gsy, int(steps_left), freedom_radius, seed, atom, B,
-                      count_next ? f->bin_par : -1, fp, step_k);
+    const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
+    if (splat)  // the frame's last step: the splat's per-particle half too
+        amh::launch_timed("k_advect", k_advect<true>, dim3(gp), dim3(kT), 0, s, Q, f->count, f->G, f->gsx, f->gsy,
+                          int(steps_left), freedom_radius, seed, atom, f->pbins, splat_par, fp, step_k, *splat);
+    else
+        amh::launch_timed("k_advect", k_advect<false>, dim3(gp), dim3(kT), 0, s, Q, f->count, f->G, f->gsx, f->gsy,
+                          int(steps_left), freedom_radius, seed, atom, B, count_next ? f->bin_par : -1, fp, step_k,
+                          SplatArgs{});
     AMH_LAUNCH_CHECK("amh_fluid_step");
     return AMH_OK;
 }
@@ -1745,68 +1865,16 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
 // k_splat_tile gathers a tile's taps in LDS (FP64 sums: weight, weight *
 // channel) and resolves its pixels (morph.cpp:1169-1209) straight into the
 // image.
-// Particle i's splat: its colour update, top-left tap pixel (xy[i], kNoTap
-// when nothing is drawn), four tap weights and packed colour.  Returns xy[i].
-__device__ __forceinline__ uint32_t splat_one(Particles P, int i, double t, int W, int H, double sp, int bx1, int by1,
-                                              int bx2, int by2, uint32_t *__restrict__ xy,
-                                              double *__restrict__ taps, uint32_t *__restrict__ cols) {
-    // every load before the first store (see k_fluid_update)
-    const int act = P.active[i];
-    const double s = P.strength[i], x = P.x[i], y = P.y[i];
-    const double R0 = P.R[i], G0 = P.G[i], B0 = P.B[i], A0 = P.A[i];
-    const double r0 = P.r[i], g0 = P.g[i], b0 = P.b[i], a0 = P.a[i];
-    if (!act) {
-        xy[i] = kNoTap;
-        return kNoTap;
-    }
-    const double w = (1.0 / (100.0 * pow((t - (1.0 * (1.0 - s))), 2.0) + 1.0)) * pow(s, 2.0);
-    const double r = (w * R0) + (1.0 - w) * r0, g = (w * G0) + (1.0 - w) * g0;
-    const double b = (w * B0) + (1.0 - w) * b0, al = (w * A0) + (1.0 - w) * a0;
-    P.r[i] = r; P.g[i] = g; P.b[i] = b; P.a[i] = al;
-    double px, py;
-    if (x <= 10.0) px = 0.0;
-    else if (x >= W * sp + 10.0) px = (W - 1);
-    else px = (x - 10.0) / sp;
-    if (y <= 10.0) py = 0.0;
-    else if (y >= H * sp + 10.0) py = (H - 1);
-    else py = (y - 10.0) / sp;
-    const int X = min(int(px), W - 1), Y = min(int(py), H - 1);
-    P.mature[i] = 1;
-    double ix, iy;
-    const double xf = modf(px, &ix), yf = modf(py, &iy);
-    if ((ix >= W || iy >= H) && (ix > bx2 || ix < bx1 || iy > by2 || iy < by1)) {
-        xy[i] = kNoTap;
-        return kNoTap;
-    }
-    const double w11 = ((1.0 - xf) * (1.0 - yf)), w21 = (xf * (1.0 - yf));
-    const double w12 = ((1.0 - xf) * yf), w22 = (xf * yf);
-    // a tap is drawn when its weight is positive, its guard holds and it lies
-    // inside the image; taps (X,Y) (X+1,Y) (X,Y+1) (X+1,Y+1)
-    double wt[4] = {0.0, 0.0, 0.0, 0.0};
-    if (w11 > 0.0) wt[0] = w11;
-    if ((X < bx2 || X + 1 < W) && w21 > 0.0 && X + 1 < W) wt[1] = w21;
-    if ((Y < by2 || Y + 1 < H) && w12 > 0.0 && Y + 1 < H) wt[2] = w12;
-    if (w22 > 0.0 && ((Y < by2 && X < bx2) || (Y + 1 < H && X + 1 < W)) && X + 1 < W && Y + 1 < H) wt[3] = w22;
-    const uint32_t v = uint32_t(X) | (uint32_t(Y) << 16);
-    xy[i] = v;
-    for (int q = 0; q < 4; ++q) taps[int64_t(i) * 4 + q] = wt[q];
-    cols[i] = rgba_pack(narrow8(round(r * 255.0)), narrow8(round(g * 255.0)), narrow8(round(b * 255.0)),
-                        narrow8(round(al * 255.0)));
-    return v;
-}
-
 // The splat of every particle, which also counts the pixel-tile bins of
 // its taps into B.count[par] (saving the binning's counting pass).
-__global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, const FrameParams *__restrict__ fp, int W,
-                                                  int H, double sp, int bx1, int by1, int bx2, int by2,
-                                                  uint32_t *__restrict__ xy, double *__restrict__ taps,
-                                                  uint32_t *__restrict__ cols, Bins B, int par) {
+__global__ __launch_bounds__(kT) void k_splat_prep(Particles P, int n, const FrameParams *__restrict__ fp,
+                                                  SplatArgs a, Bins B, int par) {
     const double t = fp->F.tl;
     const int i = blockIdx.x * kT + threadIdx.x;
-    const uint32_t v = i < n ? splat_one(P, i, t, W, H, sp, bx1, by1, bx2, by2, xy, taps, cols) : kNoTap;
+    const uint32_t v = i < n ? splat_one(P, i, t, a) : kNoTap;
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
-    const bool act = v != kNoTap && SplatTiles{xy, W, H}(i, tx0, tx1, ty0, ty1);
-    count_entries(act, tx0, tx1, ty0, ty1, i, n, B, par);  // every lane of the wave
+    if (v != kNoTap) splat_tiles(v, a.W, a.H, tx0, tx1, ty0, ty1);
+    count_entries(v != kNoTap, tx0, tx1, ty0, ty1, i, n, B, par);  // every lane of the wave
 }
 
 struct SplatItem {
@@ -2033,14 +2101,21 @@ namespace {
 // copy f->S: colour relax and splat, resolve into f->img / f->mask, then the
 // particles (colours relaxed, mature set) back to f->P.  The frame's time
 // comes from f->fp.
-void draw_splat(amh_fluid_s *f, const amh_render &R, double sparseness, hipStream_t s) {
+SplatArgs splat_args(const amh_fluid_s *f, const amh_render &R, double sparseness) {
+    return SplatArgs{f->W, f->H, sparseness, R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps, f->scol};
+}
+
+// prepped: the per-particle half (k_splat_prep) ran already, in the last
+// step's advection, counting into f->pbins.count[f->pbin_par].
+void draw_splat(amh_fluid_s *f, const amh_render &R, double sparseness, hipStream_t s, bool prepped) {
     amh::ProfScope ps_("k_fluid_splat", s);
     const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
     const int ptiles = f->pbins.ntx * f->pbins.nty;
     const int par = f->pbin_par;
     f->pbin_par ^= 1;
-    hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->fp, f->W, f->H, sparseness,
-                       R.bbox_x1, R.bbox_y1, R.bbox_x2, R.bbox_y2, f->sxy, f->staps, f->scol, f->pbins, par);
+    if (!prepped)
+        hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->fp,
+                           splat_args(f, R, sparseness), f->pbins, par);
     // the placing also scatters the particles back (f->pos: each one's slot)
     bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true,
                   ScatterBack{f->S, f->pos, f->P});
@@ -2074,12 +2149,14 @@ int enqueue_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
         // the steps and the splat run on the cell-ordered copy
         // the sort counts the first step's bins; each step's advection the next's
         if (int r = sort_particles(f, s, fa->fluidsteps > 0, true)) return r;
+        const SplatArgs sa = splat_args(f, R, fa->sparseness);
         for (int k = 0; k < fa->fluidsteps; ++k) {
-            const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), 0.0, 0, s, true,
-                                        k + 1 < fa->fluidsteps, f->fp, k);
+            const bool last = k + 1 == fa->fluidsteps;
+            const int r = fluid_step_on(f, f->S, f->S.atom, fa->fluidsteps - (k + 1), 0.0, 0, s, true, !last, f->fp,
+                                        k, last ? &sa : nullptr, f->pbin_par);
             if (r) return r;
         }
-        draw_splat(f, R, fa->sparseness, s);
+        draw_splat(f, R, fa->sparseness, s, fa->fluidsteps > 0);
     } else {
         const int64_t npx = int64_t(f->W) * f->H;
         hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((npx + kT - 1) / kT)), dim3(kT), 0, s, f->mask, npx);
@@ -2225,7 +2302,7 @@ extern "C" int amh_fluid_draw(void *fluid, const amh_fluid_frame *fa, double t, 
     hipLaunchKernelGGL(k_set_params, dim3(1), dim3(1), 0, s, f->fp, v);
     if (v.F.fk < R.nkeys && f->count > 0) {
         if (int r = sort_particles(f, s, false, false)) return r;  // no step follows: nothing to count
-        draw_splat(f, R, fa->sparseness, s);
+        draw_splat(f, R, fa->sparseness, s, false);
     } else {
         const int64_t npx = int64_t(f->W) * f->H;
         hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((npx + kT - 1) / kT)), dim3(kT), 0, s, f->mask, npx);
