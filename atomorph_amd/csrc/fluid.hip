@@ -1031,6 +1031,15 @@ struct FrameParams {
 
 __global__ void k_set_params(FrameParams *dst, FrameParams v) { *dst = v; }
 
+// An atom's PERLIN lag and slope as last computed (atom_lag_slope), for the
+// source point p1 under the tables and bounding box with fingerprint tag
+// (never 0, so zeroed memory holds nothing).  They stay the same for all
+// the frames of a key segment.
+struct LagSlopeMemo {
+    double lag, slope;
+    uint64_t p1, tag;
+};
+
 // Wall jitter seed of step k of a frame (never 0, which means none).
 __device__ __forceinline__ uint64_t step_seed(uint64_t frame, int k) { return mix64(frame * 131 + uint64_t(k)) | 1u; }
 
@@ -1419,6 +1428,7 @@ struct amh_fluid_s {
     uint64_t frame = 0;        // frames rendered (jitter seed)
     int32_t bin_par = 0;       // parity of the next step's bin counts
     FrameParams *fp = nullptr; // the current frame's values (device)
+    LagSlopeMemo *memo = nullptr;  // [n] the atoms' PERLIN lag and slope, kept between frames
     // Captured frames: the launch sequence of one frame as a hipGraph, keyed
     // by everything its launches bake in (render arguments, steps, parities).
     struct Captured {
@@ -1449,7 +1459,7 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
            (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 4 + list_entries(tiles, n) * 4 + tiles * 4 +
            (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) + max_chunks(tiles, n) * 16 + 4 + nn * 4 * kGridFields +
-           npx * 4 * 2 + 256 + 128 * 256;
+           npx * 4 * 2 + 256 + n * sizeof(LagSlopeMemo) + 128 * 256;
 }
 
 void carve_particles(Particles &Q, size_t n, const std::function<char *(size_t)> &take) {
@@ -1528,6 +1538,7 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     f->mask = reinterpret_cast<int32_t *>(take(npx * 4));
     f->img = reinterpret_cast<uint32_t *>(take(npx * 4));
     f->fp = reinterpret_cast<FrameParams *>(take(sizeof(FrameParams)));
+    f->memo = reinterpret_cast<LagSlopeMemo *>(take(n * sizeof(LagSlopeMemo)));
     if (size_t(p - static_cast<char *>(f->buf)) > bytes) {
         set_error("amh_fluid_create: layout of %zu bytes exceeds its allocation of %zu",
                   size_t(p - static_cast<char *>(f->buf)), bytes);
@@ -1760,13 +1771,28 @@ __global__ void k_fluid_reset(Particles P, int n, double x0, double y0) {
     P.active[i] = 0; P.mature[i] = 0; P.owner[i] = 0; P.key[i] = -1;
 }
 
+// Fingerprint of the PERLIN tables and the bounding box, the same in every
+// block: each thread adds a mix of each table entry it staged (hsum), the
+// block sums them in wsum[kT / 64] (after a barrier, by every thread).
+__device__ __forceinline__ uint64_t lag_slope_tag(const amh_render &R, uint64_t hsum, uint64_t *wsum) {
+    hsum = wave_sum_u64(hsum);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = hsum;
+    __syncthreads();
+    uint64_t h = 0;
+    for (int w = 0; w < kT / 64; ++w) h += wsum[w];
+    const uint64_t box = mix64((uint64_t(uint32_t(R.bbox_x1)) << 32) | uint32_t(R.bbox_y1)) ^
+                         mix64(((uint64_t(uint32_t(R.bbox_x2)) << 32) | uint32_t(R.bbox_y2)) + 0x9E3779B97F4A7C15ull);
+    return mix64(h ^ box) | 1u;
+}
+
 // update_particle for atom a of chain c (morph.cpp:680-1050); returns the
 // particle's sort bucket when sc.cnt is set (its cell's rank, or an idle
-// bucket when it is not active).
+// bucket when it is not active).  lag, slope: the atom's PERLIN lag and
+// slope (atom_lag_slope) when the fade is PERLIN.
 __device__ __forceinline__ int update_one(const amh_render &R, const Frame &F, Particles P,
                                           const int32_t *__restrict__ rank, const int32_t *__restrict__ surf,
                                           const double *__restrict__ bcol, double sp, int64_t a, int c,
-                                          const int *lagp, const int *slopep, const SortCount &sc) {
+                                          const SortCount &sc, double lag, double slope) {
     // every load first: the compiler must assume the stores to P below may
     // alias the other arrays, and would wait for each load in turn
     const int nA = surf[int64_t(F.fk) * R.nchains + c], nB = surf[int64_t(F.fnext) * R.nchains + c];
@@ -1795,7 +1821,7 @@ __device__ __forceinline__ int update_one(const amh_render &R, const Frame &F, P
     else if (h2 && !h1) c1 = nA > 0 ? (k2 & 0x00FFFFFFu) : k2;
     uint32_t qx, qy, qxf, qyf;
     traj_point(R, F, a, p1, p2, qx, qy, qxf, qyf, tk);
-    const uint32_t col = atom_colour(R, F, p1, c1, c2, lagp, slopep);
+    const uint32_t col = atom_colour_ls(R, F, c1, c2, lag, slope);
     // point2xy works in float (atomorph.h:316-319)
     const float ptx = float(qx) + float(qxf) / float(256), pty = float(qy) + float(qyf) / float(256);
     const double gxv = fmin(fmax((double(ptx) * sp) + 10.0, 1.0), double(W * sp + 10.0));
@@ -1841,20 +1867,41 @@ __device__ __forceinline__ int update_one(const amh_render &R, const Frame &F, P
 __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FrameParams *__restrict__ fp, Particles P,
                                                     const int32_t *__restrict__ rank,
                                                     const int32_t *__restrict__ surf,
-                                                    const double *__restrict__ bcol, double sp, SortCount sc) {
+                                                    const double *__restrict__ bcol, double sp, SortCount sc,
+                                                    LagSlopeMemo *__restrict__ memo) {
     const Frame F = fp->F;
     __shared__ int lagp[512], slopep[512];
-    if (R.fading == kPerlin) {
+    __shared__ uint64_t wsum[kT / 64];
+    uint64_t tag = 0;
+    if (R.fading == kPerlin) {  // (uniform)
+        uint64_t hsum = 0;
         for (int i = threadIdx.x; i < 512; i += kT) {
-            lagp[i] = R.lag_perm_dev[i];
-            slopep[i] = R.slope_perm_dev[i];
+            const int l = R.lag_perm_dev[i], q = R.slope_perm_dev[i];
+            lagp[i] = l;
+            slopep[i] = q;
+            hsum += mix64((uint64_t(i) << 32) | uint32_t(l)) + mix64((uint64_t(i + 512) << 32) | uint32_t(q));
         }
+        tag = lag_slope_tag(R, hsum, wsum);  // (its barrier also publishes the tables)
     }
     __syncthreads();
     const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
     const int c = block_chain(R.chain_off_dev, R.nchains, a);
     const bool in = a < R.wtot;
-    const int key = in ? update_one(R, F, P, rank, surf, bcol, sp, a, c, lagp, slopep, sc) : -1;
+    // PERLIN: the atom's lag and slope from memo[a] when it holds them for
+    // this source point and tag, else computed and kept there (on its own,
+    // before the update's loads, so few values are live around the noise)
+    double lag = 0.5, slope = 0.5;
+    if (tag != 0 && in) {
+        const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a];
+        const LagSlopeMemo mm = memo[a];
+        lag = mm.lag;
+        slope = mm.slope;
+        if (!(mm.p1 == p1 && mm.tag == tag)) {
+            atom_lag_slope(R, p1, lagp, slopep, lag, slope);
+            memo[a] = LagSlopeMemo{lag, slope, p1, tag};
+        }
+    }
+    const int key = in ? update_one(R, F, P, rank, surf, bcol, sp, a, c, sc, lag, slope) : -1;
     if (sc.cnt) sort_rank(key, in, int(a), sc.cell, sc.pos, sc.cnt);  // uniform; every lane of the wave
 }
 
@@ -2145,7 +2192,7 @@ int enqueue_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
         const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
         // the update counts the cell sort's buckets on the way
         amh::launch_timed("k_fluid_update", k_fluid_update, dim3(gp), dim3(kT), 0, s, R, f->fp, f->P, fa->rank_dev,
-                          fa->surface_dev, fa->blob_col_dev, fa->sparseness, sort_count_of(f));
+                          fa->surface_dev, fa->blob_col_dev, fa->sparseness, sort_count_of(f), f->memo);
         // the steps and the splat run on the cell-ordered copy
         // the sort counts the first step's bins; each step's advection the next's
         if (int r = sort_particles(f, s, fa->fluidsteps > 0, true)) return r;

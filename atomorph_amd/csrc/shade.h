@@ -40,7 +40,10 @@ __device__ double pnoise(const int *p, double x, double y) {
     return plerp(v, plerp(u, pgrad(p[AA], x, y), pgrad(p[BA], x - 1, y)),
                  plerp(u, pgrad(p[AB], x, y - 1), pgrad(p[BB], x - 1, y - 1)));
 }
-__device__ double poctave(const int *p, double x, double y) {
+// Out of line (scalar arguments): inlined into a kernel, its octaves' table
+// lookups are scheduled together with the caller's values and the kernel
+// takes about forty more registers.
+__device__ __noinline__ double poctave(const int *p, double x, double y) {
     double result = 0.0, amp = 1.0;
     for (int i = 0; i < 8; ++i) {
         result += pnoise(p, x, y) * amp;
@@ -169,17 +172,40 @@ __device__ inline void traj_point(const amh_render &R, const Frame &F, int64_t a
     }
 }
 
-// Colour of an atom at the frame's time (morph.cpp:537-550): the Perlin
-// lag / slope coordinates follow the source point p1 over the bounding box.
+// The PERLIN fade's lag and slope at Perlin coordinates (px, py).
+__device__ __forceinline__ double2 lag_slope_at(double px, double py, const int *lagp, const int *slopep) {
+    return double2{poctave(lagp, px, py) * 0.5 + 0.5, poctave(slopep, px, py) * 0.5 + 0.5};
+}
+
+// The PERLIN fade's lag and slope of an atom with source point p1
+// (morph.cpp:537-550): the Perlin coordinates follow p1 over the bounding
+// box.  They depend on p1, the bounding box and the two tables only, not on
+// the time.
+__device__ __forceinline__ void atom_lag_slope(const amh_render &R, uint64_t p1, const int *lagp, const int *slopep,
+                                               double &lag, double &slope) {
+    const double bw = R.bbox_x2 - R.bbox_x1 + 1.0, bh = R.bbox_y2 - R.bbox_y1 + 1.0;
+    const double perx = (((int(pt_x(p1)) - R.bbox_x1) * 256 + int(pt_xf(p1))) / double(bw * 256)) * 8.0;
+    const double pery = (((int(pt_y(p1)) - R.bbox_y1) * 256 + int(pt_yf(p1))) / double(bh * 256)) * 8.0;
+    const double2 v = lag_slope_at(perx, pery, lagp, slopep);
+    lag = v.x;
+    slope = v.y;
+}
+
+// Colour of an atom at the frame's time (morph.cpp:537-550); for PERLIN
+// with its lag and slope (atom_lag_slope) given.
+__device__ inline uint32_t atom_colour_ls(const amh_render &R, const Frame &F, uint32_t c1, uint32_t c2, double lag,
+                                          double slope) {
+    const double str = 1.0 - F.tl;
+    if (R.fading == kPerlin) return fade_mix(c1, c2, lag, slope, str, R.fading);
+    if (R.fading == kCosine) return fade_mix(c1, c2, 0.5, 0.5, str, R.fading);
+    return mix(c1, c2, str);
+}
+
 __device__ inline uint32_t atom_colour(const amh_render &R, const Frame &F, uint64_t p1, uint32_t c1, uint32_t c2,
                                        const int *lagp, const int *slopep) {
-    double perx = 0.0, pery = 0.0;
-    if (R.fading == kPerlin) {
-        const double bw = R.bbox_x2 - R.bbox_x1 + 1.0, bh = R.bbox_y2 - R.bbox_y1 + 1.0;
-        perx = (((int(pt_x(p1)) - R.bbox_x1) * 256 + int(pt_xf(p1))) / double(bw * 256)) * 8.0;
-        pery = (((int(pt_y(p1)) - R.bbox_y1) * 256 + int(pt_yf(p1))) / double(bh * 256)) * 8.0;
-    }
-    return fade_colour(c1, c2, perx, pery, 1.0 - F.tl, R.fading, lagp, slopep);
+    double lag = 0.5, slope = 0.5;
+    if (R.fading == kPerlin) atom_lag_slope(R, p1, lagp, slopep, lag, slope);
+    return atom_colour_ls(R, F, c1, c2, lag, slope);
 }
 
 __device__ __forceinline__ int find_chain(const int64_t *off, int nchains, int64_t a) {

@@ -240,6 +240,63 @@ def test_fluid_frames_advance_and_count(golden_cfg1):
             lib.amh_fluid_destroy(fluid)
 
 
+def _renderer(g, variant, swap_tables=False):
+    cfg, keys, off, widths, order, nb, blob_rgba, (lag, slope) = render_inputs(g, variant)
+    vals = {"bbox": [cfg.bbox_x1, cfg.bbox_y1, cfg.bbox_x2, cfg.bbox_y2], "motion": cfg.motion,
+            "fading": cfg.fading, "show": cfg.show_blobs, "density": cfg.density, "feather": cfg.feather,
+            "keep_bg": cfg.keep_background}
+    if swap_tables:
+        lag, slope = slope, lag
+    return GpuRenderer(keys, g["chain_after"], off, widths, order, blob_rgba, vals, lag, slope)
+
+
+def test_fluid_perlin_lag_slope_kept_between_frames(golden_cfg1):
+    """A model keeps each atom's PERLIN lag and slope from frame to frame of
+    a key segment and computes them again when the noise tables change: the
+    colour targets (R, G, B, A) its update sets equal those of a fresh model
+    at the same time, and they depend on the tables."""
+    g = golden_cfg1
+    variant = "spline_perlin_bg_feather2"
+    assert int(g[f"render_{variant}_opts"][1]) == 6  # PERLIN
+    lib = G.lib()
+    n = int(np.sum(g["chain_w"]))
+    made = []
+
+    def particles(f):
+        Q = np.zeros((n, 20))
+        G.check(lib.amh_fluid_get_particles(f, op.ptr(Q), G.stream()), "get")
+        return Q
+
+    def same_targets(QA, QB):
+        both = (QA[:, 16] > 0) & (QB[:, 16] > 0)
+        assert both.sum() > 0.5 * (QB[:, 16] > 0).sum()
+        return (QA[both, 7:11] == QB[both, 7:11]).all()
+
+    try:
+        rend = _renderer(g, variant)
+        swapped = _renderer(g, variant, swap_tables=True)
+        fa = None
+        for t in (0.0, 0.1, 0.2, 0.3):  # key segment 0: the lag and slope of t = 0 are kept
+            G.log(f"perlin memo: frame t={t}")
+            _, fa, _, _, _ = _fluid_frame(g, variant, t, 2, fluid=fa, rend=rend)
+            if not made:
+                made.append(fa)
+        _, fb, _, _, _ = _fluid_frame(g, variant, 0.3, 2, rend=rend)
+        made.append(fb)
+        assert same_targets(particles(fa), particles(fb))
+        # other tables: the kept values no longer hold
+        _, fa, _, _, _ = _fluid_frame(g, variant, 0.35, 2, fluid=fa, rend=swapped)
+        _, fc, _, _, _ = _fluid_frame(g, variant, 0.35, 2, rend=swapped)
+        made.append(fc)
+        _, fd, _, _, _ = _fluid_frame(g, variant, 0.35, 2, rend=rend)
+        made.append(fd)
+        assert same_targets(particles(fa), particles(fc))
+        assert not same_targets(particles(fd), particles(fc))  # the tables matter at t = 0.35
+    finally:
+        for f in made:
+            lib.amh_fluid_destroy(f)
+
+
 def test_engine_fluid_render():
     """am::morph with set_fluid: frames render through the fluid path and
     differ from the atom path."""
