@@ -1663,8 +1663,8 @@ void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int p
 
 // The fluid steps' bins and work items (counted: the counts and ranks are in
 // place already).  Beyond kLdsTiles tiles the tiles' item counts are
-// scanned first, by the three-launch device scan on the workspace sums,
-// offs (scan::blocks(tiles) and one more).
+// scanned first, by the device scan (scan::exclusive) on the workspace
+// sums, offs (scan::blocks(tiles) and one more).
 void place_items(const StencilTiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s,
                  bool counted, int *sums, int *offs) {
     const uint32_t gp = uint32_t((n + kT - 1) / kT);
