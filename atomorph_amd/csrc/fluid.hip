@@ -1333,72 +1333,57 @@ __global__ __launch_bounds__(kT) void k_sort_count(Particles P, int n, int gsx, 
     sort_rank(key, in, i, cell, pos, cnt);
 }
 
-// Slot s of particle i in the cell order: S.atom[s] = i, and slot[i] = s
-// (slot may alias pos: each thread reads its own entry first).  The
-// buckets' counts, scanned already, are cleared for the next frame's sort.
-__global__ __launch_bounds__(kT) void k_sort_slots(int n, const int32_t *__restrict__ cell, const int32_t *pos,
-                                                  const int32_t *__restrict__ start, int32_t *slot, Particles S,
-                                                  int32_t *__restrict__ cnt, int64_t nk) {
+// The new slot t of the particle in slot i of P: S.atom[t] = P.atom[i]
+// and prev[t] = i.  The buckets' counts, scanned already, are cleared for
+// the next frame's sort.
+__global__ __launch_bounds__(kT) void k_sort_slots(int n, const int32_t *__restrict__ cell,
+                                                  const int32_t *__restrict__ pos, const int32_t *__restrict__ start,
+                                                  const int32_t *__restrict__ atom, int32_t *__restrict__ prev,
+                                                  Particles S, int32_t *__restrict__ cnt, int64_t nk) {
     const int i = blockIdx.x * kT + threadIdx.x;
     for (int64_t k = i; k < nk; k += int64_t(gridDim.x) * kT) cnt[k] = 0;
     if (i >= n) return;
-    const int s = start[cell[i]] + pos[i];
-    S.atom[s] = i;
-    slot[i] = s;
+    const int t = start[cell[i]] + pos[i];
+    S.atom[t] = atom[i];
+    prev[t] = i;
 }
 
-// S = P in cell order, one thread per slot: the 18 fields are written
-// coalesced and read through the slot's particle (a scattered store per
-// field moved whole cache lines for 8 bytes).  par >= 0: the frame's first
-// step's bin entries are counted too (as k_bin_count does, into B.count[par]
-// and B.rank), from the position just gathered.
-__global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, Particles S, int gsx, int gsy, Bins B,
-                                                   int par) {
-    const int s = blockIdx.x * kT + threadIdx.x;
-    const bool in = s < n;
+// S = P in cell order, one thread per new slot t reading slot prev[t] of P:
+// the fields are written coalesced, and since P is in the last frame's cell
+// order the reads of neighbouring slots mostly share cache lines.  Inactive
+// particles are copied too (S becomes the particle state, and their values
+// are kept while they rest).  par >= 0: the frame's first step's bin
+// entries are counted too (as k_bin_count does, into B.count[par] and
+// B.rank), from the position just gathered.
+__global__ __launch_bounds__(kT) void k_sort_gather(Particles P, int n, const int32_t *__restrict__ prev, Particles S,
+                                                   int gsx, int gsy, Bins B, int par) {
+    const int t = blockIdx.x * kT + threadIdx.x;
+    const bool in = t < n;
     int act = 0;
     double x = 0.0, y = 0.0;
     if (in) {
-        const int i = S.atom[s];
+        const int i = prev[t];
         act = P.active[i];
-        S.active[s] = act;
-    }
-    // an inactive particle's slot keeps only its flag: every reader of the
-    // other fields (steps, splat, scatter back) skips inactive slots, and the
-    // inactive particles fill the last slots, so whole waves skip the gather
-    if (act) {
-        const int i = S.atom[s];
+        S.active[t] = act;
         // one field at a time: with all 18 random gathers of a wave in flight at
-        // once the L2 held fewer of their lines (36 against 30 us per frame)
+        // once the L2 held fewer of their lines
         x = P.x[i];
         y = P.y[i];
-        S.x[s] = x; S.y[s] = y; S.u[s] = P.u[i]; S.v[s] = P.v[i];
-        S.gx[s] = P.gx[i]; S.gy[s] = P.gy[i]; S.fr[s] = P.fr[i];
-        S.R[s] = P.R[i]; S.G[s] = P.G[i]; S.B[s] = P.B[i]; S.A[s] = P.A[i];
-        S.r[s] = P.r[i]; S.g[s] = P.g[i]; S.b[s] = P.b[i]; S.a[s] = P.a[i];
-        S.strength[s] = P.strength[i];
-        S.mature[s] = P.mature[i];
+        S.x[t] = x; S.y[t] = y; S.u[t] = P.u[i]; S.v[t] = P.v[i];
+        S.gx[t] = P.gx[i]; S.gy[t] = P.gy[i]; S.fr[t] = P.fr[i];
+        S.R[t] = P.R[i]; S.G[t] = P.G[i]; S.B[t] = P.B[i]; S.A[t] = P.A[i];
+        S.r[t] = P.r[i]; S.g[t] = P.g[i]; S.b[t] = P.b[i]; S.a[t] = P.a[i];
+        S.strength[t] = P.strength[i];
+        S.mature[t] = P.mature[i];
+        S.owner[t] = P.owner[i];
+        S.key[t] = P.key[i];
     }
     if (par < 0) return;  // uniform: no step follows
     int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
     if (in) stencil_tiles(x, y, gsx, gsy, tx0, tx1, ty0, ty1);
-    count_entries(in && act != 0, tx0, tx1, ty0, ty1, s, n, B, par);  // every lane of the wave
+    count_entries(in && act != 0, tx0, tx1, ty0, ty1, t, n, B, par);  // every lane of the wave
 }
 
-// What the steps and the splat change goes back to particle order, for
-// particle i from its slot (coalesced stores over consecutive i).
-struct ScatterBack {
-    Particles S;
-    const int32_t *slot;
-    Particles P;
-    __device__ void operator()(int i) const {
-        const int s = slot[i];
-        if (!S.active[s]) return;
-        P.x[i] = S.x[s]; P.y[i] = S.y[s]; P.u[i] = S.u[s]; P.v[i] = S.v[s];
-        P.r[i] = S.r[s]; P.g[i] = S.g[s]; P.b[i] = S.b[s]; P.a[i] = S.a[s];
-        P.mature[i] = S.mature[s];
-    }
-};
 
 
 }  // namespace
@@ -1407,12 +1392,17 @@ struct ScatterBack {
 struct amh_fluid_s {
     int32_t gsx = 0, gsy = 0, count = 0;
     void *buf = nullptr;       // one allocation for particles + grid + render
-    Particles P;               // particle i = atom i
-    Particles S;               // the frame's cell-ordered copy
+    // The particle state is kept in the order of the last cell sort: slot i
+    // of P holds atom P.atom[i] (the identity until the first sort).  A
+    // frame sorts P into S, steps and draws S, and the two swap roles.
+    Particles P;
+    Particles S;
+    int32_t swapped = 0;       // P and S have swapped roles an odd number of times
     Grid G;
     // cell ordering: per particle its cell and rank in it; per cell counts
     // and starts ([nn + kIdleBuckets + 1]: the inactive particles last)
     int32_t *cell = nullptr, *pos = nullptr, *cnt = nullptr, *start = nullptr;
+    int32_t *prev = nullptr;   // [n] per slot of the sorted copy: its slot in P before
     int32_t *scan_sums = nullptr, *scan_offs = nullptr;
     int32_t *cell_rank = nullptr;  // [nn] sort key of each cell (Morton order)
     Bins bins;                     // per-step binning by node tile
@@ -1456,7 +1446,7 @@ size_t fluid_bytes(int gsx, int gsy, int count, int W, int H, int nkeys) {
     const size_t ptiles = size_t((W + kPxTile - 1) / kPxTile) * size_t((H + kPxTile - 1) / kPxTile);
     (void)nkeys;
     // every carved array is rounded up to 256 bytes: the last term covers that
-    return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 2 + (nn + kIdleBuckets + 1) * 4 * 2 +
+    return 2 * (n * 8 * kParticleDoubles + n * 4 * kParticleInts) + n * 4 * 3 + (nn + kIdleBuckets + 1) * 4 * 2 +
            (2 * nb + 1) * 4 + nn * 4 + (3 * tiles + 1) * 4 + n * 4 * 4 + list_entries(tiles, n) * 4 + tiles * 4 +
            (3 * ptiles + 1) * 4 + n * (4 + 32 + 4) + max_chunks(tiles, n) * 16 + 4 + nn * 4 * kGridFields +
            npx * 4 * 2 + 256 + n * sizeof(LagSlopeMemo) + 128 * 256;
@@ -1498,6 +1488,7 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     carve_particles(f->S, n, take);
     f->cell = reinterpret_cast<int32_t *>(take(n * 4));
     f->pos = reinterpret_cast<int32_t *>(take(n * 4));
+    f->prev = reinterpret_cast<int32_t *>(take(n * 4));
     f->cnt = reinterpret_cast<int32_t *>(take((nn + kIdleBuckets + 1) * 4));
     f->start = reinterpret_cast<int32_t *>(take((nn + kIdleBuckets + 1) * 4));
     const size_t nb = size_t(scan::blocks(int64_t(nn) + kIdleBuckets));
@@ -1568,6 +1559,10 @@ extern "C" int amh_fluid_create(void **fluid, int32_t gsx, int32_t gsy, int32_t 
     int r = check_hip(hipMemsetAsync(f->buf, 0, bytes, st), "amh_fluid_create");
     if (!r) r = check_hip(hipMemcpyAsync(f->cell_rank, rank.data(), nn * 4, hipMemcpyHostToDevice, st),
                           "amh_fluid_create");
+    std::vector<int32_t> iota(n);
+    for (size_t i = 0; i < n; ++i) iota[i] = int32_t(i);
+    if (!r && n) r = check_hip(hipMemcpyAsync(f->P.atom, iota.data(), n * 4, hipMemcpyHostToDevice, st),
+                               "amh_fluid_create");
     if (!r) r = check_hip(hipStreamSynchronize(st), "amh_fluid_create");
     if (r) {
         (void)hipFree(f->buf);
@@ -1611,6 +1606,10 @@ extern "C" int amh_fluid_set_particles(void *fluid, const double *host, void *st
         if (int r = check_hip(hipMemcpyAsync(idst[k], ic.data(), n * 4, hipMemcpyHostToDevice, s), "set")) return r;
         if (int r = check_hip(hipStreamSynchronize(s), "set")) return r;
     }
+    // the state is now in atom order: slot i holds atom i
+    for (size_t i = 0; i < n; ++i) ic[i] = int32_t(i);
+    if (int r = check_hip(hipMemcpyAsync(f->P.atom, ic.data(), n * 4, hipMemcpyHostToDevice, s), "set")) return r;
+    if (int r = check_hip(hipStreamSynchronize(s), "set")) return r;
     return AMH_OK;
 }
 
@@ -1621,18 +1620,27 @@ extern "C" int amh_fluid_get_particles(void *fluid, double *host, void *stream) 
     std::vector<double> col(n);
     std::vector<int32_t> ic(n);
     hipStream_t s = static_cast<hipStream_t>(stream);
+    // the state is in slot order: slot i holds atom P.atom[i]
+    std::vector<int32_t> atom(n);
+    if (int r = check_hip(hipMemcpyAsync(atom.data(), f->P.atom, n * 4, hipMemcpyDeviceToHost, s), "get")) return r;
+    if (int r = check_hip(hipStreamSynchronize(s), "get")) return r;
+    for (size_t i = 0; i < n; ++i)
+        if (atom[i] < 0 || size_t(atom[i]) >= n) {
+            set_error("amh_fluid_get_particles: slot %zu holds atom %d of %zu", i, atom[i], n);
+            return AMH_E_ARG;
+        }
     const double *src[] = {f->P.x, f->P.y, f->P.u, f->P.v, f->P.gx, f->P.gy, f->P.fr, f->P.R, f->P.G, f->P.B,
                            f->P.A, f->P.r, f->P.g, f->P.b, f->P.a, f->P.strength};
     for (int k = 0; k < 16; ++k) {
         if (int r = check_hip(hipMemcpyAsync(col.data(), src[k], n * 8, hipMemcpyDeviceToHost, s), "get")) return r;
         if (int r = check_hip(hipStreamSynchronize(s), "get")) return r;
-        for (size_t i = 0; i < n; ++i) host[20 * i + k] = col[i];
+        for (size_t i = 0; i < n; ++i) host[20 * size_t(atom[i]) + k] = col[i];
     }
     const int32_t *isrc[] = {f->P.active, f->P.mature};
     for (int k = 0; k < 2; ++k) {
         if (int r = check_hip(hipMemcpyAsync(ic.data(), isrc[k], n * 4, hipMemcpyDeviceToHost, s), "get")) return r;
         if (int r = check_hip(hipStreamSynchronize(s), "get")) return r;
-        for (size_t i = 0; i < n; ++i) host[20 * i + 16 + k] = ic[i];
+        for (size_t i = 0; i < n; ++i) host[20 * size_t(atom[i]) + 16 + k] = ic[i];
     }
     return AMH_OK;
 }
@@ -1724,7 +1732,7 @@ SortCount sort_count_of(amh_fluid_s *f) {
     return SortCount{f->cell_rank, f->cell, f->pos, f->cnt, f->gsx, f->gsy};
 }
 
-// f->S = f->P in cell order (f->S.atom maps slots back to particles).  The
+// f->S = f->P in cell order (f->S.atom: the atom in each slot).  The
 // bucket counts f->cnt are zero on entry (from the handle's creation, then
 // from the last sort's k_sort_slots).  count_first: the first step's bins
 // are counted on the way (its parity f->bin_par).  counted: the buckets are
@@ -1738,22 +1746,22 @@ int sort_particles(amh_fluid_s *f, hipStream_t s, bool count_first, bool counted
         hipLaunchKernelGGL(k_sort_count, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->gsx, f->gsy, f->cell_rank,
                            f->cell, f->pos, f->cnt);
     scan::exclusive(f->cnt, nk, f->scan_sums, f->scan_offs, f->start, s);
-    // f->pos becomes each particle's slot (ScatterBack reads it after the steps)
-    hipLaunchKernelGGL(k_sort_slots, dim3(gp), dim3(kT), 0, s, f->count, f->cell, f->pos, f->start, f->pos, f->S,
-                       f->cnt, nk);
-    hipLaunchKernelGGL(k_sort_gather, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->S, f->gsx, f->gsy, f->bins,
-                       count_first ? f->bin_par : -1);
+    hipLaunchKernelGGL(k_sort_slots, dim3(gp), dim3(kT), 0, s, f->count, f->cell, f->pos, f->start, f->P.atom,
+                       f->prev, f->S, f->cnt, nk);
+    hipLaunchKernelGGL(k_sort_gather, dim3(gp), dim3(kT), 0, s, f->P, f->count, f->prev, f->S, f->gsx, f->gsy,
+                       f->bins, count_first ? f->bin_par : -1);
     AMH_LAUNCH_CHECK("fluid sort");
     return AMH_OK;
 }
 }  // namespace
 
 // One FluidModel::step (fluidmodel.cpp:165-580) on the particles as they are
-// stored (particle order).
+// stored (in the order of the last cell sort; the wall jitter is keyed by
+// atom).
 extern "C" int amh_fluid_step(void *fluid, int32_t steps_left, double freedom_radius, uint64_t seed, void *stream) {
     amh_fluid_s *f = static_cast<amh_fluid_s *>(fluid);
     if (!f) { set_error("amh_fluid_step: NULL"); return AMH_E_ARG; }
-    return fluid_step_on(f, f->P, nullptr, steps_left, freedom_radius, seed, static_cast<hipStream_t>(stream));
+    return fluid_step_on(f, f->P, f->P.atom, steps_left, freedom_radius, seed, static_cast<hipStream_t>(stream));
 }
 
 // ----------------------------------------------------------- frame path
@@ -1785,28 +1793,28 @@ __device__ __forceinline__ uint64_t lag_slope_tag(const amh_render &R, uint64_t 
     return mix64(h ^ box) | 1u;
 }
 
-// update_particle for atom a of chain c (morph.cpp:680-1050); returns the
-// particle's sort bucket when sc.cnt is set (its cell's rank, or an idle
-// bucket when it is not active).  lag, slope: the atom's PERLIN lag and
+// update_particle for the particle in slot s of P, atom a of chain c
+// (morph.cpp:680-1050); returns its sort bucket when sc.cnt is set (its
+// cell's rank, or an idle bucket when it is not active).  lag, slope: the atom's PERLIN lag and
 // slope (atom_lag_slope) when the fade is PERLIN.
 __device__ __forceinline__ int update_one(const amh_render &R, const Frame &F, Particles P,
                                           const int32_t *__restrict__ rank, const int32_t *__restrict__ surf,
-                                          const double *__restrict__ bcol, double sp, int64_t a, int c,
+                                          const double *__restrict__ bcol, double sp, int s, int64_t a, int c,
                                           const SortCount &sc, double lag, double slope) {
     // every load first: the compiler must assume the stores to P below may
     // alias the other arrays, and would wait for each load in turn
     const int nA = surf[int64_t(F.fk) * R.nchains + c], nB = surf[int64_t(F.fnext) * R.nchains + c];
     const int rk = rank[int64_t(F.fk) * R.wtot + a];
     const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a], p2 = R.pts_dev[int64_t(F.fnext) * R.wtot + a];
-    const int act0 = P.active[a], key0 = P.key[a], mat0 = P.mature[a], own0 = P.owner[a];
-    const double r0 = P.r[a], g0 = P.g[a], b0 = P.b[a], a0 = P.a[a];
-    const double x0 = sc.cnt ? P.x[a] : 0.0, y0 = sc.cnt ? P.y[a] : 0.0;
+    const int act0 = P.active[s], key0 = P.key[s], mat0 = P.mature[s], own0 = P.owner[s];
+    const double r0 = P.r[s], g0 = P.g[s], b0 = P.b[s], a0 = P.a[s];
+    const double x0 = sc.cnt ? P.x[s] : 0.0, y0 = sc.cnt ? P.y[s] : 0.0;
     uint64_t tk[4] = {p1, p1, p2, p2};
     traj_keys(R, F, a, tk);
     const int limit = int(round((1.0 - F.tl) * double(nA) + F.tl * double(nB)));
     if (rk >= limit) {
-        P.active[a] = 0;
-        return idle_bucket(int(a), sc.gsx * sc.gsy);
+        P.active[s] = 0;
+        return idle_bucket(s, sc.gsx * sc.gsy);
     }
     // a new particle (Particle::clear, fluidmodel.cpp:80-95)
     const bool fresh = !(act0 && key0 == F.fk);
@@ -1838,31 +1846,47 @@ __device__ __forceinline__ int update_one(const amh_render &R, const Frame &F, P
     double r = t * Rr + (1.0 - t) * r0, g = t * Gg + (1.0 - t) * g0;
     double b = t * Bb + (1.0 - t) * b0, al = t * Aa + (1.0 - t) * a0;
     if (fresh) {
-        P.u[a] = 0.0; P.v[a] = 0.0;
-        P.mature[a] = 0;
-        P.owner[a] = 1;
-        P.key[a] = F.fk;
+        P.u[s] = 0.0; P.v[s] = 0.0;
+        P.mature[s] = 0;
+        P.owner[s] = 1;
+        P.key[s] = F.fk;
     }
-    P.gx[a] = gxv;
-    P.gy[a] = gyv;
-    P.R[a] = Rr; P.G[a] = Gg; P.B[a] = Bb; P.A[a] = Aa;
+    P.gx[s] = gxv;
+    P.gy[s] = gyv;
+    P.R[s] = Rr; P.G[s] = Gg; P.B[s] = Bb; P.A[s] = Aa;
     const bool placed = !mature && owner;  // the particle moves to its target
     if (!mature) {  // (a fresh particle's strength 1.0 is replaced here)
         if (owner) {
-            P.x[a] = gxv;
-            P.y[a] = gyv;
+            P.x[s] = gxv;
+            P.y[s] = gyv;
             r = Rr; g = Gg; b = Bb; al = Aa;
         }
-        P.strength[a] = nA == 0 ? 0.1 : 1.0;
-        P.fr[a] = 1.0;
+        P.strength[s] = nA == 0 ? 0.1 : 1.0;
+        P.fr[s] = 1.0;
     }
-    P.r[a] = r; P.g[a] = g; P.b[a] = b; P.a[a] = al;
-    P.active[a] = 1;
+    P.r[s] = r; P.g[s] = g; P.b[s] = b; P.a[s] = al;
+    P.active[s] = 1;
     return sc.cnt ? sc.cell_rank[cell_of(placed ? gxv : x0, placed ? gyv : y0, sc.gsx, sc.gsy)] : 0;
 }
 
+// Chains whose first atoms the update stages in LDS (more: searched in
+// global memory).
+constexpr int kChainLds = 4096;
+
+// The last k with v[k] <= x (v ascending, v[0] <= x).
+__device__ __forceinline__ int last_at_or_below(const int32_t *v, int n, int32_t x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (v[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // step_fluid's bookkeeping and update_particle (morph.cpp:680-1050), one
-// thread per atom; with sc.cnt set, also the counting pass of the cell sort
+// thread per slot of P (the particles in the last cell order; P.atom the
+// atom in each); with sc.cnt set, also the counting pass of the cell sort
 // that follows (as k_sort_count).
 __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FrameParams *__restrict__ fp, Particles P,
                                                     const int32_t *__restrict__ rank,
@@ -1872,6 +1896,7 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
     const Frame F = fp->F;
     __shared__ int lagp[512], slopep[512];
     __shared__ uint64_t wsum[kT / 64];
+    __shared__ int32_t choff[kChainLds];
     uint64_t tag = 0;
     if (R.fading == kPerlin) {  // (uniform)
         uint64_t hsum = 0;
@@ -1883,10 +1908,16 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
         }
         tag = lag_slope_tag(R, hsum, wsum);  // (its barrier also publishes the tables)
     }
+    // the chains' first atoms, for the chain of each slot's atom (the slots
+    // are in the last cell order, so a block's atoms are not consecutive)
+    const bool lds_chains = R.nchains <= kChainLds;
+    if (lds_chains)
+        for (int k = threadIdx.x; k < R.nchains; k += kT) choff[k] = int32_t(R.chain_off_dev[k]);
     __syncthreads();
-    const int64_t a = int64_t(blockIdx.x) * kT + threadIdx.x;
-    const int c = block_chain(R.chain_off_dev, R.nchains, a);
-    const bool in = a < R.wtot;
+    const int s = int(blockIdx.x) * kT + int(threadIdx.x);
+    const bool in = s < R.wtot;
+    const int64_t a = in ? P.atom[s] : 0;
+    const int c = lds_chains ? last_at_or_below(choff, R.nchains, int32_t(a)) : find_chain(R.chain_off_dev, R.nchains, a);
     // PERLIN: the atom's lag and slope from memo[a] when it holds them for
     // this source point and tag, else computed and kept there (on its own,
     // before the update's loads, so few values are live around the noise)
@@ -1901,8 +1932,8 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
             memo[a] = LagSlopeMemo{lag, slope, p1, tag};
         }
     }
-    const int key = in ? update_one(R, F, P, rank, surf, bcol, sp, a, c, sc, lag, slope) : -1;
-    if (sc.cnt) sort_rank(key, in, int(a), sc.cell, sc.pos, sc.cnt);  // uniform; every lane of the wave
+    const int key = in ? update_one(R, F, P, rank, surf, bcol, sp, s, a, c, sc, lag, slope) : -1;
+    if (sc.cnt) sort_rank(key, in, s, sc.cell, sc.pos, sc.cnt);  // uniform; every lane of the wave
 }
 
 // draw_fluid, splat (morph.cpp:1080-1167), in two passes without atomics
@@ -2163,9 +2194,7 @@ void draw_splat(amh_fluid_s *f, const amh_render &R, double sparseness, hipStrea
     if (!prepped)
         hipLaunchKernelGGL(k_splat_prep, dim3(gp), dim3(kT), 0, s, f->S, f->count, f->fp,
                            splat_args(f, R, sparseness), f->pbins, par);
-    // the placing also scatters the particles back (f->pos: each one's slot)
-    bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true,
-                  ScatterBack{f->S, f->pos, f->P});
+    bin_particles(SplatTiles{f->sxy, f->W, f->H}, f->count, f->pbins, ptiles, par, s, true);
     hipLaunchKernelGGL(k_splat_tile, dim3(uint32_t(ptiles)), dim3(kT), 0, s, f->pbins, f->sxy, f->staps, f->scol,
                        f->W, f->H, f->fp, R.keep_background, f->img, f->mask);
 }
@@ -2181,6 +2210,12 @@ void draw_finish(amh_fluid_s *f, const amh_render &R, hipStream_t s) {
         hipLaunchKernelGGL(k_fluid_commit, dim3(gpx), dim3(kT), 0, s, f->mask, npx);
     }
     hipLaunchKernelGGL(k_fluid_compose, dim3(gpx), dim3(kT), 0, s, R, f->fp, f->img, f->mask);
+}
+
+// Once a frame's launches on f->S are enqueued, S holds the particle state.
+void swap_state(amh_fluid_s *f) {
+    std::swap(f->P, f->S);
+    f->swapped ^= 1;
 }
 
 // One frame's launches after its values are in f->fp: step_fluid's
@@ -2204,6 +2239,7 @@ int enqueue_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
             if (r) return r;
         }
         draw_splat(f, R, fa->sparseness, s, fa->fluidsteps > 0);
+        swap_state(f);
     } else {
         const int64_t npx = int64_t(f->W) * f->H;
         hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((npx + kT - 1) / kT)), dim3(kT), 0, s, f->mask, npx);
@@ -2217,7 +2253,7 @@ int enqueue_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
 std::vector<unsigned char> frame_key(const amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn) {
     amh_fluid_frame fc = *fa;
     fc.r = nullptr;
-    const int32_t state[4] = {drawn ? 1 : 0, f->pbin_par, f->bin_par, f->count};
+    const int32_t state[5] = {drawn ? 1 : 0, f->pbin_par, f->bin_par, f->count, f->swapped};
     std::vector<unsigned char> k(sizeof(amh_render) + sizeof(fc) + sizeof(state));
     std::memcpy(k.data(), fa->r, sizeof(amh_render));
     std::memcpy(k.data() + sizeof(amh_render), &fc, sizeof(fc));
@@ -2245,12 +2281,16 @@ bool replay_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
             (void)hipGetLastError();
             return false;
         }
-        const int32_t p0 = f->pbin_par, b0 = f->bin_par;
+        const int32_t p0 = f->pbin_par, b0 = f->bin_par, w0 = f->swapped;
+        const Particles P0 = f->P, S0 = f->S;
         const int r = enqueue_frame(f, fa, drawn, s);
         hipGraph_t graph = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &graph);
         f->pbin_par = p0;  // the replay below advances them
         f->bin_par = b0;
+        f->P = P0;
+        f->S = S0;
+        f->swapped = w0;
         hipGraphExec_t exec = nullptr;
         if (r || e != hipSuccess || !graph || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -2268,6 +2308,7 @@ bool replay_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
     *rc = check_hip(hipGraphLaunch(hit->exec, s), "amh_fluid_render graph");
     f->pbin_par = pbin_after;
     f->bin_par = bin_after;
+    if (drawn) swap_state(f);
     return true;
 }
 }  // namespace
@@ -2350,6 +2391,7 @@ extern "C" int amh_fluid_draw(void *fluid, const amh_fluid_frame *fa, double t, 
     if (v.F.fk < R.nkeys && f->count > 0) {
         if (int r = sort_particles(f, s, false, false)) return r;  // no step follows: nothing to count
         draw_splat(f, R, fa->sparseness, s, false);
+        swap_state(f);
     } else {
         const int64_t npx = int64_t(f->W) * f->H;
         hipLaunchKernelGGL(k_fill_i32, dim3(uint32_t((npx + kT - 1) / kT)), dim3(kT), 0, s, f->mask, npx);
