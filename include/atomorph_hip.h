@@ -327,8 +327,10 @@ int amh_fluid_destroy(void *fluid);
 /* Particles back to the state morph::synchronize gives a new model
  * (morph.cpp:243-262): centred, inactive, green. */
 int amh_fluid_reset(void *fluid, double sparseness, void *stream);
-/* Particle state as 20 doubles per particle: x y u v gravity_x gravity_y
- * freedom_r R G B A r g b a strength active mature, 2 unused (synchronous). */
+/* Particle state as 20 doubles per particle, particle i = atom i: x y u v
+ * gravity_x gravity_y freedom_r R G B A r g b a strength active mature, 2
+ * unused (synchronous).  The model keeps its particles in the order of its
+ * last cell sort; these two convert to and from atom order. */
 int amh_fluid_set_particles(void *fluid, const double *host, void *stream);
 int amh_fluid_get_particles(void *fluid, double *host, void *stream);
 /* One FluidModel::step (fluidmodel.cpp:165-580); the wall bounce jitter
