@@ -2084,17 +2084,33 @@ __global__ __launch_bounds__(kT) void k_fluid_compose(amh_render R, const FrameP
 // of key segment y -> y+1: slots ordered by class (HAS_FLUID in both rows,
 // only in row y, only in row y+1, in neither), then by a pseudo-random
 // permutation of the slots keyed by (seed, chain, y).  One block per
-// (y, chain): pass 1 counts the classes, pass 2 walks the slots in
-// permutation order and hands out ranks with a block-wide ballot scan.
-constexpr int kRankT = 256;
+// (y, chain).
+constexpr int kRankT = 1024;  // the longest chain's block sets the time: its Feistel walks are ALU work
 constexpr int kRankWaves = kRankT / 64;
 
-__device__ __forceinline__ int slot_class(const uint64_t *__restrict__ pts, int64_t wtot, int y, int yn, int64_t a) {
-    const bool f1 = pt_flags(pts[int64_t(y) * wtot + a]) & HAS_FLUID;
-    const bool f2 = pt_flags(pts[int64_t(yn) * wtot + a]) & HAS_FLUID;
-    return f1 && f2 ? 0 : f1 ? 1 : f2 ? 2 : 3;
+// The classes of up to kRankBatch consecutive permutation places [p, p1)
+// (4 past the end): every place's two loads are issued before the first is
+// used.
+constexpr int kRankBatch = 8;
+__device__ __forceinline__ void rank_batch(const uint64_t *__restrict__ pts, int64_t wtot, int y, int yn,
+                                           int64_t o, uint32_t w, uint64_t key, int p, int p1,
+                                           uint32_t (&x)[kRankBatch], int (&cls)[kRankBatch]) {
+    uint64_t q1[kRankBatch], q2[kRankBatch];
+    for (int j = 0; j < kRankBatch; ++j) {
+        x[j] = p + j < p1 ? feistel_perm(uint32_t(p + j), w, key) : 0u;
+        q1[j] = pts[int64_t(y) * wtot + o + x[j]];
+        q2[j] = pts[int64_t(yn) * wtot + o + x[j]];
+    }
+    for (int j = 0; j < kRankBatch; ++j) {
+        const bool f1 = pt_flags(q1[j]) & HAS_FLUID, f2 = pt_flags(q2[j]) & HAS_FLUID;
+        cls[j] = p + j >= p1 ? 4 : f1 && f2 ? 0 : f1 ? 1 : f2 ? 2 : 3;
+    }
 }
 
+// Thread t takes the run of permutation places [t J, (t + 1) J), J =
+// ceil(w / kRankT): it counts its run's classes (loads in batches), a block
+// scan gives each class's first rank in the run, and a second walk hands
+// the ranks out in place order.
 __global__ __launch_bounds__(kRankT) void k_fluid_ranks(const uint64_t *__restrict__ pts, int h, int64_t wtot,
                                                         const int64_t *__restrict__ off,
                                                         const int32_t *__restrict__ cw, int nchains, uint64_t seed,
@@ -2105,54 +2121,44 @@ __global__ __launch_bounds__(kRankT) void k_fluid_ranks(const uint64_t *__restri
     if (w <= 0) return;
     const uint64_t key = mix64(seed ^ (uint64_t(uint32_t(c)) << 32) ^ uint64_t(uint32_t(y)));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    __shared__ int base[4];
-    __shared__ int wcnt[kRankWaves][4];
-    if (threadIdx.x < 4) base[threadIdx.x] = 0;
-    __syncthreads();
-    // pass 1: slots per class
+    __shared__ int wtot4[kRankWaves][4];
+    const int J = (w + kRankT - 1) / kRankT;
+    const int p0 = min(w, int(threadIdx.x) * J), p1 = min(w, p0 + J);
+    uint32_t x[kRankBatch];
+    int cls[kRankBatch];
     int cnt[4] = {0, 0, 0, 0};
-    for (int p = threadIdx.x; p < w; p += kRankT) cnt[slot_class(pts, wtot, y, yn, o + feistel_perm(uint32_t(p), uint32_t(w), key))]++;
+    for (int p = p0; p < p1; p += kRankBatch) {
+        rank_batch(pts, wtot, y, yn, o, uint32_t(w), key, p, p1, x, cls);
+        for (int j = 0; j < kRankBatch; ++j)
+            for (int k = 0; k < 4; ++k) cnt[k] += cls[j] == k;
+    }
+    // exclusive scan of the four counts over the threads, in thread order
+    int incl[4];
     for (int k = 0; k < 4; ++k) {
-        int v = cnt[k];
-        for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
-        if (lane == 0) atomicAdd(&base[k], v);
+        incl[k] = cnt[k];
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl[k], d, 64);
+            if (lane >= d) incl[k] += v;
+        }
+        if (lane == 63) wtot4[wave][k] = incl[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0) {  // exclusive scan of the class totals
-        int run = 0;
-        for (int k = 0; k < 4; ++k) {
-            const int t = base[k];
-            base[k] = run;
-            run += t;
+    int next[4];
+    int classbase = 0;
+    for (int k = 0; k < 4; ++k) {
+        int before = 0, total = 0;
+        for (int v = 0; v < kRankWaves; ++v) {
+            before += v < wave ? wtot4[v][k] : 0;
+            total += wtot4[v][k];
         }
+        next[k] = classbase + before + incl[k] - cnt[k];
+        classbase += total;
     }
-    __syncthreads();
-    // pass 2: ranks in permutation order
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (int start = 0; start < w; start += kRankT) {
-        const int p = start + threadIdx.x;
-        const bool valid = p < w;
-        const uint32_t x = valid ? feistel_perm(uint32_t(p), uint32_t(w), key) : 0u;
-        const int cls = valid ? slot_class(pts, wtot, y, yn, o + x) : 4;
-        int mine = 0;
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t m = __ballot(cls == k);
-            if (cls == k) mine = __popcll(m & below);
-            if (lane == 0) wcnt[wave][k] = __popcll(m);
-        }
-        __syncthreads();
-        if (valid) {
-            int r = base[cls] + mine;
-            for (int v = 0; v < wave; ++v) r += wcnt[v][cls];
-            rank[int64_t(y) * wtot + o + x] = r;
-        }
-        __syncthreads();
-        if (threadIdx.x < 4) {
-            int add = 0;
-            for (int v = 0; v < kRankWaves; ++v) add += wcnt[v][threadIdx.x];
-            base[threadIdx.x] += add;
-        }
-        __syncthreads();
+    for (int p = p0; p < p1; p += kRankBatch) {
+        rank_batch(pts, wtot, y, yn, o, uint32_t(w), key, p, p1, x, cls);
+        for (int j = 0; j < kRankBatch; ++j)
+            for (int k = 0; k < 4; ++k)
+                if (cls[j] == k) rank[int64_t(y) * wtot + o + x[j]] = next[k]++;
     }
 }
 
