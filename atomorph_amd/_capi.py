@@ -51,7 +51,7 @@ class RenderArgs(C.Structure):
 
 class FluidFrame(C.Structure):
     _fields_ = [("r", C.POINTER(RenderArgs)), ("rank_dev", vp), ("surface_dev", vp), ("blob_col_dev", vp),
-                ("sparseness", f64), ("fluidsteps", i32), ("pad_", i32)]
+                ("sparseness", f64), ("fluidsteps", i32), ("pad_", i32), ("atom_keys_dev", vp)]
 
 
 # name -> (restype, argtypes); every entry point declared in include/*.h
@@ -93,6 +93,7 @@ PROTOTYPES = {
     "amh_graph_launch": (C.c_int, [vp, vp]),
     "amh_graph_destroy": (C.c_int, [vp]),
     "amh_fluid_ranks": (C.c_int, [vp, i32, i64, vp, vp, i32, u64, vp, vp]),
+    "amh_fluid_atom_keys": (C.c_int, [vp, i32, i64, vp, vp]),
     "amh_refine_autotune": (C.c_int, [vp, vp, i32, i64, vp, vp, vp, i32, f64, u64, i32, C.POINTER(i32),
                                       C.POINTER(i32), vp, i32, C.POINTER(i32), vp]),
     "amh_rgb_to_hsp": (u32, [u32]),

@@ -1795,22 +1795,27 @@ __device__ __forceinline__ uint64_t lag_slope_tag(const amh_render &R, uint64_t 
 
 // update_particle for the particle in slot s of P, atom a of chain c
 // (morph.cpp:680-1050); returns its sort bucket when sc.cnt is set (its
-// cell's rank, or an idle bucket when it is not active).  lag, slope: the atom's PERLIN lag and
+// cell's rank, or an idle bucket when it is not active).  ka: the atom's
+// key points in one record (atom-major copy), else NULL (read from
+// R.pts_dev, one row per key).  lag, slope: the atom's PERLIN lag and
 // slope (atom_lag_slope) when the fade is PERLIN.
 __device__ __forceinline__ int update_one(const amh_render &R, const Frame &F, Particles P,
                                           const int32_t *__restrict__ rank, const int32_t *__restrict__ surf,
                                           const double *__restrict__ bcol, double sp, int s, int64_t a, int c,
-                                          const SortCount &sc, double lag, double slope) {
+                                          const SortCount &sc, double lag, double slope,
+                                          const uint64_t *__restrict__ ka) {
     // every load first: the compiler must assume the stores to P below may
     // alias the other arrays, and would wait for each load in turn
     const int nA = surf[int64_t(F.fk) * R.nchains + c], nB = surf[int64_t(F.fnext) * R.nchains + c];
     const int rk = rank[int64_t(F.fk) * R.wtot + a];
-    const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a], p2 = R.pts_dev[int64_t(F.fnext) * R.wtot + a];
+    const uint64_t p1 = ka ? ka[F.fk] : R.pts_dev[int64_t(F.fk) * R.wtot + a];
+    const uint64_t p2 = ka ? ka[F.fnext] : R.pts_dev[int64_t(F.fnext) * R.wtot + a];
     const int act0 = P.active[s], key0 = P.key[s], mat0 = P.mature[s], own0 = P.owner[s];
     const double r0 = P.r[s], g0 = P.g[s], b0 = P.b[s], a0 = P.a[s];
     const double x0 = sc.cnt ? P.x[s] : 0.0, y0 = sc.cnt ? P.y[s] : 0.0;
     uint64_t tk[4] = {p1, p1, p2, p2};
-    traj_keys(R, F, a, tk);
+    if (ka) traj_keys_of(R, F, ka, tk);
+    else traj_keys(R, F, a, tk);
     const int limit = int(round((1.0 - F.tl) * double(nA) + F.tl * double(nB)));
     if (rk >= limit) {
         P.active[s] = 0;
@@ -1892,7 +1897,8 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
                                                     const int32_t *__restrict__ rank,
                                                     const int32_t *__restrict__ surf,
                                                     const double *__restrict__ bcol, double sp, SortCount sc,
-                                                    LagSlopeMemo *__restrict__ memo) {
+                                                    LagSlopeMemo *__restrict__ memo,
+                                                    const uint64_t *__restrict__ atom_keys) {
     const Frame F = fp->F;
     __shared__ int lagp[512], slopep[512];
     __shared__ uint64_t wsum[kT / 64];
@@ -1917,13 +1923,14 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
     const int s = int(blockIdx.x) * kT + int(threadIdx.x);
     const bool in = s < R.wtot;
     const int64_t a = in ? P.atom[s] : 0;
+    const uint64_t *ka = atom_keys ? atom_keys + a * R.nkeys : nullptr;
     const int c = lds_chains ? last_at_or_below(choff, R.nchains, int32_t(a)) : find_chain(R.chain_off_dev, R.nchains, a);
     // PERLIN: the atom's lag and slope from memo[a] when it holds them for
     // this source point and tag, else computed and kept there (on its own,
     // before the update's loads, so few values are live around the noise)
     double lag = 0.5, slope = 0.5;
     if (tag != 0 && in) {
-        const uint64_t p1 = R.pts_dev[int64_t(F.fk) * R.wtot + a];
+        const uint64_t p1 = ka ? ka[F.fk] : R.pts_dev[int64_t(F.fk) * R.wtot + a];
         const LagSlopeMemo mm = memo[a];
         lag = mm.lag;
         slope = mm.slope;
@@ -1932,7 +1939,7 @@ __global__ __launch_bounds__(kT) void k_fluid_update(amh_render R, const FramePa
             memo[a] = LagSlopeMemo{lag, slope, p1, tag};
         }
     }
-    const int key = in ? update_one(R, F, P, rank, surf, bcol, sp, s, a, c, sc, lag, slope) : -1;
+    const int key = in ? update_one(R, F, P, rank, surf, bcol, sp, s, a, c, sc, lag, slope, ka) : -1;
     if (sc.cnt) sort_rank(key, in, s, sc.cell, sc.pos, sc.cnt);  // uniform; every lane of the wave
 }
 
@@ -2164,6 +2171,31 @@ __global__ __launch_bounds__(kRankT) void k_fluid_ranks(const uint64_t *__restri
 
 }  // namespace
 
+namespace {
+__global__ __launch_bounds__(kT) void k_atom_keys(const uint64_t *__restrict__ pts, int h, int64_t wtot,
+                                                 uint64_t *__restrict__ out) {
+    const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+    if (i >= wtot * h) return;
+    const int64_t a = i / h;
+    const int y = int(i - a * h);
+    out[i] = pts[int64_t(y) * wtot + a];
+}
+}  // namespace
+
+extern "C" int amh_fluid_atom_keys(const uint64_t *pts_dev, int32_t nkeys, int64_t wtot, uint64_t *out_dev,
+                                   void *stream) {
+    if (!pts_dev || !out_dev || nkeys <= 0 || wtot < 0 || wtot > INT32_MAX) {
+        set_error("amh_fluid_atom_keys: invalid argument");
+        return AMH_E_ARG;
+    }
+    const int64_t total = wtot * nkeys;
+    if (total == 0) return AMH_OK;
+    hipLaunchKernelGGL(k_atom_keys, dim3(uint32_t((total + kT - 1) / kT)), dim3(kT), 0,
+                       static_cast<hipStream_t>(stream), pts_dev, int(nkeys), wtot, out_dev);
+    AMH_LAUNCH_CHECK("amh_fluid_atom_keys");
+    return AMH_OK;
+}
+
 extern "C" int amh_fluid_ranks(const uint64_t *pts_dev, int32_t nkeys, int64_t wtot, const int64_t *chain_off_dev,
                                const int32_t *chain_w_dev, int32_t nchains, uint64_t seed, int32_t *rank_dev,
                                void *stream) {
@@ -2233,7 +2265,8 @@ int enqueue_frame(amh_fluid_s *f, const amh_fluid_frame *fa, bool drawn, hipStre
         const uint32_t gp = uint32_t((f->count + kT - 1) / kT);
         // the update counts the cell sort's buckets on the way
         amh::launch_timed("k_fluid_update", k_fluid_update, dim3(gp), dim3(kT), 0, s, R, f->fp, f->P, fa->rank_dev,
-                          fa->surface_dev, fa->blob_col_dev, fa->sparseness, sort_count_of(f), f->memo);
+                          fa->surface_dev, fa->blob_col_dev, fa->sparseness, sort_count_of(f), f->memo,
+                          fa->atom_keys_dev);
         // the steps and the splat run on the cell-ordered copy
         // the sort counts the first step's bins; each step's advection the next's
         if (int r = sort_particles(f, s, fa->fluidsteps > 0, true)) return r;

@@ -128,6 +128,17 @@ __device__ __forceinline__ void traj_keys(const amh_render &R, const Frame &F, i
     for (int k = 0; k < 4; ++k) q[k] = R.pts_dev[int64_t(wrap(pk - 1 + k)) * R.wtot + a];
 }
 
+// traj_keys from the atom's own record ka[nkeys] (its key points
+// atom-major).
+__device__ __forceinline__ void traj_keys_of(const amh_render &R, const Frame &F, const uint64_t *ka, uint64_t q[4]) {
+    if (R.motion != kSpline) return;
+    const int h = R.nkeys;
+    const double dt = 1.0 / h;
+    const int pk = (int)(F.tn / dt);
+    auto wrap = [h](int i) { return i < 0 ? h - 1 : (i >= h ? i - h : i); };
+    for (int k = 0; k < 4; ++k) q[k] = ka[wrap(pk - 1 + k)];
+}
+
 __device__ inline void traj_point(const amh_render &R, const Frame &F, int64_t a, uint64_t p1, uint64_t p2,
                                   uint32_t &qx, uint32_t &qy, uint32_t &qxf, uint32_t &qyf,
                                   const uint64_t *keys = nullptr) {

@@ -229,6 +229,7 @@ struct morph::engine {
     DevBuf d_tune;
     // render snapshot (made at synchronize)
     DevBuf d_snap, d_rank, d_brgba, d_lag, d_slope, d_rws, d_out;
+    DevBuf d_fkeys;  // the render snapshot's key points atom-major, for the fluid update
     std::vector<int32_t> rank;                    // [h][nchains]
     std::vector<uint32_t> brgba;                  // [h][bw]
     bool have_snap = false;
@@ -430,6 +431,9 @@ bool morph::engine::prepare_fluid() {
     if (amh_fluid_ranks(d_snap.as<uint64_t>(), h, wtot, d_off.as<int64_t>(), d_w.as<int32_t>(), int32_t(nch),
                         mix_seed(), d_frank.as<int32_t>(), rstream))
         return fail("fluid ranks");
+    if (!d_fkeys.ensure(8 * size_t(wtot) * h) ||
+        amh_fluid_atom_keys(d_snap.as<uint64_t>(), h, wtot, d_fkeys.as<uint64_t>(), rstream))
+        return fail("fluid atom keys");
     if (amh_memcpy_h2d(d_fsurf.p, surf.data(), 4 * surf.size(), rstream) ||
         amh_memcpy_h2d(d_fbcol.p, bcol.data(), 8 * bcol.size(), rstream) || amh_stream_sync(rstream))
         return fail("fluid inputs");
@@ -449,6 +453,7 @@ bool morph::engine::render_fluid(double t, uint32_t *out_dev) {
     fa.blob_col_dev = opt.show_blobs == TEXTURE ? nullptr : d_fbcol.as<double>();
     fa.sparseness = fluid_sp;
     fa.fluidsteps = int32_t(opt.fluidsteps);
+    fa.atom_keys_dev = d_fkeys.as<uint64_t>();
     if (amh_fluid_render(fluid, &fa, t, out_dev, rstream)) return fail("fluid render");
     return true;
 }

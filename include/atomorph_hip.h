@@ -356,7 +356,14 @@ typedef struct amh_fluid_frame {
     double sparseness;            /* grid cells per pixel (reference: 1)     */
     int32_t fluidsteps;           /* FluidModel::step calls per frame        */
     int32_t pad_;
+    const uint64_t *atom_keys_dev; /* [wtot][nkeys]: r->pts_dev atom-major
+                                     (amh_fluid_atom_keys), one record per
+                                     atom for the update; NULL: read r->pts_dev */
 } amh_fluid_frame;
+/* The key points atom-major: out_dev[a * nkeys + y] = pts_dev[y * wtot + a]
+ * (no counterpart in the reference; the fluid update reads an atom's key
+ * points in one record instead of one row per key). */
+int amh_fluid_atom_keys(const uint64_t *pts_dev, int32_t nkeys, int64_t wtot, uint64_t *out_dev, void *stream);
 /* One frame at global time t: particle update, fluidsteps model steps,
  * splat, resolve, outward feather, background; out_dev[width*height].  On a
  * stream made by amh_stream_create the frame's launches (about 5 per model

@@ -150,8 +150,9 @@ def test_fluid_steps_after_new_particles_match_oracle(grid):
             assert d <= 2e-3, (k, col, d)
 
 
-def _fluid_frame(g, variant, t, fluidsteps, sp=1.0, reset=True, fluid=None, rend=None):
-    """Render one fluid frame of the cfg1 chains through the C ABI."""
+def _fluid_frame(g, variant, t, fluidsteps, sp=1.0, reset=True, fluid=None, rend=None, atom_keys=None):
+    """Render one fluid frame of the cfg1 chains through the C ABI (atom_keys:
+    a device copy of the key points atom-major for the update, or None)."""
     lib = G.lib()
     cfg, keys, off, widths, order, nb, blob_rgba, (lag, slope) = render_inputs(g, variant)
     if rend is None:
@@ -176,7 +177,8 @@ def _fluid_frame(g, variant, t, fluidsteps, sp=1.0, reset=True, fluid=None, rend
             rank[y, o_ + order_x] = np.arange(w)
             surf[y, c] = int((f1 > 0).sum())
     keep = [G.to_dev(rank.ravel()), G.to_dev(surf.ravel())]
-    fa = A.FluidFrame(C.pointer(rend.args), keep[0].data_ptr(), keep[1].data_ptr(), None, sp, fluidsteps, 0)
+    fa = A.FluidFrame(C.pointer(rend.args), keep[0].data_ptr(), keep[1].data_ptr(), None, sp, fluidsteps, 0,
+                      None if atom_keys is None else atom_keys.data_ptr())
     if fluid is None:
         fluid = C.c_void_p()
         G.check(lib.amh_fluid_create(C.byref(fluid), int(64 * sp) + 20, int(64 * sp) + 20, wtot, 64, 64, h,
@@ -292,6 +294,48 @@ def test_fluid_perlin_lag_slope_kept_between_frames(golden_cfg1):
         made.append(fd)
         assert same_targets(particles(fa), particles(fc))
         assert not same_targets(particles(fd), particles(fc))  # the tables matter at t = 0.35
+    finally:
+        for f in made:
+            lib.amh_fluid_destroy(f)
+
+
+@pytest.mark.parametrize("variant", ["default", "spline_perlin_bg_feather2"])
+def test_fluid_update_from_atom_major_keys(golden_cfg1, variant):
+    """amh_fluid_atom_keys transposes the key points atom-major, and frames
+    whose update reads an atom's key points from that copy equal frames
+    that read them row by row: the same active particles with the same
+    colour targets, and (as between any two runs: tiles split over several
+    work items add their node sums with FP32 atomics, in no fixed order)
+    images within 1 LSB and positions and velocities within 1e-4."""
+    g = golden_cfg1
+    lib = G.lib()
+    T = G.torch()
+    h, n = 2, int(np.sum(g["chain_w"]))
+    rend = _renderer(g, variant)
+    keys = G.zeros(n * h, T.int64)
+    G.check(lib.amh_fluid_atom_keys(G.dptr(rend.keep[1]), h, n, G.dptr(keys), G.stream()), "atom keys")
+    want = np.ascontiguousarray(G.key_major(g["chain_after"], h)).view(np.uint64).reshape(h, n).T.ravel()
+    assert np.array_equal(G.to_host(keys, np.uint64), want)
+    made = []
+    try:
+        fa = fb = None
+        for t in (0.0, 0.3, 0.55, 0.8):
+            G.log(f"atom-major keys: {variant} t={t}")
+            ia, fa, _, _, _ = _fluid_frame(g, variant, t, 2, fluid=fa, rend=rend)
+            if len(made) == 0:
+                made.append(fa)
+            ib, fb, _, _, _ = _fluid_frame(g, variant, t, 2, fluid=fb, rend=rend, atom_keys=keys)
+            if len(made) == 1:
+                made.append(fb)
+            assert np.abs(ia.astype(np.int32) - ib.astype(np.int32)).max() <= 1, t
+            QA, QB = np.zeros((n, 20)), np.zeros((n, 20))
+            G.check(lib.amh_fluid_get_particles(fa, op.ptr(QA), G.stream()), "get")
+            G.check(lib.amh_fluid_get_particles(fb, op.ptr(QB), G.stream()), "get")
+            assert np.array_equal(QA[:, 16], QB[:, 16]), t
+            act = QA[:, 16] > 0
+            assert act.sum() > 0
+            assert np.array_equal(QA[act, 7:11], QB[act, 7:11]), t
+            assert np.abs(QA[:, :4] - QB[:, :4]).max() <= 1e-4, t
     finally:
         for f in made:
             lib.amh_fluid_destroy(f)
