@@ -1646,13 +1646,12 @@ extern "C" int amh_fluid_get_particles(void *fluid, double *host, void *stream) 
 }
 
 namespace {
-// One FluidModel::step (fluidmodel.cpp:165-580) over the particles in Q
-// (particle order, or the cell-ordered copy with its atom map).
 // Counting sort of the particles' entries into the bins of B (tiles bins,
 // counts of parity par; the other parity is cleared for the next use).
-// counted: the counts and ranks are in place already.
-// Whole bins (the splat's pixel tiles).  Beyond kLdsTiles tiles the counts
-// are scanned first by one workgroup.
+// counted: the counts and ranks are in place already.  Whole bins (the
+// splat's pixel tiles).  Beyond kLdsTiles tiles the counts are scanned
+// first by one workgroup.  tail(i) runs for every particle i after its
+// placing (k_bin_place).
 template <class Tiles, class Tail = NoTail>
 void bin_particles(const Tiles &tiles_of, int n, const Bins &B, int tiles, int par, hipStream_t s,
                    bool counted = false, const Tail &tail = Tail()) {
@@ -1687,7 +1686,8 @@ void place_items(const StencilTiles &tiles_of, int n, const Bins &B, int tiles, 
     }
 }
 
-// One model step on the particles Q.  counted: the previous step's
+// One FluidModel::step (fluidmodel.cpp:165-580) on the particles Q, atom[i]
+// the atom in slot i (the wall jitter is keyed by atom).  counted: the previous step's
 // advection counted this step's bins; count_next: this step's advection
 // counts the next step's (when another step follows with unchanged
 // particles).
